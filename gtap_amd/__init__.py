@@ -1,0 +1,233 @@
+"""MI355X-native GTaP runtime: Python host-side mirror of the GTaP host API.
+
+The product is ``_lib/libgtap_amd.so`` (HIP for gfx950, built by ``gtap_amd.build``);
+this module only binds its C ABI (``include/gtap.h``) with ctypes. Names follow the
+reference host API -- ``gtap_initialize``, ``gtap_finalize``, ``gtap_reset``,
+``gtap_synchronize``, ``gtap_get_runtime_error_code``, ``gtap_visualize_profile``
+(runtime/detail/thread/gtap_runtime_thread.cuh:338-441,
+runtime/detail/common/gtap_runtime_error.cuh:38-249) -- and return the same int status.
+The task-program wrappers (``fib`` ...) raise :class:`GtapError` on a nonzero status.
+
+There is no CPU fallback: if the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libgtap_amd.so")
+
+# gtap_error_code (gtap_amd/csrc/gtap_types.h)
+ERROR_NAMES = {
+    0: "GTAP_ERROR_NONE",
+    1: "GTAP_ERROR_INVALID_QUEUE_IDX",
+    2: "GTAP_ERROR_INVALID_QUEUE_IDX_AFTER_JOIN",
+    3: "GTAP_ERROR_QUEUE_OVERFLOW",
+    4: "GTAP_ERROR_RESULT_HANDLE_OVERFLOW",
+    5: "GTAP_ERROR_TASK_ID_POOL_SLOT_BUSY",
+    6: "GTAP_ERROR_TASK_ID_POOL_LOW_HEADROOM",
+    7: "GTAP_ERROR_GENERATED_TASK_ID_BUFFER_OVERFLOW",
+    8: "GTAP_ERROR_INVALID_TASKWAIT",
+    9: "GTAP_ERROR_WATCHDOG",
+    10: "GTAP_ERROR_INVALID_FUNCTION",
+    11: "GTAP_ERROR_INVALID_ARGUMENT",
+    12: "GTAP_ERROR_NOT_INITIALIZED",
+    13: "GTAP_ERROR_HIP",
+}
+
+TERM_JOBS, TERM_QUIESCE, TERM_SERVE = 0, 1, 2
+
+
+class ErrorReport(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("valid", "code", "src_line", "block_idx", "thread_idx", "tid", "queue_idx", "value", "limit")]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [
+        ("kernel_ms", ctypes.c_double),
+        ("tasks_executed", ctypes.c_uint64),
+        ("tasks_spawned", ctypes.c_uint64),
+        ("rounds", ctypes.c_uint64),
+        ("pops", ctypes.c_uint64),
+        ("steals", ctypes.c_uint64),
+        ("steal_attempts", ctypes.c_uint64),
+        ("jobs_taken", ctypes.c_uint64),
+        ("pool_refills", ctypes.c_uint64),
+        ("workers", ctypes.c_uint32),
+        ("blocks", ctypes.c_uint32),
+        ("error_code", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("pool_bytes", ctypes.c_uint64),
+        ("record_stride", ctypes.c_uint32),
+        ("deque_capacity", ctypes.c_uint32),
+        ("num_queues", ctypes.c_uint32),
+        ("blocks", ctypes.c_uint32),
+        ("steal_batch", ctypes.c_uint32),
+        ("probe_lanes", ctypes.c_uint32),
+        ("max_jobs", ctypes.c_uint32),
+        ("timeout_ms", ctypes.c_uint32),
+        ("block_workers", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 6),
+    ]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [
+        ("func", ctypes.c_uint32),
+        ("queue", ctypes.c_uint32),
+        ("status", ctypes.c_uint32),
+        ("kind", ctypes.c_uint32),
+        ("result", ctypes.c_int64),
+        ("t_submit", ctypes.c_uint64),
+        ("t_start", ctypes.c_uint64),
+        ("t_end", ctypes.c_uint64),
+        ("t_done_host", ctypes.c_uint64),
+        ("reserved", ctypes.c_uint64),
+        ("args", ctypes.c_uint8 * 64),
+    ]
+
+
+class GtapError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        code = ctypes.c_int(0)
+        rep = ErrorReport()
+        try:
+            lib().gtap_get_runtime_error_code(ctypes.byref(code))
+            lib().gtap_get_runtime_error_report(ctypes.byref(rep))
+        except Exception:  # pragma: no cover - library unusable
+            pass
+        self.status = status
+        self.code = code.value
+        self.report = {f: getattr(rep, f) for f, _ in rep._fields_}
+        super().__init__(f"{where}: status {status}, GTaP error {self.code} "
+                         f"({ERROR_NAMES.get(self.code, '?')}), report {self.report}")
+
+
+# Every symbol include/gtap.h declares, with its ctypes signature.
+_i, _u32, _u64, _i64, _vp = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_void_p
+SIGNATURES: dict[str, tuple[list, object]] = {
+    "gtap_initialize": ([], _i),
+    "gtap_finalize": ([], _i),
+    "gtap_reset": ([], _i),
+    "gtap_set_config": ([ctypes.POINTER(Config)], _i),
+    "gtap_get_config": ([ctypes.POINTER(Config)], _i),
+    "gtap_synchronize": ([], _i),
+    "gtap_check_runtime_error": ([], _i),
+    "gtap_get_runtime_error_code": ([ctypes.POINTER(_i)], _i),
+    "gtap_print_runtime_error_report": ([], _i),
+    "gtap_get_runtime_error_string": ([_i], ctypes.c_char_p),
+    "gtap_get_runtime_error_report": ([ctypes.POINTER(ErrorReport)], _i),
+    "gtap_visualize_profile": ([ctypes.c_char_p], None),
+    "gtap_get_last_stats": ([ctypes.POINTER(RunStats)], _i),
+    "gtap_get_worker_stats": ([_vp, _u32, ctypes.POINTER(_u32)], _i),
+    "gtap_device_info": ([ctypes.c_char_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_u64)], _i),
+    "gtap_device_malloc": ([ctypes.POINTER(_vp), _u64], _i),
+    "gtap_device_free": ([_vp], _i),
+    "gtap_memcpy_htod": ([_vp, _vp, _u64], _i),
+    "gtap_memcpy_dtoh": ([_vp, _vp, _u64], _i),
+    "gtap_memcpy_dtod": ([_vp, _vp, _u64], _i),
+    "gtap_memset_device": ([_vp, _i, _u64], _i),
+    "gtap_fib": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libgtap_amd.so (raises if it has not been built -- there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gtap_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (argtypes, restype) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = argtypes
+            fn.restype = restype
+        _lib = L
+    return _lib
+
+
+def _check(status: int, where: str) -> None:
+    if status != 0:
+        raise GtapError(status, where)
+
+
+# ---------------------------------------------------------------- host API (reference names)
+
+def gtap_set_config(**kw) -> int:
+    c = Config()
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return lib().gtap_set_config(ctypes.byref(c))
+
+
+def gtap_get_config() -> dict:
+    c = Config()
+    _check(lib().gtap_get_config(ctypes.byref(c)), "gtap_get_config")
+    return {f: getattr(c, f) for f, _ in c._fields_ if f != "reserved"}
+
+
+def gtap_initialize() -> int:
+    return lib().gtap_initialize()
+
+
+def gtap_finalize() -> int:
+    return lib().gtap_finalize()
+
+
+def gtap_reset() -> int:
+    return lib().gtap_reset()
+
+
+def gtap_synchronize() -> int:
+    return lib().gtap_synchronize()
+
+
+def gtap_get_runtime_error_code() -> int:
+    c = ctypes.c_int(0)
+    _check(lib().gtap_get_runtime_error_code(ctypes.byref(c)), "gtap_get_runtime_error_code")
+    return c.value
+
+
+def gtap_get_runtime_error_string(code: int) -> str:
+    return lib().gtap_get_runtime_error_string(code).decode()
+
+
+def gtap_visualize_profile(app_name: str) -> None:
+    lib().gtap_visualize_profile(app_name.encode())
+
+
+def last_stats() -> dict:
+    s = RunStats()
+    _check(lib().gtap_get_last_stats(ctypes.byref(s)), "gtap_get_last_stats")
+    return s.as_dict()
+
+
+def device_info() -> dict:
+    name = ctypes.create_string_buffer(64)
+    cus = ctypes.c_int(0)
+    hbm = ctypes.c_uint64(0)
+    _check(lib().gtap_device_info(name, 64, ctypes.byref(cus), ctypes.byref(hbm)), "gtap_device_info")
+    return {"arch": name.value.decode(), "num_cus": cus.value, "hbm_bytes": hbm.value}
+
+
+# ---------------------------------------------------------------- task programs
+
+def fib(n: int) -> tuple[int, dict]:
+    """fib(n) as one persistent launch; returns (value, run statistics)."""
+    r = ctypes.c_int64(0)
+    s = RunStats()
+    _check(lib().gtap_fib(n, ctypes.byref(r), ctypes.byref(s)), f"gtap_fib({n})")
+    return r.value, s.as_dict()

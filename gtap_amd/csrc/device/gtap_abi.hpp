@@ -1,0 +1,50 @@
+// gtap_abi.hpp -- the compiler <-> runtime device ABI used by lowered task functions.
+//
+// A `#pragma gtap function` becomes a switch-based state machine over the resume state
+// (the contract of the reference's clang-gtap; runtime side in
+// runtime/detail/thread/gtap_runtime_thread.cuh:663-847). The names below keep the
+// reference's `__gtap_*` spelling; their arguments follow this runtime's design:
+// tasks are named by a function id (a `switch` dispatch instead of an indirect call,
+// which would block inlining on AMDGPU), and a spawn names the byte offset of the
+// parent's result slot instead of retaining the child frame.
+//
+//   reference                                   here
+//   __gtap_get_task_state(tid)                  __gtap_get_task_state(L)
+//   __gtap_spawn_task(ctx, tid, &cnt, f, q,     __gtap_spawn_task(L, f, q, res_off, args)
+//                     &out_tid, retain)         __gtap_spawn_task_detached(L, f, q, args)
+//   __gtap_set_state_for_join(tid, cnt, s, q)   __gtap_set_state_for_join(L, s, q)
+//   __gtap_finish_task(tid, ctx)                __gtap_finish_task(L) / __gtap_return(L, v)
+//   __gtap_get_task_data(tid)                   __gtap_get_task_data(L)
+#pragma once
+
+#include "gtap_thread.hpp"
+
+template <class L>
+__device__ __forceinline__ uint32_t __gtap_get_task_state(const L& lane) {
+    return lane.get_state();
+}
+template <class L, class A>
+__device__ __forceinline__ uint32_t __gtap_spawn_task(L& lane, uint32_t func, uint32_t queue, int res_off,
+                                                      const A& args) {
+    return lane.spawn(func, queue, res_off, args);
+}
+template <class L, class A>
+__device__ __forceinline__ uint32_t __gtap_spawn_task_detached(L& lane, uint32_t func, uint32_t queue, const A& args) {
+    return lane.spawn_detached(func, queue, args);
+}
+template <class L>
+__device__ __forceinline__ bool __gtap_set_state_for_join(L& lane, uint32_t next_state, uint32_t queue_after) {
+    return lane.taskwait(next_state, queue_after);
+}
+template <class L>
+__device__ __forceinline__ void __gtap_finish_task(L& lane) {
+    lane.finish();
+}
+template <class L, class T>
+__device__ __forceinline__ void __gtap_return(L& lane, T value) {
+    lane.ret(value);
+}
+template <class L>
+__device__ __forceinline__ uint8_t* __gtap_get_task_data(L& lane) {
+    return lane.rec + ::gtap::kHdrBytes;
+}

@@ -1,0 +1,257 @@
+// gtap_dev.hpp -- device-side runtime descriptor and CDNA4 primitives shared by the
+// thread-mode and block-mode schedulers.
+//
+// Coherence model (gfx950, 8 XCDs with one L2 each):
+//  * Every word that one worker writes and another worker reads (task records, deque
+//    slots, id-pool cells, job entries) is accessed with agent-scope relaxed atomics,
+//    which compile to `sc1` loads/stores. They are coherent across XCDs without any
+//    cache maintenance.
+//  * Ordering is "complete, then publish": a wave issues its sc1 stores, waits for them
+//    with `s_waitcnt vmcnt(0)`, and only then performs the atomic that makes them
+//    reachable (deque publish, join decrement, pool-cell release). A reader only issues
+//    its sc1 loads after the atomic that handed it the work has returned.
+//  * Agent-scope acquire/release fences are never used: each one is a whole-L2
+//    writeback or invalidate, which costs tens of microseconds when thousands of waves
+//    issue them (measured: 69 us per release fence at 8192 waves).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../gtap_types.h"
+
+namespace gtap {
+
+constexpr int kWave = 64;
+
+// Control words, each on its own 128-byte line (index * 32 ints).
+enum CtlIndex : int {
+    CTL_DONE = 0,       // set once: all waves leave the scheduler loop
+    CTL_ABORT = 1,      // set on a device error or watchdog expiry
+    CTL_IDLE = 2,       // idle-worker count (quiescence termination)
+    CTL_ERROR = 3,      // first error code (CAS from 0)
+    CTL_JOB_NEXT = 4,   // next job index to admit
+    CTL_JOBS_DONE = 5,  // completed jobs
+    CTL_LIVE_JOBS = 6,  // admitted and not yet completed
+    CTL_COUNT = 8
+};
+
+// Host-visible control (pinned host memory, read with system-scope loads).
+enum HostCtlIndex : int {
+    HCTL_JOB_AVAIL = 0,  // jobs the host has made available (serve mode)
+    HCTL_STOP = 1,       // host asks the served kernel to drain and exit
+    HCTL_COUNT = 2
+};
+
+// Per-wave statistics slots (uint64 each, 16 per wave).
+enum StatIndex : int {
+    ST_TASKS = 0, ST_SPAWNS, ST_ROUNDS, ST_POPS, ST_STEALS, ST_STEAL_ATTEMPTS, ST_JOBS, ST_REFILLS,
+    ST_COUNT = 16
+};
+
+// Everything a persistent launch needs, passed by value as a kernel argument.
+struct GtapDev {
+    uint8_t* recs;             // task records, `stride` bytes each
+    uint32_t* priv_ids;        // per-wave overflow id stacks, [worker][8192]
+    uint32_t* pool_ids;        // id rings, [shard][cell][64]
+    uint32_t* pool_seq;        // per-cell sequence numbers, [shard][cell]
+    uint64_t* pool_ctl;        // [32s] enqueue pos, [32s+16] dequeue pos, [32*nshards] fresh ids
+    uint64_t* dq_word;         // deque words, [queue][worker], one 128-byte line each
+    uint32_t* dq_slots;        // deque slots, [queue][worker][cap]
+    int32_t* ctl;              // device control words (CtlIndex * 32)
+    int32_t* host_ctl;         // host-visible control (HostCtlIndex * 32), pinned memory
+    gtap_error_report* report; // host-mapped error report
+    gtap_job* jobs;            // job table
+    uint64_t* stats;           // [worker][ST_COUNT]
+    uint64_t timeout_ticks;    // watchdog, wall-clock ticks (100 MHz)
+    uint32_t stride;           // record stride in bytes (multiple of 16)
+    uint32_t nrecs;            // task ids [0, nrecs); ids [nrecs, nrecs + nanchors) are anchors
+    uint32_t nanchors;
+    uint32_t ncells;           // cells per ring (power of two)
+    uint32_t nshards;          // id rings
+    uint32_t cap;              // deque capacity (power of two)
+    uint32_t nqueues;          // provisioned deque planes
+    uint32_t nworkers;         // waves in this launch
+    uint32_t steal_batch;      // <= 64
+    uint32_t probe_lanes;      // <= 64
+    uint32_t term_mode;        // gtap_term_mode
+    uint32_t njobs;            // jobs of this launch (JOBS / QUIESCE modes)
+    uint32_t job_ring;         // job-table entries (anchors are job index % job_ring)
+    uint32_t admit_reserve;    // free ids required per admitted job
+    uint32_t max_live_jobs;    // admission cap on concurrently running jobs
+};
+
+// Record header (16 bytes) at the start of every task record and anchor.
+//   word0: func id [0..11] | state [12..23] | queue [24..27]
+//   word1: parent id (int32, -1 = detached / none)
+//   word2: join counter (children still running)
+//   word3: byte offset of the result slot inside the parent record
+constexpr uint32_t kHdrBytes = 16;
+constexpr uint32_t kNoParent = 0xFFFFFFFFu;
+constexpr uint32_t kMaxFunc = 0xFFF;
+constexpr uint32_t kMaxState = 0xFFF;
+
+__device__ __forceinline__ uint32_t hdr_word0(uint32_t func, uint32_t state, uint32_t queue) {
+    return (func & 0xFFFu) | ((state & 0xFFFu) << 12) | ((queue & 0xFu) << 24);
+}
+__device__ __forceinline__ uint32_t hdr_func(uint32_t w0) { return w0 & 0xFFFu; }
+__device__ __forceinline__ uint32_t hdr_state(uint32_t w0) { return (w0 >> 12) & 0xFFFu; }
+__device__ __forceinline__ uint32_t hdr_queue(uint32_t w0) { return (w0 >> 24) & 0xFu; }
+
+// ------------------------------------------------------------------ coherent access
+
+__device__ __forceinline__ uint32_t ld32(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t ld32(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld64(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st32(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st32(int32_t* p, int32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st64(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t ld32_sys(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint32_t ld32_sys(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint64_t ld64_sys(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st32_sys(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st32_sys(int32_t* p, int32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st64_sys(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <typename T>
+__device__ __forceinline__ T atomic_add(T* p, T v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T atomic_sub(T* p, T v) {
+    return __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ bool atomic_cas(T* p, T& expected, T desired) {
+    return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until every memory operation this wave issued has completed. After it, the wave's
+// sc1 stores are visible device-wide, so the next atomic may publish them.
+__device__ __forceinline__ void complete_stores() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ void compiler_barrier() { __asm__ volatile("" ::: "memory"); }
+
+// Typed coherent loads/stores of 4- or 8-byte aligned fields inside records.
+template <typename T>
+__device__ __forceinline__ T ldT(const void* p) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte fields only");
+    if constexpr (sizeof(T) == 4) {
+        uint32_t v = ld32(reinterpret_cast<const uint32_t*>(p));
+        return __builtin_bit_cast(T, v);
+    } else {
+        uint64_t v = ld64(reinterpret_cast<const uint64_t*>(p));
+        return __builtin_bit_cast(T, v);
+    }
+}
+template <typename T>
+__device__ __forceinline__ void stT(void* p, T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte fields only");
+    if constexpr (sizeof(T) == 4) {
+        st32(reinterpret_cast<uint32_t*>(p), __builtin_bit_cast(uint32_t, v));
+    } else {
+        st64(reinterpret_cast<uint64_t*>(p), __builtin_bit_cast(uint64_t, v));
+    }
+}
+// Copy `nbytes` (multiple of 4, 4-byte aligned) with coherent accesses.
+__device__ __forceinline__ void copy_words(void* dst, const void* src, int nbytes) {
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+    for (int i = 0; i < nbytes / 4; ++i) st32(d + i, s[i]);
+}
+
+// ------------------------------------------------------------------ wave helpers
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+__device__ __forceinline__ uint64_t active_mask() { return __ballot(1); }
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+// number of set bits of `mask` below this lane
+__device__ __forceinline__ int rank_in(uint64_t mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+__device__ __forceinline__ int popc(uint64_t m) { return __popcll(m); }
+__device__ __forceinline__ int first_lane(uint64_t m) { return __ffsll((long long)m) - 1; }
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int32_t shfl(int32_t v, int src) { return __shfl(v, src, kWave); }
+__device__ __forceinline__ uint32_t shfl(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, kWave); }
+__device__ __forceinline__ uint64_t shfl(uint64_t v, int src) {
+    uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, kWave);
+    uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Volatile accessors for wave-private LDS state. Lanes of one wave update this state
+// inside divergent code; volatile keeps the compiler from reusing a value loaded
+// before another lane's store.
+template <typename T>
+__device__ __forceinline__ T vld(const T* p) { return *reinterpret_cast<const volatile T*>(p); }
+template <typename T>
+__device__ __forceinline__ void vst(T* p, T v) { *reinterpret_cast<volatile T*>(p) = v; }
+
+__device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
+
+__device__ __forceinline__ uint32_t xorshift32(uint32_t& s) {
+    s ^= s << 13;
+    s ^= s >> 17;
+    s ^= s << 5;
+    return s;
+}
+
+// ------------------------------------------------------------------ errors
+
+// First error wins; the report goes to host-mapped memory; every wave then leaves its
+// scheduler loop (the kernel ends normally instead of trapping).
+// Takes scalars only, so the kernel argument block is never copied to scratch.
+__device__ __noinline__ void record_error(int32_t* ctl, gtap_error_report* report, int code, int tid, int queue,
+                                          int value, int limit, int line);
+
+#define GTAP_DEVICE_ERROR(dev, code, tid, queue, value, limit)                                             \
+    ::gtap::record_error((dev).ctl, (dev).report, (code), (int)(tid), (int)(queue), (int)(value), (int)(limit), \
+                         __LINE__)
+
+__device__ __noinline__ void record_error(int32_t* ctl, gtap_error_report* report, int code, int tid, int queue,
+                                          int value, int limit, int line) {
+    int32_t expected = 0;
+    if (atomic_cas(ctl + CTL_ERROR * 32, expected, (int32_t)code)) {
+        gtap_error_report* r = report;
+        if (r != nullptr) {
+            st32_sys(&r->code, code);
+            st32_sys(&r->src_line, line);
+            st32_sys(&r->block_idx, (int32_t)blockIdx.x);
+            st32_sys(&r->thread_idx, (int32_t)threadIdx.x);
+            st32_sys(&r->tid, tid);
+            st32_sys(&r->queue_idx, queue);
+            st32_sys(&r->value, value);
+            st32_sys(&r->limit, limit);
+            complete_stores();
+            st32_sys(&r->valid, 1);
+        }
+    }
+    st32(ctl + CTL_ABORT * 32, 1);
+}
+
+}  // namespace gtap

@@ -1,0 +1,814 @@
+// gtap_thread.hpp -- thread-executed (per-lane) persistent scheduler for gfx950.
+//
+// Counterpart of the reference's thread mode (runtime/detail/thread/gtap_runtime_thread.cuh,
+// gtap_thread_core.cuh), redesigned for wave64 and for CDNA4's per-XCD L2:
+//
+//  * Worker = one wavefront. A round runs up to 64 tasks, one per lane
+//    (reference: 32 per warp, gtap_runtime_thread.cuh:849-1040).
+//  * Task records are fixed-stride slots in HBM addressed by task id; a 16-byte header
+//    (function id, resume state, EPAQ queue, parent id, join counter, result offset) is
+//    followed by the frame. A child writes its return value straight into the slot the
+//    parent named at spawn time, so nothing has to be retained after the join
+//    (reference: retained child ids + frames, gtap_runtime_thread.cuh:749-754).
+//  * Per worker and EPAQ queue there is one deque in HBM: a 64-bit word
+//    (top << 32 | bottom) and a ring of task ids. The owner publishes with one atomic
+//    add on the word and pops up to 64 of its newest tasks with one CAS; thieves take
+//    up to half (<= 64) of the oldest with one CAS. One word arbitrates every claim,
+//    so no lock is needed (reference: count + lock + head, gtap_runtime_thread.cuh:28-34,
+//    511-593).
+//  * Children are appended to the owner's ring *unpublished* while a round runs. After
+//    the round the wave keeps the newest <= 64 of the busiest queue for its next round and
+//    publishes the rest (reference: push_batch, gtap_runtime_thread.cuh:595-660).
+//  * Task ids come from a wave-private stack in LDS, refilled and spilled in chunks of 64
+//    through a bounded MPMC ring in HBM (reference: per-warp id lists,
+//    gtap_thread_core.cuh:111-146).
+//  * Joins are one agent-scope atomic decrement per finished child. The last child
+//    appends the parent continuation to its own wave's queue given by the parent's
+//    taskwait queue (reference: notify_parent, gtap_runtime_thread.cuh:714-730).
+//  * Every cross-worker access is an sc1 (agent-coherent) access; there are no cache
+//    fences (see gtap_dev.hpp).
+#pragma once
+
+#include "gtap_dev.hpp"
+
+namespace gtap {
+
+constexpr uint32_t kFreeCap = 512;          // wave-private id stack (LDS)
+constexpr uint32_t kRefillTarget = 192;     // refill the LDS stack up to this many ids
+constexpr uint32_t kSpillTarget = 320;      // spill the LDS stack down to this many ids
+constexpr uint32_t kPrivCap = 8192;         // wave-private overflow stack in HBM (ids)
+constexpr uint32_t kChunk = 64;             // ids per pool cell
+constexpr uint32_t kAnchorFunc = 0xFFFu;    // function id stored in anchor headers
+constexpr uint32_t kMargin = 1024;          // deque slots kept free beyond the tail
+
+enum Outcome : uint32_t { OUT_RUNNING = 0, OUT_FINISHED = 1, OUT_SUSPENDED = 2 };
+
+template <int Q>
+struct WaveCtx {
+    uint32_t free_top;
+    uint32_t priv_top;
+    uint32_t cur_queue;
+    uint32_t spawns;
+    uint32_t refills;
+    uint32_t tail[Q];       // owner's private tail per queue (absolute position)
+    uint32_t bottom[Q];     // published bottom per queue (low word of the deque word)
+    uint32_t top_cache[Q];  // last top observed per queue
+    uint32_t free_ids[kFreeCap];
+    uint32_t mirror[Q][kWave];  // the last 64 ids appended per queue, by position & 63
+};
+
+__device__ __forceinline__ uint8_t* rec_ptr(const GtapDev& dev, uint32_t id) {
+    return dev.recs + (size_t)id * dev.stride;
+}
+__device__ __forceinline__ uint64_t* dq_word_ptr(const GtapDev& dev, uint32_t q, uint32_t w) {
+    return dev.dq_word + ((size_t)q * dev.nworkers + w) * 16;
+}
+__device__ __forceinline__ uint32_t* dq_slot_ptr(const GtapDev& dev, uint32_t q, uint32_t w, uint32_t pos) {
+    return dev.dq_slots + ((size_t)q * dev.nworkers + w) * dev.cap + (pos & (dev.cap - 1));
+}
+
+// ------------------------------------------------------------------ id pool
+// Three levels. (1) A 512-id stack per wave in LDS. (2) An 8192-id overflow stack per
+// wave in HBM, touched only by its owner (no atomics). (3) `nshards` bounded MPMC rings of
+// 64-id cells (Vyukov's sequence numbers) that move ids between waves, plus a counter of
+// never-used ids. Every ring holds all ids, so an enqueue ticket (fetch-add) never has to
+// wait for more than a dequeuer that is still copying the cell out.
+// Ring s: pool_ctl[32s] = enqueue position, pool_ctl[32s+16] = dequeue position;
+// pool_ctl[32*nshards] = fresh-id counter. Cell c of lap L is full when seq == pos+1.
+
+__device__ __forceinline__ uint32_t* pool_enq_ptr(const GtapDev& dev, uint32_t s) {
+    return reinterpret_cast<uint32_t*>(dev.pool_ctl + (size_t)s * 32);
+}
+__device__ __forceinline__ uint32_t* pool_deq_ptr(const GtapDev& dev, uint32_t s) {
+    return reinterpret_cast<uint32_t*>(dev.pool_ctl + (size_t)s * 32 + 16);
+}
+__device__ __forceinline__ uint32_t* pool_fresh_ptr(const GtapDev& dev) {
+    return reinterpret_cast<uint32_t*>(dev.pool_ctl + (size_t)dev.nshards * 32);
+}
+__device__ __forceinline__ uint32_t* pool_seq_ptr(const GtapDev& dev, uint32_t s, uint32_t pos) {
+    return dev.pool_seq + (size_t)s * dev.ncells + (pos & (dev.ncells - 1));
+}
+__device__ __forceinline__ uint32_t* pool_cell_ptr(const GtapDev& dev, uint32_t s, uint32_t pos) {
+    return dev.pool_ids + ((size_t)s * dev.ncells + (pos & (dev.ncells - 1))) * kChunk;
+}
+
+// Leader lane only: claim a full cell of ring s (a few CAS attempts; false when the ring
+// is empty or contended).
+__device__ __forceinline__ bool ring_take_claim(const GtapDev& dev, uint32_t s, uint32_t& pos_out) {
+    uint32_t* deq = pool_deq_ptr(dev, s);
+    for (int tries = 0; tries < 4; ++tries) {
+        const uint32_t pos = ld32(deq);
+        const uint32_t seq = ld32(pool_seq_ptr(dev, s, pos));
+        const int32_t diff = (int32_t)(seq - (pos + 1));
+        if (diff == 0) {
+            uint32_t expected = pos;
+            if (atomic_cas(deq, expected, pos + 1)) {
+                pos_out = pos;
+                return true;
+            }
+        } else if (diff < 0) {
+            return false;
+        }
+    }
+    return false;
+}
+
+// Leader lane only: take an enqueue ticket of ring s and wait until its cell is free.
+__device__ __forceinline__ bool ring_put_claim(const GtapDev& dev, uint32_t s, uint32_t& pos_out) {
+    const uint32_t pos = atomic_add(pool_enq_ptr(dev, s), 1u);
+    for (uint32_t spin = 0; spin < (1u << 22); ++spin) {
+        if (ld32(pool_seq_ptr(dev, s, pos)) == pos) {
+            pos_out = pos;
+            return true;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return false;
+}
+
+// Move 64 ids into the wave's LDS stack. Called by all active lanes.
+template <int Q>
+__device__ __forceinline__ void refill_chunk(const GtapDev& dev, WaveCtx<Q>* w, uint32_t me) {
+    const uint64_t m = active_mask();
+    const uint32_t cnt = (uint32_t)popc(m);
+    const uint32_t r = (uint32_t)rank_in(m);
+    const int leader = first_lane(m);
+    const uint32_t top = uniform(vld(&w->free_top));
+    const uint32_t ptop = uniform(vld(&w->priv_top));
+    if (ptop >= kChunk) {
+        const uint32_t* src = dev.priv_ids + (size_t)me * kPrivCap + (ptop - kChunk);
+        for (uint32_t i = r; i < kChunk; i += cnt) vst(&w->free_ids[top + i], src[i]);
+        vst(&w->priv_top, ptop - kChunk);
+    } else {
+        uint32_t pos = 0, shard = 0, fresh = 0;
+        int kind = 0;  // 0 = exhausted, 1 = ring cell, 2 = fresh ids
+        if ((int)r == 0) {
+            for (uint32_t i = 0; i < dev.nshards && kind == 0; ++i) {
+                shard = (me + i) % dev.nshards;
+                if (ring_take_claim(dev, shard, pos)) kind = 1;
+            }
+            if (kind == 0) {
+                fresh = atomic_add(pool_fresh_ptr(dev), kChunk);
+                if (fresh <= dev.nrecs - kChunk) kind = 2;
+            }
+        }
+        kind = shfl((int32_t)kind, leader);
+        pos = shfl(pos, leader);
+        shard = shfl(shard, leader);
+        fresh = shfl(fresh, leader);
+        if (kind == 1) {
+            const uint32_t* ids = pool_cell_ptr(dev, shard, pos);
+            for (uint32_t i = r; i < kChunk; i += cnt) vst(&w->free_ids[top + i], ld32(ids + i));
+            complete_stores();  // the cell is read out before it is released
+            if ((int)r == 0) st32(pool_seq_ptr(dev, shard, pos), pos + dev.ncells);
+        } else if (kind == 2) {
+            for (uint32_t i = r; i < kChunk; i += cnt) vst(&w->free_ids[top + i], fresh + i);
+        } else {
+            if ((int)r == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_TASK_ID_POOL_LOW_HEADROOM, -1, -1, (int)fresh, (int)dev.nrecs);
+            // hand out the sink record (id 0) so the round can finish harmlessly
+            for (uint32_t i = r; i < kChunk; i += cnt) vst(&w->free_ids[top + i], 0u);
+        }
+        if ((int)r == 0) vst(&w->refills, vld(&w->refills) + 1u);
+    }
+    vst(&w->free_top, top + kChunk);
+}
+
+// Move the 64 most recently freed ids out of the wave's LDS stack.
+template <int Q>
+__device__ __forceinline__ void spill_chunk(const GtapDev& dev, WaveCtx<Q>* w, uint32_t me) {
+    const uint64_t m = active_mask();
+    const uint32_t cnt = (uint32_t)popc(m);
+    const uint32_t r = (uint32_t)rank_in(m);
+    const int leader = first_lane(m);
+    const uint32_t top = uniform(vld(&w->free_top));
+    const uint32_t ptop = uniform(vld(&w->priv_top));
+    if (ptop + kChunk <= kPrivCap) {
+        uint32_t* dst = dev.priv_ids + (size_t)me * kPrivCap + ptop;
+        for (uint32_t i = r; i < kChunk; i += cnt) dst[i] = vld(&w->free_ids[top - kChunk + i]);
+        vst(&w->priv_top, ptop + kChunk);
+    } else {
+        const uint32_t shard = me % dev.nshards;
+        uint32_t pos = 0;
+        int ok = 0;
+        if ((int)r == 0) ok = ring_put_claim(dev, shard, pos) ? 1 : 0;
+        ok = shfl((int32_t)ok, leader);
+        pos = shfl(pos, leader);
+        if (ok) {
+            uint32_t* ids = pool_cell_ptr(dev, shard, pos);
+            for (uint32_t i = r; i < kChunk; i += cnt) st32(ids + i, vld(&w->free_ids[top - kChunk + i]));
+            complete_stores();
+            if ((int)r == 0) st32(pool_seq_ptr(dev, shard, pos), pos + 1);
+        } else if ((int)r == 0) {
+            GTAP_DEVICE_ERROR(dev, GTAP_ERROR_TASK_ID_POOL_SLOT_BUSY, -1, (int)shard, (int)pos, (int)dev.ncells);
+        }
+    }
+    vst(&w->free_top, top - kChunk);
+}
+
+template <int Q>
+__device__ __forceinline__ uint32_t alloc_id(const GtapDev& dev, WaveCtx<Q>* w, uint32_t me) {
+    const uint64_t m = active_mask();
+    const uint32_t cnt = (uint32_t)popc(m);
+    const uint32_t r = (uint32_t)rank_in(m);
+    uint32_t top = uniform(vld(&w->free_top));
+    if (top < cnt) {
+        while (top < kRefillTarget) {
+            refill_chunk(dev, w, me);
+            top = uniform(vld(&w->free_top));
+        }
+    }
+    const uint32_t id = vld(&w->free_ids[top - 1 - r]);
+    vst(&w->free_top, top - cnt);
+    return id;
+}
+
+template <int Q>
+__device__ __forceinline__ void free_id(const GtapDev& dev, WaveCtx<Q>* w, uint32_t me, uint32_t id) {
+    const uint64_t m = active_mask();
+    const uint32_t cnt = (uint32_t)popc(m);
+    const uint32_t r = (uint32_t)rank_in(m);
+    uint32_t top = uniform(vld(&w->free_top));
+    if (top + cnt > kFreeCap) {
+        while (top > kSpillTarget) {
+            spill_chunk(dev, w, me);
+            top = uniform(vld(&w->free_top));
+        }
+    }
+    vst(&w->free_ids[top + r], id);
+    vst(&w->free_top, top + cnt);
+}
+
+// ------------------------------------------------------------------ deque append
+
+// Append `id` to the private (unpublished) end of queue `q` of this wave.
+// Called by all active lanes; lanes may target different queues.
+template <int Q>
+__device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtx<Q>* w, uint32_t me, uint32_t q, uint32_t id) {
+#pragma unroll
+    for (int qq = 0; qq < Q; ++qq) {
+        const uint64_t mq = ballot(q == (uint32_t)qq);
+        if (mq == 0) continue;
+        if (q == (uint32_t)qq) {
+            const uint32_t cnt = (uint32_t)popc(mq);
+            const uint32_t r = (uint32_t)rank_in(mq);
+            const uint32_t t = uniform(vld(&w->tail[qq]));
+            uint32_t top = uniform(vld(&w->top_cache[qq]));
+            if (t + cnt - top > dev.cap - kMargin) {
+                // refresh the cached top from the deque word before declaring overflow
+                uint64_t wv = ld64(dq_word_ptr(dev, qq, me));
+                top = uniform((uint32_t)(wv >> 32));
+                vst(&w->top_cache[qq], top);
+            }
+            if (t + cnt - top > dev.cap - kMargin) {
+                if (r == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, id, qq, t + cnt - top, dev.cap - kMargin);
+            } else {
+                const uint32_t pos = t + r;
+                st32(dq_slot_ptr(dev, qq, me, pos), id);
+                vst(&w->mirror[qq][pos & (kWave - 1)], id);
+                vst(&w->tail[qq], t + cnt);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ per-lane task context
+
+template <class P>
+struct Lane {
+    static constexpr int Q = P::kQueues;
+    const GtapDev* dev;
+    WaveCtx<Q>* w;
+    const typename P::Args* args;
+    uint8_t* rec;           // this task's record
+    uint32_t me;            // worker (wave) index
+    uint32_t tid;
+    uint32_t func;
+    uint32_t state;
+    uint32_t queue;
+    uint32_t parent;
+    uint32_t res_off;
+    uint32_t child_count;
+    uint32_t outcome;
+    uint32_t frame[P::kFrameWords];  // the first frame words, loaded when the round starts
+
+    // -- reads of the loaded frame (arguments written by the spawner)
+    template <typename T>
+    __device__ __forceinline__ T arg(int byte_off) const {
+        static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4/8-byte args");
+        if constexpr (sizeof(T) == 4) {
+            return __builtin_bit_cast(T, frame[byte_off / 4]);
+        } else {
+            uint64_t v = (uint64_t)frame[byte_off / 4] | ((uint64_t)frame[byte_off / 4 + 1] << 32);
+            return __builtin_bit_cast(T, v);
+        }
+    }
+    // -- coherent access to any part of this task's frame (offsets relative to the frame)
+    template <typename T>
+    __device__ __forceinline__ T frame_ld(int byte_off) const { return ldT<T>(rec + kHdrBytes + byte_off); }
+    template <typename T>
+    __device__ __forceinline__ void frame_st(int byte_off, T v) { stT<T>(rec + kHdrBytes + byte_off, v); }
+
+    __device__ __forceinline__ uint32_t get_state() const { return state; }
+
+    // Spawn a child. `res_off` is the byte offset (inside this task's frame) of the slot
+    // that receives the child's return value. Returns the child id.
+    template <typename A>
+    __device__ __forceinline__ uint32_t spawn(uint32_t f, uint32_t q, int res_off_in_frame, const A& a) {
+        return spawn_impl(f, q, tid, kHdrBytes + (uint32_t)res_off_in_frame, a);
+    }
+    // Spawn a child nobody joins (the parent may return without taskwait).
+    template <typename A>
+    __device__ __forceinline__ uint32_t spawn_detached(uint32_t f, uint32_t q, const A& a) {
+        return spawn_impl(f, q, kNoParent, 0u, a);
+    }
+
+    template <typename A>
+    __device__ __forceinline__ uint32_t spawn_impl(uint32_t f, uint32_t q, uint32_t par, uint32_t roff, const A& a) {
+        static_assert(sizeof(A) % 8 == 0, "argument blocks are padded to 8 bytes");
+        static_assert(sizeof(A) / 4 <= P::kFrameWords, "arguments must fit the loaded frame");
+        if (q >= (uint32_t)Q) {
+            GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_QUEUE_IDX, tid, q, q, Q);
+            q = 0;
+        }
+        const uint32_t id = alloc_id(*dev, w, me);
+        uint8_t* crec = rec_ptr(*dev, id);
+        st64(reinterpret_cast<uint64_t*>(crec), (uint64_t)hdr_word0(f, 0, q) | ((uint64_t)par << 32));
+        st64(reinterpret_cast<uint64_t*>(crec + 8), ((uint64_t)roff << 32));
+        const uint64_t* src = reinterpret_cast<const uint64_t*>(&a);
+        uint64_t* dst = reinterpret_cast<uint64_t*>(crec + kHdrBytes);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(A) / 8); ++i) st64(dst + i, src[i]);
+        queue_append(*dev, w, me, q, id);
+        if (par != kNoParent) child_count += 1;
+        vst(&w->spawns, vld(&w->spawns) + 1u);
+        return id;
+    }
+
+    // taskwait: returns true when the task must return now (it has children to wait for);
+    // the task resumes at `next_state`, re-enqueued on queue `q_after`.
+    // (reference: __gtap_set_state_for_join, gtap_runtime_thread.cuh:672-696)
+    __device__ __forceinline__ bool taskwait(uint32_t next_state, uint32_t q_after) {
+        if (q_after >= (uint32_t)Q) {
+            GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_QUEUE_IDX_AFTER_JOIN, tid, q_after, q_after, Q);
+            q_after = 0;
+        }
+        if (child_count == 0) {
+            state = next_state;
+            return false;
+        }
+        st32(reinterpret_cast<uint32_t*>(rec), hdr_word0(func, next_state, q_after));
+        st32(reinterpret_cast<uint32_t*>(rec + 8), child_count);
+        child_count = 0;
+        outcome = OUT_SUSPENDED;
+        return true;
+    }
+
+    // return value: written into the parent's result slot
+    template <typename T>
+    __device__ __forceinline__ void ret(T v) {
+        if (parent != kNoParent) stT<T>(rec_ptr(*dev, parent) + res_off, v);
+        finish();
+    }
+    __device__ __forceinline__ void finish() {
+        if (child_count != 0) {
+            // joined children spawned after the last taskwait would notify a finished
+            // parent; such children must be spawned detached
+            GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_TASKWAIT, tid, -1, child_count, 0);
+        }
+        outcome = OUT_FINISHED;
+    }
+};
+
+// ------------------------------------------------------------------ scheduler
+
+struct LoopStats {
+    uint64_t tasks, rounds, pops, steals, attempts, jobs;
+};
+
+template <class P>
+__device__ __forceinline__ void load_task(Lane<P>& L, uint32_t id) {
+    const GtapDev& dev = *L.dev;
+    L.tid = id;
+    L.rec = rec_ptr(dev, id);
+    const uint64_t h0 = ld64(reinterpret_cast<const uint64_t*>(L.rec));
+    const uint64_t h1 = ld64(reinterpret_cast<const uint64_t*>(L.rec + 8));
+    const uint32_t w0 = (uint32_t)h0;
+    L.func = hdr_func(w0);
+    L.state = hdr_state(w0);
+    L.queue = hdr_queue(w0);
+    L.parent = (uint32_t)(h0 >> 32);
+    L.res_off = (uint32_t)(h1 >> 32);
+    L.child_count = 0;
+    L.outcome = OUT_RUNNING;
+    const uint64_t* fp = reinterpret_cast<const uint64_t*>(L.rec + kHdrBytes);
+#pragma unroll
+    for (int i = 0; i < P::kFrameWords / 2; ++i) {
+        const uint64_t v = ld64(fp + i);
+        L.frame[2 * i] = (uint32_t)v;
+        L.frame[2 * i + 1] = (uint32_t)(v >> 32);
+    }
+}
+
+// Admit up to 64 jobs from the job table as root tasks. Returns the number admitted; lane
+// i < count receives the root id in `tid_out`.
+template <class P>
+__device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtx<P::kQueues>* w, uint32_t me,
+                                              uint32_t& tid_out, bool& idle_registered) {
+    const int lane = lane_id();
+    uint32_t avail = dev.njobs;
+    if (dev.term_mode == GTAP_TERM_SERVE) avail = (uint32_t)ld32_sys(dev.host_ctl + HCTL_JOB_AVAIL * 32);
+    int32_t* nextp = dev.ctl + CTL_JOB_NEXT * 32;
+    uint32_t next = uniform((uint32_t)ld32(nextp));
+    if (next >= avail) return 0;
+    uint32_t k = avail - next;
+    if (k > kWave) k = kWave;
+    // admission: cap on live jobs and free task ids
+    const uint32_t live = uniform((uint32_t)ld32(dev.ctl + CTL_LIVE_JOBS * 32));
+    if (live >= dev.max_live_jobs) return 0;
+    if (k > dev.max_live_jobs - live) k = dev.max_live_jobs - live;
+    if (dev.admit_reserve > 0) {
+        // ids never handed out plus ids parked in the shared rings
+        const uint32_t fresh = uniform(ld32(pool_fresh_ptr(dev)));
+        uint64_t free_ids = (uint64_t)(dev.nrecs > fresh ? dev.nrecs - fresh : 0);
+        for (uint32_t sh = 0; sh < dev.nshards; ++sh) {
+            const uint32_t enq = uniform(ld32(pool_enq_ptr(dev, sh)));
+            const uint32_t deq = uniform(ld32(pool_deq_ptr(dev, sh)));
+            free_ids += (uint64_t)(enq - deq) * kChunk;
+        }
+        uint64_t fit = free_ids / dev.admit_reserve;
+        if (fit == 0) return 0;
+        if (k > fit) k = (uint32_t)fit;
+    }
+    if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
+        if (lane == 0) atomic_add(dev.ctl + CTL_IDLE * 32, -1);
+        idle_registered = false;
+    }
+    int ok = 0;
+    if (lane == 0) {
+        int32_t expected = (int32_t)next;
+        ok = atomic_cas(nextp, expected, (int32_t)(next + k)) ? 1 : 0;
+        if (ok) atomic_add(dev.ctl + CTL_LIVE_JOBS * 32, (int32_t)k);
+    }
+    ok = shfl((int32_t)ok, 0);
+    if (!ok) return 0;
+    if ((uint32_t)lane < k) {
+        const uint32_t j = next + (uint32_t)lane;
+        const uint32_t slot = j % dev.job_ring;
+        gtap_job* job = dev.jobs + slot;
+        const uint32_t func = ld32_sys(&job->func);
+        const uint32_t q = ld32_sys(&job->queue);
+        const uint32_t anchor = dev.nrecs + slot;
+        const uint32_t id = alloc_id(dev, w, me);
+        uint8_t* arec = rec_ptr(dev, anchor);
+        st64(reinterpret_cast<uint64_t*>(arec), (uint64_t)hdr_word0(kAnchorFunc, 0, 0) | ((uint64_t)kNoParent << 32));
+        st64(reinterpret_cast<uint64_t*>(arec + 8), 1ull);  // join = 1 (the root), res_off = 0
+        st64(reinterpret_cast<uint64_t*>(arec + kHdrBytes), 0ull);
+        uint8_t* rrec = rec_ptr(dev, id);
+        st64(reinterpret_cast<uint64_t*>(rrec), (uint64_t)hdr_word0(func, 0, q < (uint32_t)P::kQueues ? q : 0) |
+                                                    ((uint64_t)anchor << 32));
+        st64(reinterpret_cast<uint64_t*>(rrec + 8), ((uint64_t)kHdrBytes << 32));
+        const uint64_t* ja = reinterpret_cast<const uint64_t*>(job->args);
+        uint64_t* ra = reinterpret_cast<uint64_t*>(rrec + kHdrBytes);
+        constexpr int kArgWords = (P::kFrameWords / 2) < 8 ? (P::kFrameWords / 2) : 8;
+#pragma unroll
+        for (int i = 0; i < kArgWords; ++i) st64(ra + i, ld64_sys(ja + i));
+        st64_sys(&job->t_start, now_ticks());
+        st32_sys(&job->status, (uint32_t)GTAP_JOB_RUNNING);
+        tid_out = id;
+    }
+    complete_stores();
+    return k;
+}
+
+// A job's root finished: publish its result to the job table.
+template <class P>
+__device__ __forceinline__ void complete_job(const GtapDev& dev, uint32_t anchor) {
+    const uint32_t slot = anchor - dev.nrecs;
+    gtap_job* job = dev.jobs + slot;
+    const uint64_t res = ld64(reinterpret_cast<const uint64_t*>(rec_ptr(dev, anchor) + kHdrBytes));
+    st64_sys(reinterpret_cast<uint64_t*>(&job->result), res);
+    st64_sys(&job->t_end, now_ticks());
+    complete_stores();
+    st32_sys(&job->status, (uint32_t)GTAP_JOB_DONE);
+    atomic_add(dev.ctl + CTL_LIVE_JOBS * 32, -1);
+    const int32_t done = atomic_add(dev.ctl + CTL_JOBS_DONE * 32, 1) + 1;
+    if (dev.term_mode == GTAP_TERM_JOBS && (uint32_t)done == dev.njobs) st32(dev.ctl + CTL_DONE * 32, 1);
+}
+
+// Try to steal from random victims on queue q. Returns the number of ids stolen into `tid_out`.
+template <class P>
+__device__ __forceinline__ uint32_t try_steal(const GtapDev& dev, uint32_t me, uint32_t q, uint32_t& rng,
+                                              uint32_t& tid_out, bool& idle_registered, LoopStats& st) {
+    const int lane = lane_id();
+    if (dev.nworkers < 2) return 0;
+    uint32_t victim = 0;
+    uint64_t wv = 0;
+    int32_t sz = 0;
+    if ((uint32_t)lane < dev.probe_lanes) {
+        victim = xorshift32(rng) % (dev.nworkers - 1);
+        if (victim >= me) victim += 1;
+        wv = ld64(dq_word_ptr(dev, q, victim));
+        sz = (int32_t)((uint32_t)wv - (uint32_t)(wv >> 32));
+    }
+    uint64_t cand = ballot(sz > 0);
+    if (cand == 0) return 0;
+    // rotate the candidate mask by a random amount so thieves spread over victims
+    const uint32_t rot = uniform(xorshift32(rng)) & 63u;
+    const uint64_t rotated = rot ? ((cand >> rot) | (cand << (64 - rot))) : cand;
+    const int src = (first_lane(rotated) + (int)rot) & 63;
+    victim = shfl(victim, src);
+    wv = shfl(wv, src);
+    sz = shfl(sz, src);
+    uint32_t k = (uint32_t)(sz + 1) / 2;
+    if (k > dev.steal_batch) k = dev.steal_batch;
+    if (k == 0) k = 1;
+    if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
+        if (lane == 0) atomic_add(dev.ctl + CTL_IDLE * 32, -1);
+        idle_registered = false;
+    }
+    int ok = 0;
+    if (lane == 0) {
+        uint64_t expected = wv;
+        ok = atomic_cas(dq_word_ptr(dev, q, victim), expected, wv + ((uint64_t)k << 32)) ? 1 : 0;
+    }
+    ok = shfl((int32_t)ok, 0);
+    st.attempts += 1;
+    if (!ok) return 0;
+    const uint32_t t = (uint32_t)(wv >> 32);
+    if ((uint32_t)lane < k) tid_out = ld32(dq_slot_ptr(dev, q, victim, t + (uint32_t)lane));
+    st.steals += 1;
+    return k;
+}
+
+// Pop up to 64 of this wave's newest published tasks from queue q.
+template <class P>
+__device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtx<P::kQueues>* w, uint32_t me, uint32_t q,
+                                            uint32_t& tid_out) {
+    const int lane = lane_id();
+    uint64_t* wp = dq_word_ptr(dev, q, me);
+    for (int tries = 0; tries < 8; ++tries) {
+        uint64_t wv = 0;
+        if (lane == 0) wv = ld64(wp);
+        wv = shfl(wv, 0);
+        const uint32_t t = (uint32_t)(wv >> 32), b = (uint32_t)wv;
+        const int32_t sz = (int32_t)(b - t);
+        if (sz <= 0) return 0;
+        const uint32_t k = sz < kWave ? (uint32_t)sz : (uint32_t)kWave;
+        int ok = 0;
+        if (lane == 0) {
+            uint64_t expected = wv;
+            ok = atomic_cas(wp, expected, ((uint64_t)t << 32) | (uint64_t)(b - k)) ? 1 : 0;
+        }
+        ok = shfl((int32_t)ok, 0);
+        if (ok) {
+            if ((uint32_t)lane < k) tid_out = ld32(dq_slot_ptr(dev, q, me, b - k + (uint32_t)lane));
+            vst(&w->bottom[q], b - k);
+            vst(&w->tail[q], b - k);
+            vst(&w->top_cache[q], t);
+            return k;
+        }
+    }
+    return 0;
+}
+
+__device__ __forceinline__ void backoff_sleep(uint32_t level) {
+    switch (level) {
+        case 0: break;
+        case 1: __builtin_amdgcn_s_sleep(1); break;
+        case 2: __builtin_amdgcn_s_sleep(2); break;
+        case 3: __builtin_amdgcn_s_sleep(4); break;
+        case 4: __builtin_amdgcn_s_sleep(8); break;
+        case 5: __builtin_amdgcn_s_sleep(16); break;
+        default: __builtin_amdgcn_s_sleep(32); break;
+    }
+}
+
+template <class P>
+__device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& args, WaveCtx<P::kQueues>* w) {
+    constexpr int Q = P::kQueues;
+    const int lane = lane_id();
+    const uint32_t me = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint64_t t_begin = now_ticks();
+
+    if (lane == 0) {
+        vst(&w->free_top, 0u);
+        vst(&w->priv_top, 0u);
+        vst(&w->cur_queue, 0u);
+        vst(&w->spawns, 0u);
+        vst(&w->refills, 0u);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            vst(&w->tail[q], 0u);
+            vst(&w->bottom[q], 0u);
+            vst(&w->top_cache[q], 0u);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    Lane<P> L;
+    L.dev = &dev;
+    L.w = w;
+    L.args = &args;
+    L.me = me;
+
+    uint32_t rng = 0x9E3779B9u ^ (me * 0x85EBCA6Bu) ^ ((uint32_t)lane * 0xC2B2AE35u) ^ (uint32_t)t_begin;
+    if (rng == 0) rng = 1;
+    uint32_t n = 0;        // tasks in the current batch (wave-uniform)
+    uint32_t my_tid = 0;   // lane's task when lane < n
+    bool idle_registered = false;
+    uint32_t idle_iter = 0;
+    uint32_t busy_iter = 0;
+    LoopStats st = {0, 0, 0, 0, 0, 0};
+
+    while (true) {
+        if (n == 0) {
+            // ---- acquire work: own deque, then the job table, then steal
+            const uint32_t cq = uniform(vld(&w->cur_queue));
+#pragma unroll
+            for (int i = 0; i < Q && n == 0; ++i) {
+                const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
+                n = try_pop<P>(dev, w, me, q, my_tid);
+                if (n) {
+                    vst(&w->cur_queue, q);
+                    st.pops += 1;
+                }
+            }
+            if (n == 0 && dev.jobs != nullptr) {
+                n = admit_jobs<P>(dev, w, me, my_tid, idle_registered);
+                st.jobs += n;
+            }
+            if (n == 0) {
+#pragma unroll
+                for (int i = 0; i < Q && n == 0; ++i) {
+                    const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
+                    n = try_steal<P>(dev, me, q, rng, my_tid, idle_registered, st);
+                }
+            }
+            if (n == 0) {
+                // ---- idle: termination, abort, watchdog, backoff
+                if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered) {
+                    idle_registered = true;
+                    int32_t c = 0;
+                    if (lane == 0) c = atomic_add(dev.ctl + CTL_IDLE * 32, 1) + 1;
+                    c = shfl(c, 0);
+                    if ((uint32_t)c == dev.nworkers) {
+                        const uint32_t next = (uint32_t)ld32(dev.ctl + CTL_JOB_NEXT * 32);
+                        if (next >= dev.njobs && lane == 0) st32(dev.ctl + CTL_DONE * 32, 1);
+                    }
+                }
+                idle_iter += 1;
+                int32_t flags = 0;
+                if (lane == 0) {
+                    flags = ld32(dev.ctl + CTL_DONE * 32) | (ld32(dev.ctl + CTL_ABORT * 32) << 1);
+                    if (dev.term_mode == GTAP_TERM_SERVE && (idle_iter & 15u) == 0) {
+                        const int32_t stop = ld32_sys(dev.host_ctl + HCTL_STOP * 32);
+                        if (stop) {
+                            const uint32_t avail = (uint32_t)ld32_sys(dev.host_ctl + HCTL_JOB_AVAIL * 32);
+                            const uint32_t next = (uint32_t)ld32(dev.ctl + CTL_JOB_NEXT * 32);
+                            const int32_t live = ld32(dev.ctl + CTL_LIVE_JOBS * 32);
+                            if (next >= avail && live == 0) {
+                                st32(dev.ctl + CTL_DONE * 32, 1);
+                                flags |= 1;
+                            }
+                        }
+                    }
+                    if ((idle_iter & 63u) == 0 && now_ticks() - t_begin > dev.timeout_ticks) {
+                        GTAP_DEVICE_ERROR(dev, GTAP_ERROR_WATCHDOG, -1, -1, (int)((now_ticks() - t_begin) / 100000u),
+                                          (int)(dev.timeout_ticks / 100000u));
+                        flags |= 2;
+                    }
+                }
+                flags = shfl(flags, 0);
+                if (flags) break;
+                backoff_sleep(idle_iter < 8 ? 0 : (idle_iter < 64 ? 2 : (idle_iter < 256 ? 4 : 6)));
+                continue;
+            }
+            idle_iter = 0;
+        }
+
+        // ---- execute one round: lane i runs task my_tid when i < n
+        st.rounds += 1;
+        st.tasks += n;
+        uint32_t fin_tid = 0, fin_parent = kNoParent;
+        bool fin = false;
+        if ((uint32_t)lane < n) {
+            if (my_tid == 0u || my_tid >= dev.nrecs) {
+                // a slot never holds the sink or an anchor id; treat as a protocol error
+                GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, my_tid, -1, my_tid, dev.nrecs);
+                my_tid = 0u;
+            }
+            load_task<P>(L, my_tid);
+            if (my_tid == 0u) {
+                L.outcome = OUT_FINISHED;
+                L.parent = kNoParent;
+            } else if (L.func == 0 || L.func > P::kMaxFunc) {
+                GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, L.tid, L.queue, L.func, P::kMaxFunc);
+                L.outcome = OUT_FINISHED;
+            } else {
+                P::run(L);
+            }
+            if (L.outcome == OUT_RUNNING) {
+                GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, L.tid, L.queue, L.state, 0);
+                L.outcome = OUT_FINISHED;
+            }
+            fin = (L.outcome == OUT_FINISHED);
+            fin_tid = L.tid;
+            fin_parent = L.parent;
+            if (fin_parent != kNoParent && fin_parent >= dev.nrecs + dev.nanchors) {
+                GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, fin_tid, -1, fin_parent, dev.nrecs + dev.nanchors);
+                fin_parent = kNoParent;
+            }
+        }
+
+        // ---- epilogue: joins, continuations, id release, next batch, publish
+        complete_stores();  // results, child headers/arguments and slots of this round
+        uint32_t ready = kNoParent;
+        if (fin && fin_parent != kNoParent) {
+            const uint32_t old = atomic_sub(reinterpret_cast<uint32_t*>(rec_ptr(dev, fin_parent) + 8), 1u);
+            if (old == 1u) ready = fin_parent;
+            else if (old == 0u) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, fin_tid, -1, 0, 0);
+        }
+        if (ballot(ready != kNoParent)) {
+            if (ready != kNoParent && ready >= dev.nrecs) complete_job<P>(dev, ready);
+            if (ready != kNoParent && ready < dev.nrecs) {
+                const uint32_t w0 = ld32(reinterpret_cast<const uint32_t*>(rec_ptr(dev, ready)));
+                queue_append(dev, w, me, hdr_queue(w0) < (uint32_t)Q ? hdr_queue(w0) : 0u, ready);
+            }
+        }
+        if (fin && fin_tid != 0u) free_id(dev, w, me, fin_tid);  // id 0 is the sink record
+
+        // choose the busiest queue, keep its newest <= 64 tasks, publish the rest
+        uint32_t kmax = 0, cmax = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const uint32_t c = uniform(vld(&w->tail[q]) - vld(&w->bottom[q]));
+            if (c > cmax) {
+                cmax = c;
+                kmax = (uint32_t)q;
+            }
+        }
+        const uint32_t keep = cmax < (uint32_t)kWave ? cmax : (uint32_t)kWave;
+        uint32_t next_tid = 0;
+        if (keep) {
+            const uint32_t t = uniform(vld(&w->tail[kmax]));
+            if ((uint32_t)lane < keep) next_tid = vld(&w->mirror[kmax][(t - keep + (uint32_t)lane) & (kWave - 1)]);
+            vst(&w->tail[kmax], t - keep);
+            vst(&w->cur_queue, kmax);
+        }
+        bool waited = false;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const uint32_t t = uniform(vld(&w->tail[q]));
+            const uint32_t b = uniform(vld(&w->bottom[q]));
+            if (t != b) {
+                if (!waited) {
+                    complete_stores();  // slots of continuations appended above
+                    waited = true;
+                }
+                uint64_t old = 0;
+                if (lane == 0) old = atomic_add(dq_word_ptr(dev, q, me), (uint64_t)(t - b));
+                old = shfl(old, 0);
+                vst(&w->bottom[q], t);
+                vst(&w->top_cache[q], (uint32_t)(old >> 32));
+            }
+        }
+        n = keep;
+        my_tid = next_tid;
+
+        busy_iter += 1;
+        if ((busy_iter & 63u) == 0) {
+            int32_t a = 0;
+            if (lane == 0) {
+                a = ld32(dev.ctl + CTL_ABORT * 32);
+                if (!a && (busy_iter & 1023u) == 0 && now_ticks() - t_begin > dev.timeout_ticks) {
+                    GTAP_DEVICE_ERROR(dev, GTAP_ERROR_WATCHDOG, -1, -1, (int)((now_ticks() - t_begin) / 100000u),
+                                      (int)(dev.timeout_ticks / 100000u));
+                    a = 1;
+                }
+            }
+            a = shfl(a, 0);
+            if (a) break;
+        }
+    }
+
+    if (lane == 0) {
+        uint64_t* s = dev.stats + (size_t)me * ST_COUNT;
+        s[ST_TASKS] = st.tasks;
+        s[ST_SPAWNS] = vld(&w->spawns);
+        s[ST_ROUNDS] = st.rounds;
+        s[ST_POPS] = st.pops;
+        s[ST_STEALS] = st.steals;
+        s[ST_STEAL_ATTEMPTS] = st.attempts;
+        s[ST_JOBS] = st.jobs;
+        s[ST_REFILLS] = vld(&w->refills);
+    }
+}
+
+// Launch shape: 256-thread workgroups (4 waves), one WaveCtx per wave in LDS.
+template <class P>
+__global__ __launch_bounds__(256) void thread_kernel(GtapDev dev, typename P::Args args) {
+    __shared__ WaveCtx<P::kQueues> ctxs[4];
+    thread_scheduler<P>(dev, args, &ctxs[threadIdx.x / kWave]);
+}
+
+}  // namespace gtap

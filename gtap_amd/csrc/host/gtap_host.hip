@@ -1,0 +1,444 @@
+// gtap_host.hip -- host runtime behind the C ABI (include/gtap.h).
+//
+// Reference host API: gtap_initialize / gtap_finalize / gtap_reset
+// (runtime/detail/thread/gtap_runtime_thread.cuh:338-441), gtap_synchronize and the error
+// report helpers (runtime/detail/common/gtap_runtime_error.cuh:38-249). Same names,
+// int (hipError_t-compatible) status returns, no exceptions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "../gtap_types.h"
+#include "gtap_launch.hpp"
+
+namespace gtap {
+namespace host {
+
+static Runtime g_runtime;
+Runtime& rt() { return g_runtime; }
+
+static const char* const kErrorShort[GTAP_ERROR_COUNT] = {
+    "No error",
+    "Invalid queue index",
+    "Invalid queue index after join",
+    "Queue overflow",
+    "Result handle table overflow",
+    "Task ID pool exhausted",
+    "Task ID pool exhausted",
+    "Generated task ID buffer overflow",
+    "Invalid taskwait lowering",
+    "Watchdog expired",
+    "Invalid task function",
+    "Invalid argument",
+    "Runtime not initialized",
+    "HIP error",
+};
+
+int fail(int code, const char* msg) {
+    rt().last_error_code = code;
+    fprintf(stderr, "GTaP error: %s (%s)\n", msg, (code >= 0 && code < GTAP_ERROR_COUNT) ? kErrorShort[code] : "?");
+    return (int)hipErrorLaunchFailure;
+}
+
+int invalid_argument(const char* msg) {
+    rt().last_error_code = GTAP_ERROR_INVALID_ARGUMENT;
+    fprintf(stderr, "GTaP: invalid argument: %s\n", msg);
+    return (int)hipErrorInvalidValue;
+}
+
+int hip_status(hipError_t e, const char* what) {
+    fprintf(stderr, "GTaP HIP error: %s in %s\n", hipGetErrorString(e), what);
+    return (int)e;
+}
+
+gtap_job make_job(uint32_t func, uint32_t queue, const void* args, size_t nbytes) {
+    gtap_job j;
+    memset(&j, 0, sizeof(j));
+    j.func = func;
+    j.queue = queue;
+    j.status = GTAP_JOB_READY;
+    if (nbytes > sizeof(j.args)) nbytes = sizeof(j.args);
+    if (args && nbytes) memcpy(j.args, args, nbytes);
+    return j;
+}
+
+static uint32_t next_pow2(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return (uint32_t)p;
+}
+
+GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job* jobs, uint32_t job_ring) {
+    Runtime& R = rt();
+    GtapDev d;
+    memset(&d, 0, sizeof(d));
+    d.recs = R.recs;
+    d.priv_ids = R.priv_ids;
+    d.nshards = R.nshards;
+    d.pool_ids = R.pool_ids;
+    d.pool_seq = R.pool_seq;
+    d.pool_ctl = R.pool_ctl;
+    d.dq_word = R.dq_word;
+    d.dq_slots = R.dq_slots;
+    d.ctl = R.ctl;
+    d.host_ctl = R.host_ctl_dev;
+    d.report = R.report_dev;
+    d.jobs = jobs;
+    d.stats = R.stats;
+    d.timeout_ticks = (uint64_t)R.cfg.timeout_ms * 100000ull;  // 100 MHz wall clock
+    d.stride = R.stride;
+    d.nrecs = R.nrecs;
+    d.nanchors = R.nanchors;
+    d.ncells = R.ncells;
+    d.cap = R.cap;
+    d.nqueues = R.nqueues;
+    d.nworkers = nworkers;
+    d.steal_batch = R.cfg.steal_batch;
+    d.probe_lanes = R.cfg.probe_lanes;
+    d.term_mode = term_mode;
+    d.njobs = njobs;
+    d.job_ring = job_ring;
+    d.admit_reserve = 0;
+    d.max_live_jobs = R.nanchors;
+    return d;
+}
+
+// Clears the per-launch state: control words, deque words, id-pool ring, statistics.
+__global__ void reset_kernel(GtapDev dev) {
+    const uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t gs = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = gid; i < (uint64_t)CTL_COUNT * 32; i += gs) dev.ctl[i] = 0;
+    for (uint64_t i = gid; i < (uint64_t)dev.nqueues * dev.nworkers; i += gs) dev.dq_word[i * 16] = 0;
+    for (uint64_t i = gid; i < (uint64_t)dev.nshards * dev.ncells; i += gs)
+        dev.pool_seq[i] = (uint32_t)(i & (dev.ncells - 1));
+    for (uint64_t i = gid; i < (uint64_t)dev.nworkers * ST_COUNT; i += gs) dev.stats[i] = 0;
+    // ring positions start at 0; the fresh-id counter starts at 64 (ids 0..63 are
+    // reserved, id 0 is the sink record)
+    for (uint64_t i = gid; i < (uint64_t)dev.nshards * 32 + 32; i += gs)
+        dev.pool_ctl[i] = (i == (uint64_t)dev.nshards * 32) ? 64u : 0u;
+}
+
+int reset_launch_state(const GtapDev& dev, hipStream_t stream) {
+    hipLaunchKernelGGL(reset_kernel, dim3(1024), dim3(256), 0, stream, dev);
+    GTAP_HIP_TRY(hipGetLastError());
+    Runtime& R = rt();
+    if (R.report) memset(R.report, 0, sizeof(gtap_error_report));
+    return 0;
+}
+
+int occupancy_blocks(const void* kernel, uint32_t requested, uint32_t* blocks_out) {
+    Runtime& R = rt();
+    int per_cu = 0;
+    GTAP_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+    if (per_cu < 1) per_cu = 1;
+    uint32_t blocks = (uint32_t)per_cu * (uint32_t)R.num_cus;
+    if (requested && requested < blocks) blocks = requested;
+    *blocks_out = blocks;
+    return 0;
+}
+
+int finish_launch(const GtapDev& dev, uint32_t blocks, gtap_run_stats* stats) {
+    Runtime& R = rt();
+    float ms = 0.f;
+    GTAP_HIP_TRY(hipEventElapsedTime(&ms, R.ev0, R.ev1));
+    int32_t err = 0;
+    GTAP_HIP_TRY(hipMemcpy(&err, R.ctl + CTL_ERROR * 32, sizeof(err), hipMemcpyDeviceToHost));
+    const size_t nst = (size_t)dev.nworkers * ST_COUNT;
+    R.stats_host.resize(nst);
+    GTAP_HIP_TRY(hipMemcpy(R.stats_host.data(), R.stats, nst * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    gtap_run_stats s;
+    memset(&s, 0, sizeof(s));
+    s.kernel_ms = ms;
+    for (uint32_t w = 0; w < dev.nworkers; ++w) {
+        const uint64_t* p = &R.stats_host[(size_t)w * ST_COUNT];
+        s.tasks_executed += p[ST_TASKS];
+        s.tasks_spawned += p[ST_SPAWNS];
+        s.rounds += p[ST_ROUNDS];
+        s.pops += p[ST_POPS];
+        s.steals += p[ST_STEALS];
+        s.steal_attempts += p[ST_STEAL_ATTEMPTS];
+        s.jobs_taken += p[ST_JOBS];
+        s.pool_refills += p[ST_REFILLS];
+    }
+    s.workers = dev.nworkers;
+    s.blocks = blocks;
+    s.error_code = err;
+    R.last = s;
+    R.last_error_code = err;
+    if (stats) *stats = s;
+    if (err != 0) {
+        if (R.report && R.report->valid) {
+            fprintf(stderr, "GTaP Runtime Error at block %d, thread %d: %s (code=%d, tid=%d, queue=%d, value=%d, "
+                            "limit=%d) (source_line: %d)\n",
+                    R.report->block_idx, R.report->thread_idx,
+                    (err > 0 && err < GTAP_ERROR_COUNT) ? kErrorShort[err] : "?", R.report->code, R.report->tid,
+                    R.report->queue_idx, R.report->value, R.report->limit, R.report->src_line);
+        }
+        return (int)hipErrorLaunchFailure;
+    }
+    return 0;
+}
+
+static void free_all() {
+    Runtime& R = rt();
+    auto dfree = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    dfree(R.recs);
+    dfree(R.priv_ids);
+    dfree(R.pool_ids);
+    dfree(R.pool_seq);
+    dfree(R.pool_ctl);
+    dfree(R.dq_word);
+    dfree(R.dq_slots);
+    dfree(R.ctl);
+    dfree(R.jobs);
+    dfree(R.stats);
+    if (R.host_ctl) (void)hipHostFree(R.host_ctl);
+    if (R.report) (void)hipHostFree(R.report);
+    if (R.jobs_host) (void)hipHostFree(R.jobs_host);
+    if (R.ev0) (void)hipEventDestroy(R.ev0);
+    if (R.ev1) (void)hipEventDestroy(R.ev1);
+    if (R.stream) (void)hipStreamDestroy(R.stream);
+    gtap_config keep = R.cfg;
+    R = Runtime();
+    R.cfg = keep;
+}
+
+// Largest frame over the programs compiled into this library, header included
+// (the counterpart of the compiler-defined __gtap_auto_task_data_size).
+uint32_t builtin_record_stride() { return kBuiltinStride; }
+
+}  // namespace host
+}  // namespace gtap
+
+using namespace gtap::host;
+using namespace gtap;
+
+static void apply_defaults(gtap_config* c) {
+    if (!c->pool_bytes) c->pool_bytes = 4ull << 30;
+    if (!c->record_stride) c->record_stride = builtin_record_stride();
+    if (!c->deque_capacity) c->deque_capacity = 16384;
+    if (!c->num_queues) c->num_queues = 4;
+    if (!c->steal_batch || c->steal_batch > 64) c->steal_batch = 64;
+    if (!c->probe_lanes || c->probe_lanes > 64) c->probe_lanes = 8;
+    if (!c->max_jobs) c->max_jobs = 65536;
+    if (!c->timeout_ms) c->timeout_ms = 20000;
+}
+
+extern "C" {
+
+int gtap_set_config(const gtap_config* cfg) {
+    Runtime& R = rt();
+    if (R.initialized) return invalid_argument("gtap_set_config must precede gtap_initialize");
+    if (cfg) R.cfg = *cfg;
+    else memset(&R.cfg, 0, sizeof(R.cfg));
+    return 0;
+}
+
+int gtap_get_config(gtap_config* out) {
+    if (!out) return invalid_argument("null config");
+    gtap_config c = rt().cfg;
+    apply_defaults(&c);
+    *out = c;
+    return 0;
+}
+
+int gtap_initialize(void) {
+    Runtime& R = rt();
+    if (R.initialized) return 0;
+    apply_defaults(&R.cfg);
+    gtap_config& c = R.cfg;
+    if (c.record_stride % 16 != 0 || c.record_stride < 32) return invalid_argument("record_stride must be a multiple of 16 >= 32");
+    if ((c.deque_capacity & (c.deque_capacity - 1)) != 0 || c.deque_capacity < 4096)
+        return invalid_argument("deque_capacity must be a power of two >= 4096");
+    if (c.num_queues < 1 || c.num_queues > 8) return invalid_argument("num_queues must be in [1, 8]");
+    GTAP_HIP_TRY(hipGetDevice(&R.device));
+    hipDeviceProp_t prop;
+    GTAP_HIP_TRY(hipGetDeviceProperties(&prop, R.device));
+    R.num_cus = prop.multiProcessorCount;
+    GTAP_HIP_TRY(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
+    GTAP_HIP_TRY(hipEventCreate(&R.ev0));
+    GTAP_HIP_TRY(hipEventCreate(&R.ev1));
+
+    R.stride = c.record_stride;
+    R.nanchors = c.max_jobs;
+    uint64_t n = c.pool_bytes / R.stride;
+    n = (n / 64) * 64;
+    if (n < 4096) n = 4096;
+    if (n + R.nanchors > 0x7FFFFFFFull) n = ((0x7FFFFFFFull - R.nanchors) / 64) * 64;
+    R.nrecs = (uint32_t)n;
+    R.recs_bytes = (uint64_t)(R.nrecs + R.nanchors) * R.stride;
+    GTAP_HIP_TRY(hipMalloc(&R.recs, R.recs_bytes));
+    GTAP_HIP_TRY(hipMemset(R.recs, 0, R.recs_bytes));
+
+    R.max_workers = (uint32_t)R.num_cus * 32;  // 8 workgroups x 4 waves per CU at most
+    R.nshards = 8;
+    R.ncells = next_pow2(R.nrecs / 64 + 1);  // every ring can hold every id
+    GTAP_HIP_TRY(hipMalloc(&R.priv_ids, (size_t)R.max_workers * kPrivCap * sizeof(uint32_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.pool_ids, (size_t)R.nshards * R.ncells * 64 * sizeof(uint32_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.pool_seq, (size_t)R.nshards * R.ncells * sizeof(uint32_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.pool_ctl, ((size_t)R.nshards * 32 + 64) * sizeof(uint64_t)));
+    GTAP_HIP_TRY(hipMemset(R.pool_ctl, 0, ((size_t)R.nshards * 32 + 64) * sizeof(uint64_t)));
+
+    R.cap = c.deque_capacity;
+    R.nqueues = c.num_queues;
+    GTAP_HIP_TRY(hipMalloc(&R.dq_word, (size_t)R.nqueues * R.max_workers * 16 * sizeof(uint64_t)));
+    GTAP_HIP_TRY(hipMemset(R.dq_word, 0, (size_t)R.nqueues * R.max_workers * 16 * sizeof(uint64_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.dq_slots, (size_t)R.nqueues * R.max_workers * R.cap * sizeof(uint32_t)));
+
+    GTAP_HIP_TRY(hipMalloc(&R.ctl, (size_t)CTL_COUNT * 32 * sizeof(int32_t)));
+    GTAP_HIP_TRY(hipMemset(R.ctl, 0, (size_t)CTL_COUNT * 32 * sizeof(int32_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.jobs, (size_t)R.nanchors * sizeof(gtap_job)));
+    GTAP_HIP_TRY(hipMemset(R.jobs, 0, (size_t)R.nanchors * sizeof(gtap_job)));
+    GTAP_HIP_TRY(hipMalloc(&R.stats, (size_t)R.max_workers * ST_COUNT * sizeof(uint64_t)));
+
+    GTAP_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&R.host_ctl), HCTL_COUNT * 32 * sizeof(int32_t),
+                               hipHostMallocMapped | hipHostMallocCoherent));
+    memset(R.host_ctl, 0, HCTL_COUNT * 32 * sizeof(int32_t));
+    GTAP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&R.host_ctl_dev), R.host_ctl, 0));
+    GTAP_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&R.report), sizeof(gtap_error_report),
+                               hipHostMallocMapped | hipHostMallocCoherent));
+    memset(R.report, 0, sizeof(gtap_error_report));
+    GTAP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&R.report_dev), R.report, 0));
+    GTAP_HIP_TRY(hipDeviceSynchronize());
+    R.initialized = true;
+    return 0;
+}
+
+int gtap_finalize(void) {
+    Runtime& R = rt();
+    if (!R.initialized) return 0;
+    (void)hipDeviceSynchronize();
+    free_all();
+    return 0;
+}
+
+// Clears the runtime state without reallocating (every launch also does this itself).
+int gtap_reset(void) {
+    Runtime& R = rt();
+    if (!R.initialized) return fail(GTAP_ERROR_NOT_INITIALIZED, "gtap_reset before gtap_initialize");
+    GtapDev d = make_dev(R.max_workers, GTAP_TERM_JOBS, 0, R.jobs, 1);
+    int st = reset_launch_state(d, R.stream);
+    if (st) return st;
+    GTAP_HIP_TRY(hipStreamSynchronize(R.stream));
+    R.last_error_code = 0;
+    return 0;
+}
+
+int gtap_get_runtime_error_code(int* code) {
+    Runtime& R = rt();
+    if (!code) return invalid_argument("null pointer");
+    if (!R.initialized) {
+        *code = 0;
+        return 0;
+    }
+    int32_t e = 0;
+    GTAP_HIP_TRY(hipMemcpy(&e, R.ctl + CTL_ERROR * 32, sizeof(e), hipMemcpyDeviceToHost));
+    *code = e ? e : R.last_error_code;
+    return 0;
+}
+
+const char* gtap_get_runtime_error_string(int code) {
+    if (code >= 0 && code < GTAP_ERROR_COUNT) return kErrorShort[code];
+    return "Unknown error";
+}
+
+int gtap_print_runtime_error_report(void) {
+    Runtime& R = rt();
+    if (!R.report || !R.report->valid) return 0;
+    const gtap_error_report* r = R.report;
+    printf("GTaP Runtime Error at block %d, thread %d: %s (code=%d, tid=%d, queue=%d, value=%d, limit=%d) "
+           "(source_line: %d)\n",
+           r->block_idx, r->thread_idx, gtap_get_runtime_error_string(r->code), r->code, r->tid, r->queue_idx,
+           r->value, r->limit, r->src_line);
+    return 1;
+}
+
+int gtap_get_runtime_error_report(gtap_error_report* out) {
+    Runtime& R = rt();
+    if (!out) return invalid_argument("null pointer");
+    if (R.report) *out = *R.report;
+    else memset(out, 0, sizeof(*out));
+    return 0;
+}
+
+int gtap_check_runtime_error(void) {
+    if (gtap_print_runtime_error_report()) return 0;
+    int code = 0;
+    int st = gtap_get_runtime_error_code(&code);
+    if (st) return st;
+    if (code != GTAP_ERROR_NONE) printf("GTaP Runtime Error: %s (code: %d)\n", gtap_get_runtime_error_string(code), code);
+    return 0;
+}
+
+int gtap_synchronize(void) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        if (!gtap_print_runtime_error_report()) printf("HIP ERROR: %s\n", hipGetErrorString(e));
+        return (int)e;
+    }
+    return gtap_check_runtime_error();
+}
+
+int gtap_get_last_stats(gtap_run_stats* out) {
+    if (!out) return invalid_argument("null pointer");
+    *out = rt().last;
+    return 0;
+}
+
+int gtap_get_worker_stats(uint64_t* out, uint32_t max_workers, uint32_t* nworkers_out) {
+    Runtime& R = rt();
+    const uint32_t nw = (uint32_t)(R.stats_host.size() / ST_COUNT);
+    if (nworkers_out) *nworkers_out = nw;
+    if (out) {
+        const uint32_t n = std::min(nw, max_workers);
+        memcpy(out, R.stats_host.data(), (size_t)n * ST_COUNT * sizeof(uint64_t));
+    }
+    return 0;
+}
+
+int gtap_device_info(char* name, int name_len, int* num_cus, uint64_t* hbm_bytes) {
+    int dev = 0;
+    GTAP_HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    GTAP_HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (name && name_len > 0) {
+        snprintf(name, (size_t)name_len, "%s", prop.gcnArchName);
+    }
+    if (num_cus) *num_cus = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = (uint64_t)prop.totalGlobalMem;
+    return 0;
+}
+
+// ---- device memory helpers for callers that hold only host pointers (ctypes, C)
+int gtap_device_malloc(void** ptr, uint64_t bytes) {
+    if (!ptr) return invalid_argument("null pointer");
+    GTAP_HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
+    return 0;
+}
+int gtap_device_free(void* ptr) {
+    if (ptr) GTAP_HIP_TRY(hipFree(ptr));
+    return 0;
+}
+int gtap_memcpy_htod(void* dst, const void* src, uint64_t bytes) {
+    if (bytes) GTAP_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+int gtap_memcpy_dtoh(void* dst, const void* src, uint64_t bytes) {
+    if (bytes) GTAP_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+int gtap_memcpy_dtod(void* dst, const void* src, uint64_t bytes) {
+    if (bytes) GTAP_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return 0;
+}
+int gtap_memset_device(void* dst, int value, uint64_t bytes) {
+    if (bytes) GTAP_HIP_TRY(hipMemset(dst, value, bytes));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+// gtap_launch.hpp -- host side of one persistent launch (templated on the task program).
+//
+// The host runtime owns every pool (task records, id pool, deques, control words, job
+// table); a launch resets the per-launch state with one kernel, records hipEvents around
+// the persistent kernel and reads back the statistics and any device error.
+// Reference host flow: gtap_initialize / launch / synchronize
+// (runtime/detail/thread/gtap_runtime_thread.cuh:50-280, examples/fib/fib.cu:26-57).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../device/gtap_thread.hpp"
+#include "../gtap_types.h"
+
+extern "C" int gtap_initialize(void);
+
+namespace gtap {
+namespace host {
+
+// Record stride of the built-in programs: the largest frame plus the 16-byte header
+// (n-queens: 16 args + 8 + 16 int64 result slots = 152 bytes of frame).
+constexpr uint32_t kBuiltinStride = 192;
+
+struct Runtime {
+    bool initialized = false;
+    int device = 0;
+    gtap_config cfg{};
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // task records and anchors
+    uint8_t* recs = nullptr;
+    uint64_t recs_bytes = 0;
+    uint32_t stride = 0;
+    uint32_t nrecs = 0;
+    uint32_t nanchors = 0;
+    // id pool
+    uint32_t* priv_ids = nullptr;
+    uint32_t nshards = 0;
+    uint32_t* pool_ids = nullptr;
+    uint32_t* pool_seq = nullptr;
+    uint64_t* pool_ctl = nullptr;
+    uint32_t ncells = 0;
+    // deques
+    uint64_t* dq_word = nullptr;
+    uint32_t* dq_slots = nullptr;
+    uint32_t cap = 0;
+    uint32_t nqueues = 0;
+    uint32_t max_workers = 0;
+    // control, errors, jobs, statistics
+    int32_t* ctl = nullptr;
+    int32_t* host_ctl = nullptr;       // pinned, device-mapped
+    int32_t* host_ctl_dev = nullptr;
+    gtap_error_report* report = nullptr;      // pinned, device-mapped
+    gtap_error_report* report_dev = nullptr;
+    gtap_job* jobs = nullptr;          // device job table
+    gtap_job* jobs_host = nullptr;     // pinned job table (served mode)
+    gtap_job* jobs_host_dev = nullptr;
+    uint64_t* stats = nullptr;
+    std::vector<uint64_t> stats_host;
+    gtap_run_stats last{};
+    int last_error_code = 0;
+};
+
+Runtime& rt();
+int fail(int code, const char* msg);
+int invalid_argument(const char* msg);
+int hip_status(hipError_t e, const char* what);
+gtap_job make_job(uint32_t func, uint32_t queue, const void* args, size_t nbytes);
+GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job* jobs, uint32_t job_ring);
+int reset_launch_state(const GtapDev& dev, hipStream_t stream);
+int finish_launch(const GtapDev& dev, uint32_t blocks, gtap_run_stats* stats);
+int occupancy_blocks(const void* kernel, uint32_t requested, uint32_t* blocks_out);
+
+#define GTAP_HIP_TRY(call)                                              \
+    do {                                                                \
+        hipError_t _e = (call);                                         \
+        if (_e != hipSuccess) return ::gtap::host::hip_status(_e, #call); \
+    } while (0)
+
+// Persistent launch of a thread-mode program over `njobs` jobs of the job table `jobs`.
+template <class P>
+int launch_thread(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs,
+                  uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream = nullptr) {
+    Runtime& R = rt();
+    if (!R.initialized) {
+        int st = gtap_initialize();
+        if (st) return st;
+    }
+    if ((uint32_t)P::kQueues > R.nqueues) return invalid_argument("program needs more EPAQ queues than provisioned");
+    if (P::kFrameBytes + kHdrBytes > R.stride) return invalid_argument("task frame exceeds the record stride");
+    if (job_ring == 0 || job_ring > R.nanchors) return invalid_argument("job ring exceeds the anchor capacity");
+    if (!stream) stream = R.stream;
+    static uint32_t max_blocks = 0;
+    const void* kfn = reinterpret_cast<const void*>(&thread_kernel<P>);
+    if (max_blocks == 0) {
+        int st = occupancy_blocks(kfn, 0, &max_blocks);
+        if (st) return st;
+    }
+    uint32_t blocks = max_blocks;
+    if (R.cfg.blocks && R.cfg.blocks < blocks) blocks = R.cfg.blocks;
+    if (blocks * 4 > R.max_workers) blocks = R.max_workers / 4;
+    GtapDev dev = make_dev(blocks * 4, term_mode, njobs, jobs, job_ring);
+    int st = reset_launch_state(dev, stream);
+    if (st) return st;
+    GTAP_HIP_TRY(hipEventRecord(R.ev0, stream));
+    hipLaunchKernelGGL(thread_kernel<P>, dim3(blocks), dim3(256), 0, stream, dev, args);
+    GTAP_HIP_TRY(hipGetLastError());
+    GTAP_HIP_TRY(hipEventRecord(R.ev1, stream));
+    GTAP_HIP_TRY(hipStreamSynchronize(stream));
+    return finish_launch(dev, blocks, stats);
+}
+
+// One job in slot 0 of the device job table; the root's return value goes to *result.
+template <class P>
+int run_single_job(const typename P::Args& args, const gtap_job& job, int64_t* result, gtap_run_stats* stats,
+                   uint32_t term_mode) {
+    Runtime& R = rt();
+    if (!R.initialized) {
+        int st = gtap_initialize();
+        if (st) return st;
+    }
+    GTAP_HIP_TRY(hipMemcpyAsync(R.jobs, &job, sizeof(gtap_job), hipMemcpyHostToDevice, R.stream));
+    int st = launch_thread<P>(args, term_mode, 1, R.jobs, 1, stats);
+    if (st) return st;
+    gtap_job out;
+    GTAP_HIP_TRY(hipMemcpy(&out, R.jobs, sizeof(gtap_job), hipMemcpyDeviceToHost));
+    if (term_mode == GTAP_TERM_JOBS && out.status != GTAP_JOB_DONE) return fail(GTAP_ERROR_INVALID_TASKWAIT, "job did not complete");
+    if (result) *result = out.result;
+    return 0;
+}
+
+}  // namespace host
+}  // namespace gtap

@@ -1,0 +1,92 @@
+// prog_fib.hip -- fib as a thread-executed task program.
+//
+// Source program (the reference's evaluation/2-comparison/fib/gtap_fib.cu:10-22, with an
+// int64 result):
+//
+//   #pragma gtap function
+//   __device__ int64_t fib(int n) {
+//       if (n < 2) return n;
+//       int64_t a, b;
+//       #pragma gtap task
+//       a = fib(n - 1);
+//       #pragma gtap task
+//       b = fib(n - 2);
+//       #pragma gtap taskwait
+//       return a + b;
+//   }
+//
+// Lowered form below: frame = { int32 n; int32 pad; int64 a; int64 b; }, states 0 and 1.
+#include "../device/gtap_abi.hpp"
+#include "../host/gtap_launch.hpp"
+
+namespace gtap {
+namespace fib {
+
+enum : uint32_t { F_FIB = 1 };
+struct NArg {
+    int32_t n;
+    int32_t pad;
+};
+constexpr int kOffA = 8;
+constexpr int kOffB = 16;
+
+struct Program {
+    static constexpr int kQueues = 1;
+    static constexpr int kFrameWords = 2;  // n (+pad)
+    static constexpr uint32_t kMaxFunc = F_FIB;
+    static constexpr uint32_t kFrameBytes = 24;
+    struct Args {
+        int32_t unused;
+    };
+    template <class L>
+    __device__ static void run(L& lane);
+};
+
+template <class L>
+__device__ __forceinline__ void fib_task(L& lane) {
+    switch (__gtap_get_task_state(lane)) {
+        case 0: {
+            const int32_t n = lane.template arg<int32_t>(0);
+            if (n < 2) {
+                __gtap_return(lane, (int64_t)n);
+                return;
+            }
+            __gtap_spawn_task(lane, F_FIB, 0, kOffA, NArg{n - 1, 0});
+            __gtap_spawn_task(lane, F_FIB, 0, kOffB, NArg{n - 2, 0});
+            if (__gtap_set_state_for_join(lane, 1, 0)) return;
+        }
+            [[fallthrough]];
+        case 1: {
+            const int64_t a = lane.template frame_ld<int64_t>(kOffA);
+            const int64_t b = lane.template frame_ld<int64_t>(kOffB);
+            __gtap_return(lane, a + b);
+            return;
+        }
+        default:
+            GTAP_DEVICE_ERROR(*lane.dev, GTAP_ERROR_INVALID_TASKWAIT, lane.tid, 0, lane.state, 1);
+            lane.finish();
+            return;
+    }
+}
+
+template <class L>
+__device__ void Program::run(L& lane) {
+    fib_task(lane);
+}
+
+}  // namespace fib
+}  // namespace gtap
+
+using FibProgram = gtap::fib::Program;
+static_assert(FibProgram::kFrameBytes + gtap::kHdrBytes <= gtap::host::kBuiltinStride, "frame exceeds stride");
+
+// Runs fib(n) as one persistent launch (one job). Reference host flow:
+// examples/fib/fib.cu:26-57.
+extern "C" int gtap_fib(int n, int64_t* result, gtap_run_stats* stats) {
+    if (n < 0 || n > 92) return gtap::host::invalid_argument("gtap_fib: n must be in [0, 92]");
+    gtap_job job = gtap::host::make_job(gtap::fib::F_FIB, 0, &n, sizeof(n));
+    int64_t r = 0;
+    int st = gtap::host::run_single_job<FibProgram>(FibProgram::Args{0}, job, &r, stats, GTAP_TERM_JOBS);
+    if (st == 0 && result) *result = r;
+    return st;
+}

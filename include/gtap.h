@@ -1,0 +1,88 @@
+/*
+ * gtap.h -- C ABI of the MI355X-native GTaP runtime (libgtap_amd.so).
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and returns an
+ * int status that is 0 on success and a hipError_t value otherwise (the reference's
+ * functions return cudaError_t). A device-side runtime error does not trap: the
+ * persistent kernel records the error, every worker leaves its loop, and the call
+ * returns hipErrorLaunchFailure; gtap_get_runtime_error_code() and
+ * gtap_print_runtime_error_report() then give the GTaP error code and report.
+ *
+ * Citations name the reference interface each entry point replaces.
+ */
+#ifndef GTAP_H
+#define GTAP_H
+
+#include <stdint.h>
+
+#include "../gtap_amd/csrc/gtap_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ lifecycle */
+
+/* Replaces gtap_initialize (runtime/detail/thread/gtap_runtime_thread.cuh:338-340,
+ * runtime/detail/block/gtap_runtime_block.cuh:155-157): allocates every pool. */
+int gtap_initialize(void);
+/* Replaces gtap_finalize (gtap_runtime_thread.cuh:342-344). */
+int gtap_finalize(void);
+/* Replaces gtap_reset (gtap_runtime_thread.cuh:439-441): clears state, keeps allocations. */
+int gtap_reset(void);
+/* Runtime knobs that the reference fixes at compile time with GTAP_* macros
+ * (runtime/detail/common/gtap_config.cuh:6-30; examples/README.md:97-105).
+ * Must precede gtap_initialize. Zero fields take defaults. */
+int gtap_set_config(const gtap_config* cfg);
+int gtap_get_config(gtap_config* out);
+
+/* ------------------------------------------------------------ errors */
+
+/* Replaces gtap_synchronize (runtime/detail/common/gtap_runtime_error.cuh:246-249). */
+int gtap_synchronize(void);
+/* Replaces gtap_check_runtime_error (gtap_runtime_error.cuh:219-234). */
+int gtap_check_runtime_error(void);
+/* Replaces gtap_get_runtime_error_code (gtap_runtime_error.cuh:38-40). */
+int gtap_get_runtime_error_code(int* error_code);
+/* Replaces gtap_print_runtime_error_report (gtap_runtime_error.cuh:206-217); returns 1 when
+ * a report was printed. */
+int gtap_print_runtime_error_report(void);
+/* Replaces gtap_get_runtime_error_string (gtap_runtime_error.cuh:187-192). */
+const char* gtap_get_runtime_error_string(int error_code);
+/* The report itself (the reference keeps it in a static host pointer,
+ * gtap_runtime_error.cuh:36). */
+int gtap_get_runtime_error_report(gtap_error_report* out);
+
+/* ------------------------------------------------------------ profiling / statistics */
+
+/* Replaces gtap_visualize_profile (runtime/detail/thread/gtap_profile_thread.cuh:355-358):
+ * writes ./profile/<app>_warp_statistics_working.csv for the last launch. */
+void gtap_visualize_profile(const char* app_name);
+/* Statistics of the last persistent launch (no reference counterpart; the reference
+ * prints "Execution time" around its launch, examples/fib/fib.cu:36-51). */
+int gtap_get_last_stats(gtap_run_stats* out);
+/* Per-wave counters of the last launch: 16 uint64 per worker. */
+int gtap_get_worker_stats(uint64_t* out, uint32_t max_workers, uint32_t* nworkers_out);
+int gtap_device_info(char* name, int name_len, int* num_cus, uint64_t* hbm_bytes);
+
+/* ------------------------------------------------------------ device memory helpers */
+/* For callers that only hold host pointers (ctypes, plain C). */
+int gtap_device_malloc(void** ptr, uint64_t bytes);
+int gtap_device_free(void* ptr);
+int gtap_memcpy_htod(void* dst, const void* src, uint64_t bytes);
+int gtap_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+int gtap_memcpy_dtod(void* dst, const void* src, uint64_t bytes);
+int gtap_memset_device(void* dst, int value, uint64_t bytes);
+
+/* ------------------------------------------------------------ task programs */
+/* Each call is one persistent launch of the program's lowered task functions, with the
+ * root task taken from the job table (the `#pragma gtap entry` of the reference). */
+
+/* fib(n) with an int64 result; replaces the program of
+ * evaluation/2-comparison/fib/gtap_fib.cu:10-27 + main (examples/fib/fib.cu:26-57). */
+int gtap_fib(int n, int64_t* result, gtap_run_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
