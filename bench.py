@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X-native GTaP runtime -- prints ONE JSON line on stdout.
+
+Workload (the config the headline metric is quoted on): n-queens N=14, thread-executed,
+one task per legal placement, taskwait + int64 sum, no cutoff -- 27,358,553 tasks per run.
+A step is one persistent launch (gtap_nqueens(14)) with everything resident on the GPU.
+`value` = tasks x steps / wall time of the timed steps (host clock around the steps,
+each step ends with a stream synchronize). The roofline uses the average kernel duration
+from hipEvents recorded on the launch stream.
+
+Also reported (outside `value`): fib(30) tasks/s, the measured HBM-copy and atomic peaks of
+this GPU, and a CPU baseline (the reference's own sequential n-queens, built from its
+sources into oracle/_ref/, timed on this host).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import statistics
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NQ_N = 14
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(nq_tasks: int) -> dict:
+    """Sequential CPU n-queens(14) on this host: the reference's seq_nq when it was built
+    (oracle/_ref), else the oracle's restatement."""
+    import oracle
+    ref = oracle.ref_binary("seq_nq")
+    times = []
+    if ref:
+        for _ in range(3):
+            out = subprocess.run([ref, str(NQ_N)], capture_output=True, text=True, timeout=300).stdout
+            m = re.search(r"Execution time: ([0-9.]+) ms", out)
+            if m:
+                times.append(float(m.group(1)) / 1e3)
+        kind = "reference"
+        sample = f"reference seq_nq {NQ_N} (evaluation/2-comparison/nq/seq_nq.c), full workload, median of 3 runs"
+    if not times:
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oracle.nq(NQ_N)
+            times.append(time.perf_counter() - t0)
+        kind = "port"
+        sample = f"oracle nq({NQ_N}) (restatement of seq_nq.c), full workload, median of 3 runs"
+    t = statistics.median(times)
+    return {"value": nq_tasks / t, "unit": "tasks/s", "cores": 1, "kind": kind, "sample": sample,
+            "seconds": t}
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    p = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    return d.get("traffic_bytes")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
+              f"the workload would shard by subtrees, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import gtap_amd
+    from gtap_amd import roofline
+
+    import oracle
+    nq_solutions, nq_tasks = oracle.nq(NQ_N)
+
+    gtap_amd.gtap_set_config(timeout_ms=30000)
+    st = gtap_amd.gtap_initialize()
+    if st != 0:
+        raise RuntimeError(f"gtap_initialize failed: {st}")
+    info = gtap_amd.device_info()
+    log("device", info)
+    hbm_peak_meas = gtap_amd.bench_hbm_copy(2 << 30, 3)
+    atomic_peak_meas = gtap_amd.bench_atomics(3)
+    log(f"measured peaks: hbm copy {hbm_peak_meas:.0f} GB/s, atomics {atomic_peak_meas:.2f} Gop/s")
+
+    for i in range(args.warmup):
+        v, s = gtap_amd.nqueens(NQ_N)
+        assert v == nq_solutions, (v, nq_solutions)
+        log(f"warmup {i}: {s['kernel_ms']:.3f} ms")
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        v, s = gtap_amd.nqueens(NQ_N)
+        kernel_ms.append(s["kernel_ms"])
+        if v != nq_solutions:
+            raise RuntimeError(f"nqueens({NQ_N}) returned {v}, expected {nq_solutions}")
+    wall = time.perf_counter() - t0
+    workers = s["workers"]
+    log(f"timed {args.steps} steps: wall {wall*1e3:.3f} ms, kernel median {statistics.median(kernel_ms):.3f} ms")
+
+    value = nq_tasks * args.steps / wall
+    avg_kernel_s = statistics.mean(kernel_ms) / 1e3
+    achieved_gbps = roofline.nq_bytes(nq_tasks - 1) / avg_kernel_s / 1e9
+    traffic = load_traffic("nq14")
+    atomics_per_s = roofline.ATOMICS_PER_TASK * (nq_tasks - 1) / avg_kernel_s
+
+    # fib(30): the other scheduler-bound config
+    fib_tasks = oracle.fib_tasks(30)
+    fib_ms = []
+    for _ in range(5):
+        fv, fs = gtap_amd.fib(30)
+        assert fv == 832040
+        fib_ms.append(fs["kernel_ms"])
+    fib_med = statistics.median(fib_ms)
+
+    cpu = None if args.no_cpu else cpu_baseline(nq_tasks)
+    gtap_amd.gtap_finalize()
+
+    line = {
+        "metric": "tasks/sec (fib, nq, cilksort); batched-stream jobs/sec + p99 latency, 1 GPU",
+        "value": value,
+        "unit": "tasks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"nqueens N={NQ_N} thread-executed, task per legal placement, taskwait + int64 sum, "
+                        f"no cutoff ({nq_tasks} tasks per run)",
+            "n": NQ_N,
+            "tasks_per_step": nq_tasks,
+            "workers": workers,
+            "parallelism": "single GPU, wavefront workers",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved_gbps,
+            "peak": roofline.MI355X_HBM_GBPS,
+            "unit": "GB/s",
+            "frac": achieved_gbps / roofline.MI355X_HBM_GBPS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "extra": {
+            "kernel_ms_median": statistics.median(kernel_ms),
+            "kernel_tasks_per_s": nq_tasks / (statistics.median(kernel_ms) / 1e3),
+            "atomic_roofline": {
+                "achieved_gops": atomics_per_s / 1e9,
+                "measured_peak_gops": atomic_peak_meas,
+                "frac": atomics_per_s / 1e9 / atomic_peak_meas if atomic_peak_meas else None,
+            },
+            "hbm_measured_copy_gbps": hbm_peak_meas,
+            "fib30": {"kernel_ms_median": fib_med, "tasks": fib_tasks,
+                      "tasks_per_s": fib_tasks / (fib_med / 1e3)},
+            "device": info,
+        },
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -138,7 +138,10 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_memcpy_dtoh": ([_vp, _vp, _u64], _i),
     "gtap_memcpy_dtod": ([_vp, _vp, _u64], _i),
     "gtap_memset_device": ([_vp, _i, _u64], _i),
+    "gtap_bench_hbm_copy": ([_u64, _i, ctypes.POINTER(ctypes.c_double)], _i),
+    "gtap_bench_atomics": ([_i, ctypes.POINTER(ctypes.c_double)], _i),
     "gtap_fib": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
+    "gtap_nqueens": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -231,3 +234,25 @@ def fib(n: int) -> tuple[int, dict]:
     s = RunStats()
     _check(lib().gtap_fib(n, ctypes.byref(r), ctypes.byref(s)), f"gtap_fib({n})")
     return r.value, s.as_dict()
+
+
+def nqueens(n: int) -> tuple[int, dict]:
+    """Number of n-queens placements as one persistent launch; returns (count, stats)."""
+    r = ctypes.c_int64(0)
+    s = RunStats()
+    _check(lib().gtap_nqueens(n, ctypes.byref(r), ctypes.byref(s)), f"gtap_nqueens({n})")
+    return r.value, s.as_dict()
+
+
+def bench_hbm_copy(nbytes: int = 2 << 30, reps: int = 3) -> float:
+    """Measured streaming-copy bandwidth of this GPU in GB/s (read + write bytes)."""
+    g = ctypes.c_double(0)
+    _check(lib().gtap_bench_hbm_copy(nbytes, reps, ctypes.byref(g)), "gtap_bench_hbm_copy")
+    return g.value
+
+
+def bench_atomics(reps: int = 3) -> float:
+    """Measured agent-scope atomic fetch-add throughput of this GPU in Gop/s."""
+    g = ctypes.c_double(0)
+    _check(lib().gtap_bench_atomics(reps, ctypes.byref(g)), "gtap_bench_atomics")
+    return g.value

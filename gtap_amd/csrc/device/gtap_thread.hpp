@@ -340,7 +340,8 @@ struct Lane {
         for (int i = 0; i < (int)(sizeof(A) / 8); ++i) st64(dst + i, src[i]);
         queue_append(*dev, w, me, q, id);
         if (par != kNoParent) child_count += 1;
-        vst(&w->spawns, vld(&w->spawns) + 1u);
+        const uint64_t am = active_mask();
+        if (rank_in(am) == 0) vst(&w->spawns, vld(&w->spawns) + (uint32_t)popc(am));
         return id;
     }
 

@@ -65,6 +65,12 @@ int gtap_get_last_stats(gtap_run_stats* out);
 int gtap_get_worker_stats(uint64_t* out, uint32_t max_workers, uint32_t* nworkers_out);
 int gtap_device_info(char* name, int name_len, int* num_cus, uint64_t* hbm_bytes);
 
+/* Measured peaks of this GPU for roofline fractions (no reference counterpart):
+ * streaming-copy HBM bandwidth (read + write bytes, GB/s) and agent-scope relaxed atomic
+ * fetch-add throughput on distinct 64-byte words (Gop/s). Best of `reps` launches. */
+int gtap_bench_hbm_copy(uint64_t bytes, int reps, double* gbps);
+int gtap_bench_atomics(int reps, double* gops);
+
 /* ------------------------------------------------------------ device memory helpers */
 /* For callers that only hold host pointers (ctypes, plain C). */
 int gtap_device_malloc(void** ptr, uint64_t bytes);
@@ -81,6 +87,11 @@ int gtap_memset_device(void* dst, int value, uint64_t bytes);
 /* fib(n) with an int64 result; replaces the program of
  * evaluation/2-comparison/fib/gtap_fib.cu:10-27 + main (examples/fib/fib.cu:26-57). */
 int gtap_fib(int n, int64_t* result, gtap_run_stats* stats);
+
+/* Number of n-queens placements, n in [1, 16]: one task per legal placement, each
+ * returning its int64 count through a taskwait; replaces the program of
+ * evaluation/2-comparison/nq/gtap_nq.cu:25-99 (which counts with a global atomicAdd). */
+int gtap_nqueens(int n, int64_t* result, gtap_run_stats* stats);
 
 #ifdef __cplusplus
 }

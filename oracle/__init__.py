@@ -51,6 +51,8 @@ def lib() -> ctypes.CDLL:
         L.gtap_oracle_fib_tasks.restype = i64
         L.gtap_oracle_nq.argtypes = [ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.gtap_oracle_nq.restype = None
+        L.gtap_oracle_nq_internal.argtypes = [ctypes.c_int]
+        L.gtap_oracle_nq_internal.restype = i64
         L.gtap_oracle_fib_omp.argtypes = [ctypes.c_int, ctypes.c_int]
         L.gtap_oracle_fib_omp.restype = i64
         L.gtap_oracle_nq_omp.argtypes = [ctypes.c_int, ctypes.c_int]
@@ -89,6 +91,10 @@ def nq(n: int) -> tuple[int, int]:
     s, t = ctypes.c_int64(), ctypes.c_int64()
     lib().gtap_oracle_nq(n, ctypes.byref(s), ctypes.byref(t))
     return int(s.value), int(t.value)
+
+
+def nq_internal(n: int) -> int:
+    return int(lib().gtap_oracle_nq_internal(n))
 
 
 def fib_omp(n: int, cutoff: int = 0) -> int:

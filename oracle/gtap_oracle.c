@@ -86,6 +86,24 @@ void gtap_oracle_nq(int n, int64_t* solutions, int64_t* tasks) {
     *tasks = t;
 }
 
+/* Nodes of the search tree that have at least one child (they suspend at the taskwait
+ * and run a second time). */
+static int64_t nq_internal_rec(int n, int row, uint32_t col, uint32_t left, uint32_t right) {
+    if (row == n) return 0;
+    uint32_t mask = (n < 32 ? (1u << n) - 1u : 0xFFFFFFFFu);
+    uint32_t avail = mask & ~(col | left | right);
+    if (!avail) return 0;
+    int64_t c = 1;
+    while (avail) {
+        uint32_t p = avail & (0u - avail);
+        avail -= p;
+        c += nq_internal_rec(n, row + 1, col | p, (left | p) << 1, (right | p) >> 1);
+    }
+    return c;
+}
+
+int64_t gtap_oracle_nq_internal(int n) { return nq_internal_rec(n, 0, 0, 0, 0); }
+
 static int64_t nq_count_serial(int n, int row, uint32_t col, uint32_t left, uint32_t right) {
     if (row == n) return 1;
     uint32_t mask = (n < 32 ? (1u << n) - 1u : 0xFFFFFFFFu);
@@ -235,6 +253,9 @@ static void cilksort_rec_u32(uint32_t* arr, int64_t n, uint32_t* tmp, int64_t le
 void gtap_oracle_cilksort_u32(uint32_t* arr, uint32_t* tmp, int64_t n, int64_t leaf_max, int64_t merge_cutoff) {
     if (n <= 0) return;
     if (leaf_max < 1) leaf_max = 1;
+    /* the binary split only shrinks a merge of >= 3 keys (a 1+1 merge would split into
+     * itself), so merges below 3 keys are always sequential */
+    if (merge_cutoff < 3) merge_cutoff = 3;
     cilksort_rec_u32(arr, n, tmp, leaf_max, merge_cutoff);
 }
 

@@ -35,6 +35,9 @@ int64_t gtap_oracle_fib_tasks(int n);
  * Pinned: OEIS A000170 counts, reference binary seq_nq, brute force over
  * column permutations for small n. */
 void gtap_oracle_nq(int n, int64_t* solutions, int64_t* tasks);
+/* Search-tree nodes with at least one legal column (the tasks that suspend at the
+ * taskwait and resume once). Pinned: brute force in the tests. */
+int64_t gtap_oracle_nq_internal(int n);
 
 /* OpenMP-task versions timed as CPU baselines (evaluation/2-comparison/fib/omp_fib.c:5-15,
  * evaluation/2-comparison/nq/omp_nq.c:23-43 without cutoff, taskwait + int64 sum). */

@@ -1,0 +1,53 @@
+"""The C-ABI library loads and exports every symbol include/gtap.h declares (CPU only,
+no compute call)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import gtap_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gtap.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\s*\*)\s+(gtap_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for s in ("gtap_initialize", "gtap_finalize", "gtap_reset", "gtap_synchronize",
+              "gtap_get_runtime_error_code", "gtap_visualize_profile", "gtap_fib"):
+        assert s in syms
+
+
+@pytest.mark.skipif(not os.path.exists(gtap_amd.LIB_PATH), reason="libgtap_amd.so not built")
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(gtap_amd.LIB_PATH)
+    for s in declared_symbols():
+        assert hasattr(L, s), s
+        assert s in gtap_amd.SIGNATURES, f"{s} has no ctypes signature in gtap_amd"
+
+
+@pytest.mark.skipif(not os.path.exists(gtap_amd.LIB_PATH), reason="libgtap_amd.so not built")
+def test_bindings_cover_only_declared_symbols():
+    declared = set(declared_symbols())
+    assert set(gtap_amd.SIGNATURES) <= declared
+
+
+def test_error_strings_without_gpu():
+    if not os.path.exists(gtap_amd.LIB_PATH):
+        pytest.skip("libgtap_amd.so not built")
+    assert gtap_amd.gtap_get_runtime_error_string(0) == "No error"
+    assert gtap_amd.gtap_get_runtime_error_string(3) == "Queue overflow"
+    assert gtap_amd.gtap_get_runtime_error_string(999) == "Unknown error"
+
+
+def test_struct_sizes_match_c_layout():
+    assert ctypes.sizeof(gtap_amd.Job) == 128
+    assert ctypes.sizeof(gtap_amd.ErrorReport) == 36
+    assert ctypes.sizeof(gtap_amd.RunStats) == 8 * 9 + 16
