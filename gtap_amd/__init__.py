@@ -142,6 +142,14 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_bench_atomics": ([_i, ctypes.POINTER(ctypes.c_double)], _i),
     "gtap_fib": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
     "gtap_nqueens": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
+    "gtap_bfs": ([_vp, _vp, ctypes.c_int32, _i64, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32,
+                  ctypes.POINTER(RunStats)], _i),
+    "gtap_bfs_host": ([_vp, _vp, ctypes.c_int32, _i64, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32,
+                       ctypes.POINTER(RunStats)], _i),
+    "gtap_rmat_edges": ([_i, _i64, _u64, _u32, _u32, _u32, _vp, _vp], _i),
+    "gtap_csr_from_edges": ([ctypes.c_int32, _i64, _vp, _vp, _vp, _vp], _i),
+    "gtap_rmat_csr": ([_i, _i, _u64, _u32, _u32, _u32, _vp, _vp], _i),
+    "gtap_gen_keys": ([_u64, _u64, _vp], _i),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -256,3 +264,66 @@ def bench_atomics(reps: int = 3) -> float:
     g = ctypes.c_double(0)
     _check(lib().gtap_bench_atomics(reps, ctypes.byref(g)), "gtap_bench_atomics")
     return g.value
+
+
+# ---------------------------------------------------------------- graphs and BFS
+
+RMAT_THRESHOLDS = (2448131359, 3264175145, 4080218931)  # (a, a+b, a+b+c) * 2^32, a,b,c = .57,.19,.19
+INF_DEPTH = 0x3F3F3F3F
+
+
+def rmat_csr(scale: int, edge_factor: int = 16, seed: int = 1):
+    """R-MAT graph (the library's own host generator) as CSR numpy arrays."""
+    import numpy as np
+    nv = 1 << scale
+    row = np.empty(nv + 1, dtype=np.int32)
+    col = np.empty(nv * edge_factor, dtype=np.int32)
+    ta, tab, tabc = RMAT_THRESHOLDS
+    _check(lib().gtap_rmat_csr(scale, edge_factor, seed, ta, tab, tabc, row.ctypes.data, col.ctypes.data),
+           "gtap_rmat_csr")
+    return row, col
+
+
+def gen_keys(n: int, seed: int = 123456789):
+    import numpy as np
+    out = np.empty(n, dtype=np.uint32)
+    _check(lib().gtap_gen_keys(seed, n, out.ctypes.data), "gtap_gen_keys")
+    return out
+
+
+def bfs(row, col, source: int = 0, num_queues: int = 4, split: int = 64):
+    """BFS depths from host CSR arrays (copied to the GPU and back); returns (depth, stats)."""
+    import numpy as np
+    row = np.ascontiguousarray(row, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    nv = row.size - 1
+    depth = np.empty(nv, dtype=np.int32)
+    s = RunStats()
+    _check(lib().gtap_bfs_host(row.ctypes.data, col.ctypes.data, nv, col.size, source, depth.ctypes.data,
+                               num_queues, split, ctypes.byref(s)), "gtap_bfs_host")
+    return depth, s.as_dict()
+
+
+class DeviceBuffer:
+    """Device memory owned through the C ABI (no torch in the process)."""
+
+    def __init__(self, nbytes: int):
+        p = ctypes.c_void_p(0)
+        _check(lib().gtap_device_malloc(ctypes.byref(p), max(1, nbytes)), "gtap_device_malloc")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, a):
+        b = cls(a.nbytes)
+        _check(lib().gtap_memcpy_htod(b.ptr, a.ctypes.data, a.nbytes), "gtap_memcpy_htod")
+        return b
+
+    def to_array(self, out):
+        _check(lib().gtap_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), "gtap_memcpy_dtoh")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().gtap_device_free(self.ptr)
+            self.ptr = None

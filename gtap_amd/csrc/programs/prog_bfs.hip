@@ -1,0 +1,182 @@
+// prog_bfs.hip -- asynchronous (label-correcting) BFS as a thread-executed task program,
+// with EPAQ routing of vertex tasks by out-degree bucket.
+//
+// Source program (shape of the reference's examples/asynchronous_bfs/asynchronous_bfs_thread.cu:19-55
+// and evaluation/2-comparison/bfs/gtap_thread_bfs.cu:21-54):
+//
+//   #pragma gtap function
+//   __device__ void bfs(int v, int s, int e, int d) {
+//       if (depth[v] < d) return;                      // a shorter path already re-expanded v
+//       if (e - s > SPLIT) {                           // split a long adjacency range in halves
+//           int m = s + (e - s) / 2;
+//           #pragma gtap task queue(bucket(e - s))
+//           bfs(v, s, m, d);
+//           #pragma gtap task queue(bucket(e - s))
+//           bfs(v, m, e, d);
+//           return;
+//       }
+//       for (int i = s; i < e; ++i) {
+//           int u = col[i];
+//           if (atomicMin(&depth[u], d + 1) > d + 1) {
+//               #pragma gtap task queue(bucket(row[u + 1] - row[u]))
+//               bfs(u, row[u], row[u + 1], d + 1);
+//           }
+//       }
+//   }
+//
+// bucket(deg) = 0 for deg < 8, 1 for < 64, 2 for < 512, 3 otherwise (4 EPAQ queues); with one
+// queue every task goes to queue 0. No task waits for its children (detached spawns); the
+// launch ends by quiescence. Distances are the BFS hop counts whatever the schedule.
+#include "../device/gtap_abi.hpp"
+#include "../host/gtap_launch.hpp"
+
+namespace gtap {
+namespace bfs {
+
+enum : uint32_t { F_BFS = 1 };
+struct BfsArg {
+    int32_t v;
+    int32_t s;
+    int32_t e;
+    int32_t d;
+};
+struct BfsArgs {
+    const int32_t* row;
+    const int32_t* col;
+    int32_t* depth;
+    int32_t nv;
+    int32_t split;
+};
+
+__host__ __device__ __forceinline__ uint32_t degree_bucket(int32_t deg) {
+    return deg < 8 ? 0u : (deg < 64 ? 1u : (deg < 512 ? 2u : 3u));
+}
+
+template <int NQ>
+struct Program {
+    static constexpr int kQueues = NQ;
+    static constexpr int kFrameWords = 4;
+    static constexpr uint32_t kMaxFunc = F_BFS;
+    static constexpr uint32_t kFrameBytes = 16;
+    using Args = BfsArgs;
+    template <class L>
+    __device__ static void run(L& lane);
+};
+
+template <int NQ>
+__device__ __forceinline__ uint32_t route(int32_t deg) {
+    if constexpr (NQ == 1) {
+        return 0u;
+    } else {
+        return degree_bucket(deg);
+    }
+}
+
+template <int NQ, class L>
+__device__ __forceinline__ void bfs_task(L& lane) {
+    const BfsArgs& g = *lane.args;
+    const int32_t v = lane.template arg<int32_t>(0);
+    const int32_t s = lane.template arg<int32_t>(4);
+    const int32_t e = lane.template arg<int32_t>(8);
+    const int32_t d = lane.template arg<int32_t>(12);
+    if (ld32(g.depth + v) < d) {  // stale: v was reached by a shorter path meanwhile
+        __gtap_finish_task(lane);
+        return;
+    }
+    if (e - s > g.split) {
+        const int32_t m = s + (e - s) / 2;
+        const uint32_t q = route<NQ>(e - s);
+        __gtap_spawn_task_detached(lane, F_BFS, q, BfsArg{v, s, m, d});
+        __gtap_spawn_task_detached(lane, F_BFS, q, BfsArg{v, m, e, d});
+        __gtap_finish_task(lane);
+        return;
+    }
+    const int32_t nd = d + 1;
+    for (int32_t i = s; i < e; ++i) {
+        const int32_t u = g.col[i];
+        if ((uint32_t)u >= (uint32_t)g.nv) {
+            GTAP_DEVICE_ERROR(*lane.dev, GTAP_ERROR_INVALID_ARGUMENT, lane.tid, -1, u, g.nv);
+            break;
+        }
+        const int32_t old = __hip_atomic_fetch_min(g.depth + u, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old > nd) {
+            const int32_t us = g.row[u];
+            const int32_t ue = g.row[u + 1];
+            __gtap_spawn_task_detached(lane, F_BFS, route<NQ>(ue - us), BfsArg{u, us, ue, nd});
+        }
+    }
+    __gtap_finish_task(lane);
+}
+
+template <int NQ>
+template <class L>
+__device__ void Program<NQ>::run(L& lane) {
+    bfs_task<NQ>(lane);
+}
+
+}  // namespace bfs
+}  // namespace gtap
+
+using Bfs1 = gtap::bfs::Program<1>;
+using Bfs4 = gtap::bfs::Program<4>;
+
+namespace {
+
+__global__ void bfs_init_depth(int32_t* depth, int32_t nv, int32_t source) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = i; v < nv; v += stride) depth[v] = (v == source) ? 0 : 0x3f3f3f3f;
+}
+
+}  // namespace
+
+// BFS from `source` over a CSR graph already in device memory (row: nv+1 int32 offsets,
+// col: ne int32 columns). depth (nv int32, device) receives the hop counts, 0x3f3f3f3f for
+// unreached vertices. num_queues = 1 (no EPAQ) or 4 (EPAQ by out-degree bucket).
+// Replaces the program of examples/asynchronous_bfs/asynchronous_bfs_thread.cu:51-141.
+extern "C" int gtap_bfs(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne, int32_t source,
+                        int32_t* depth, int32_t num_queues, int32_t split, gtap_run_stats* stats) {
+    using namespace gtap;
+    if (!row || !col || !depth || nv <= 0 || source < 0 || source >= nv || ne < 0)
+        return host::invalid_argument("gtap_bfs: bad graph arguments");
+    if (num_queues != 1 && num_queues != 4) return host::invalid_argument("gtap_bfs: num_queues must be 1 or 4");
+    if (split <= 0) split = 64;
+    host::Runtime& R = host::rt();
+    if (!R.initialized) {
+        int st = gtap_initialize();
+        if (st) return st;
+    }
+    hipLaunchKernelGGL(bfs_init_depth, dim3(1024), dim3(256), 0, R.stream, depth, nv, source);
+    GTAP_HIP_TRY(hipGetLastError());
+    int32_t rs[2];
+    GTAP_HIP_TRY(hipMemcpy(rs, row + source, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    bfs::BfsArg root{source, rs[0], rs[1], 0};
+    const uint32_t q = (num_queues == 4) ? bfs::degree_bucket(rs[1] - rs[0]) : 0u;
+    gtap_job job = host::make_job(bfs::F_BFS, q, &root, sizeof(root));
+    bfs::BfsArgs a{row, col, depth, nv, split};
+    if (num_queues == 4) return host::run_single_job<Bfs4>(a, job, nullptr, stats, GTAP_TERM_QUIESCE);
+    return host::run_single_job<Bfs1>(a, job, nullptr, stats, GTAP_TERM_QUIESCE);
+}
+
+// Host-pointer convenience: copies the CSR in, runs gtap_bfs, copies depth out.
+extern "C" int gtap_bfs_host(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne, int32_t source,
+                             int32_t* depth_out, int32_t num_queues, int32_t split, gtap_run_stats* stats) {
+    if (!row || !col || !depth_out || nv <= 0 || ne < 0) return gtap::host::invalid_argument("gtap_bfs_host");
+    int32_t *drow = nullptr, *dcol = nullptr, *ddep = nullptr;
+    GTAP_HIP_TRY(hipMalloc(&drow, (size_t)(nv + 1) * sizeof(int32_t)));
+    GTAP_HIP_TRY(hipMalloc(&dcol, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t)));
+    GTAP_HIP_TRY(hipMalloc(&ddep, (size_t)nv * sizeof(int32_t)));
+    int st = 0;
+    hipError_t e = hipMemcpy(drow, row, (size_t)(nv + 1) * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && ne > 0) e = hipMemcpy(dcol, col, (size_t)ne * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) st = gtap::host::hip_status(e, "gtap_bfs_host copy-in");
+    if (!st) st = gtap_bfs(drow, dcol, nv, ne, source, ddep, num_queues, split, stats);
+    if (!st) {
+        e = hipMemcpy(depth_out, ddep, (size_t)nv * sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = gtap::host::hip_status(e, "gtap_bfs_host copy-out");
+    }
+    (void)hipFree(drow);
+    (void)hipFree(dcol);
+    (void)hipFree(ddep);
+    return st;
+}

@@ -93,6 +93,29 @@ int gtap_fib(int n, int64_t* result, gtap_run_stats* stats);
  * evaluation/2-comparison/nq/gtap_nq.cu:25-99 (which counts with a global atomicAdd). */
 int gtap_nqueens(int n, int64_t* result, gtap_run_stats* stats);
 
+/* Asynchronous BFS from `source` over a CSR graph in device memory (row: nv+1 int32,
+ * col: ne int32; depth: nv int32 out, 0x3f3f3f3f = unreached). num_queues 1 = one queue,
+ * 4 = EPAQ by out-degree bucket (<8, <64, <512, >=512); `split` = edges per range task
+ * (0 -> 64). Replaces examples/asynchronous_bfs/asynchronous_bfs_thread.cu:19-141 and
+ * evaluation/2-comparison/bfs/gtap_thread_bfs.cu:21-60. */
+int gtap_bfs(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne, int32_t source, int32_t* depth,
+             int32_t num_queues, int32_t split, gtap_run_stats* stats);
+/* The same with host buffers (copied in and out around the launch). */
+int gtap_bfs_host(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne, int32_t source,
+                  int32_t* depth_out, int32_t num_queues, int32_t split, gtap_run_stats* stats);
+
+/* ------------------------------------------------------------ input generators (host) */
+/* R-MAT edges, CSR by stable counting sort, and the gen_vector key stream
+ * (evaluation/util/gen_vector.c:8-18; read format examples/asynchronous_bfs/
+ * asynchronous_bfs_thread.cu:69-84). */
+int gtap_rmat_edges(int scale, int64_t nedges, uint64_t seed, uint32_t ta, uint32_t tab, uint32_t tabc,
+                    int32_t* src, int32_t* dst);
+int gtap_csr_from_edges(int32_t nv, int64_t ne, const int32_t* src, const int32_t* dst, int32_t* row,
+                        int32_t* col);
+int gtap_rmat_csr(int scale, int edge_factor, uint64_t seed, uint32_t ta, uint32_t tab, uint32_t tabc,
+                  int32_t* row, int32_t* col);
+int gtap_gen_keys(uint64_t seed, uint64_t n, uint32_t* out);
+
 #ifdef __cplusplus
 }
 #endif
