@@ -146,6 +146,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
                   ctypes.POINTER(RunStats)], _i),
     "gtap_bfs_host": ([_vp, _vp, ctypes.c_int32, _i64, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32,
                        ctypes.POINTER(RunStats)], _i),
+    "gtap_cilksort": ([_vp, _vp, _i64, _i64, _i64, ctypes.POINTER(RunStats)], _i),
+    "gtap_cilksort_host": ([_vp, _i64, _i64, _i64, ctypes.POINTER(RunStats)], _i),
     "gtap_rmat_edges": ([_i, _i64, _u64, _u32, _u32, _u32, _vp, _vp], _i),
     "gtap_csr_from_edges": ([ctypes.c_int32, _i64, _vp, _vp, _vp, _vp], _i),
     "gtap_rmat_csr": ([_i, _i, _u64, _u32, _u32, _u32, _vp, _vp], _i),
@@ -327,3 +329,15 @@ class DeviceBuffer:
         if self.ptr:
             lib().gtap_device_free(self.ptr)
             self.ptr = None
+
+
+# ---------------------------------------------------------------- sorting
+
+def cilksort(keys, leaf: int = 4096, merge_leaf: int = 65536):
+    """Sorted copy of uint32 keys (block-cooperative cilksort on the GPU); returns (keys, stats)."""
+    import numpy as np
+    a = np.ascontiguousarray(keys, dtype=np.uint32).copy()
+    s = RunStats()
+    _check(lib().gtap_cilksort_host(a.ctypes.data if a.size else None, a.size, leaf, merge_leaf, ctypes.byref(s)),
+           "gtap_cilksort_host")
+    return a, s.as_dict()

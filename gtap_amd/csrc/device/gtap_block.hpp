@@ -1,0 +1,497 @@
+// gtap_block.hpp -- block-cooperative persistent scheduler (one task per 256-thread
+// workgroup at a time), the counterpart of the reference's block mode
+// (runtime/detail/block/gtap_runtime_block.cuh:326-777, gtap_block_core.cuh).
+//
+//  * Worker = one workgroup of 256 threads (4 waves). Every thread enters the task
+//    function; task bodies are written data-parallel over threadIdx.x. taskwait, return and
+//    finish are workgroup-collective (they contain barriers), spawn may be called by any
+//    subset of threads.
+//  * Same task records, id pool and deque words as the thread-mode scheduler. A workgroup
+//    owns one deque; it publishes with one atomic add, pops one task from the bottom and
+//    thieves take one task from the top, each with one CAS on the (top | bottom) word.
+//  * Work first (reference: push, gtap_runtime_block.cuh:400-438): after a task the
+//    workgroup continues with the parent it just made ready, else with its newest child,
+//    and publishes the other children.
+//  * Thread 0 runs the scheduler; the others wait at a barrier.
+#pragma once
+
+#include "gtap_thread.hpp"
+
+namespace gtap {
+
+constexpr uint32_t kBlockThreads = 256;
+constexpr uint32_t kBlockMaxSpawn = 128;   // spawns per task invocation (ids kept ready in LDS)
+
+struct BlockCtx {
+    // scheduler state (thread 0)
+    uint32_t cur;          // task to run next (0 = none)
+    uint32_t exit;
+    uint32_t tail;         // private tail of the deque (absolute position)
+    uint32_t bottom;       // published bottom
+    uint32_t top_cache;
+    uint32_t free_top;     // ids in free_ids
+    uint32_t priv_top;     // ids in this workgroup's HBM overflow stack
+    uint32_t child_count;  // joined children spawned by the current invocation
+    uint32_t taken;        // ids taken from free_ids by the current invocation
+    uint32_t spawns;
+    uint32_t refills;
+    // header of the current task (written by thread 0, read by all)
+    uint32_t tid, func, state, parent, res_off, outcome;
+    uint32_t frame[16];
+    uint32_t free_ids[kFreeCap];
+};
+
+__device__ __forceinline__ uint32_t blk_thread() { return threadIdx.x; }
+
+// Block-collective refill of the LDS id stack with one chunk (thread 0 claims, all copy).
+template <int DUMMY = 0>
+__device__ __forceinline__ void block_refill_chunk(const GtapDev& dev, BlockCtx* c, uint32_t me) {
+    __shared__ uint32_t s_kind, s_pos, s_shard, s_fresh;
+    const uint32_t t = blk_thread();
+    const uint32_t top = c->free_top;
+    const uint32_t ptop = c->priv_top;
+    if (ptop >= kChunk) {
+        const uint32_t* src = dev.priv_ids + (size_t)me * kPrivCap + (ptop - kChunk);
+        if (t < kChunk) c->free_ids[top + t] = src[t];
+        __syncthreads();
+        if (t == 0) {
+            c->priv_top = ptop - kChunk;
+            c->free_top = top + kChunk;
+        }
+        __syncthreads();
+        return;
+    }
+    if (t == 0) {
+        uint32_t kind = 0, pos = 0, shard = 0, fresh = 0;
+        for (uint32_t i = 0; i < dev.nshards && kind == 0; ++i) {
+            shard = (me + i) % dev.nshards;
+            if (ring_take_claim(dev, shard, pos)) kind = 1;
+        }
+        if (kind == 0) {
+            fresh = atomic_add(pool_fresh_ptr(dev), kChunk);
+            if (fresh <= dev.nrecs - kChunk) kind = 2;
+        }
+        s_kind = kind;
+        s_pos = pos;
+        s_shard = shard;
+        s_fresh = fresh;
+        c->refills += 1;
+    }
+    __syncthreads();
+    const uint32_t kind = s_kind;
+    if (t < kChunk) {
+        uint32_t id = 0;
+        if (kind == 1) id = ld32(pool_cell_ptr(dev, s_shard, s_pos) + t);
+        else if (kind == 2) id = s_fresh + t;
+        c->free_ids[top + t] = id;
+    }
+    complete_stores();
+    __syncthreads();
+    if (t == 0) {
+        if (kind == 1) st32(pool_seq_ptr(dev, s_shard, s_pos), s_pos + dev.ncells);
+        if (kind == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_TASK_ID_POOL_LOW_HEADROOM, -1, -1, (int)s_fresh, (int)dev.nrecs);
+        c->free_top = top + kChunk;
+    }
+    __syncthreads();
+}
+
+// Block-collective spill of the 64 most recently freed ids.
+template <int DUMMY = 0>
+__device__ __forceinline__ void block_spill_chunk(const GtapDev& dev, BlockCtx* c, uint32_t me) {
+    __shared__ uint32_t s_ok, s_pos;
+    const uint32_t t = blk_thread();
+    const uint32_t top = c->free_top;
+    const uint32_t ptop = c->priv_top;
+    if (ptop + kChunk <= kPrivCap) {
+        uint32_t* dst = dev.priv_ids + (size_t)me * kPrivCap + ptop;
+        if (t < kChunk) dst[t] = c->free_ids[top - kChunk + t];
+        __syncthreads();
+        if (t == 0) {
+            c->priv_top = ptop + kChunk;
+            c->free_top = top - kChunk;
+        }
+        __syncthreads();
+        return;
+    }
+    const uint32_t shard = me % dev.nshards;
+    if (t == 0) {
+        uint32_t pos = 0;
+        s_ok = ring_put_claim(dev, shard, pos) ? 1u : 0u;
+        s_pos = pos;
+    }
+    __syncthreads();
+    if (s_ok && t < kChunk) st32(pool_cell_ptr(dev, shard, s_pos) + t, c->free_ids[top - kChunk + t]);
+    complete_stores();
+    __syncthreads();
+    if (t == 0) {
+        if (s_ok) st32(pool_seq_ptr(dev, shard, s_pos), s_pos + 1);
+        else GTAP_DEVICE_ERROR(dev, GTAP_ERROR_TASK_ID_POOL_SLOT_BUSY, -1, (int)shard, 0, (int)dev.ncells);
+        c->free_top = top - kChunk;
+    }
+    __syncthreads();
+}
+
+// Task context seen by every thread of the workgroup.
+template <class P>
+struct BlockTask {
+    const GtapDev* dev;
+    BlockCtx* c;
+    const typename P::Args* args;
+    typename P::Shared* sh;    // program scratch in LDS
+    uint32_t me;               // workgroup index
+    uint32_t tid, func, state, parent, res_off;
+    uint8_t* rec;
+
+    __device__ __forceinline__ uint32_t get_state() const { return state; }
+    template <typename T>
+    __device__ __forceinline__ T arg(int byte_off) const {
+        if constexpr (sizeof(T) == 4) {
+            return __builtin_bit_cast(T, c->frame[byte_off / 4]);
+        } else {
+            const uint64_t v = (uint64_t)c->frame[byte_off / 4] | ((uint64_t)c->frame[byte_off / 4 + 1] << 32);
+            return __builtin_bit_cast(T, v);
+        }
+    }
+    template <typename T>
+    __device__ __forceinline__ T frame_ld(int byte_off) const { return ldT<T>(rec + kHdrBytes + byte_off); }
+    template <typename T>
+    __device__ __forceinline__ void frame_st(int byte_off, T v) { stT<T>(rec + kHdrBytes + byte_off, v); }
+
+    // Any thread may spawn. Returns the child id.
+    template <typename A>
+    __device__ __forceinline__ uint32_t spawn(uint32_t f, int res_off_in_frame, const A& a) {
+        return spawn_impl(f, tid, kHdrBytes + (uint32_t)res_off_in_frame, a);
+    }
+    template <typename A>
+    __device__ __forceinline__ uint32_t spawn_detached(uint32_t f, const A& a) {
+        return spawn_impl(f, kNoParent, 0u, a);
+    }
+    template <typename A>
+    __device__ __forceinline__ uint32_t spawn_impl(uint32_t f, uint32_t par, uint32_t roff, const A& a) {
+        static_assert(sizeof(A) % 8 == 0 && sizeof(A) / 4 <= P::kFrameWords, "argument block");
+        // free_top does not change while a task runs; spawns claim ids below it
+        const uint32_t k = atomicAdd(&c->taken, 1u);
+        const uint32_t avail = c->free_top;
+        uint32_t id = 0;
+        if (k >= avail || k >= kBlockMaxSpawn) {
+            GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_GENERATED_TASK_ID_BUFFER_OVERFLOW, tid, -1, (int)k, (int)kBlockMaxSpawn);
+        } else {
+            id = c->free_ids[avail - 1 - k];
+        }
+        uint8_t* crec = rec_ptr(*dev, id);
+        st64(reinterpret_cast<uint64_t*>(crec), (uint64_t)hdr_word0(f, 0, 0) | ((uint64_t)par << 32));
+        st64(reinterpret_cast<uint64_t*>(crec + 8), ((uint64_t)roff << 32));
+        const uint64_t* src = reinterpret_cast<const uint64_t*>(&a);
+        uint64_t* dst = reinterpret_cast<uint64_t*>(crec + kHdrBytes);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(A) / 8); ++i) st64(dst + i, src[i]);
+        if (id != 0u) {
+            const uint32_t pos = atomicAdd(&c->tail, 1u);
+            if (pos + 1u - c->top_cache > dev->cap - kMargin) {
+                GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_QUEUE_OVERFLOW, id, 0, pos + 1u - c->top_cache, dev->cap - kMargin);
+            } else {
+                st32(dq_slot_ptr(*dev, 0, me, pos), id);
+            }
+            if (par != kNoParent) atomicAdd(&c->child_count, 1u);
+            atomicAdd(&c->spawns, 1u);
+        }
+        return id;
+    }
+
+    // Workgroup-collective taskwait: true -> return now, the task resumes at next_state.
+    __device__ __forceinline__ bool taskwait(uint32_t next_state) {
+        complete_stores();
+        __syncthreads();
+        const uint32_t cc = c->child_count;
+        if (cc == 0) {
+            state = next_state;
+            __syncthreads();
+            return false;
+        }
+        if (threadIdx.x == 0) {
+            st32(reinterpret_cast<uint32_t*>(rec), hdr_word0(func, next_state, 0));
+            st32(reinterpret_cast<uint32_t*>(rec + 8), cc);
+            complete_stores();
+            c->outcome = OUT_SUSPENDED;
+        }
+        __syncthreads();
+        return true;
+    }
+    // Workgroup-collective return; thread 0's value is stored.
+    template <typename T>
+    __device__ __forceinline__ void ret(T v) {
+        if (threadIdx.x == 0 && parent != kNoParent) stT<T>(rec_ptr(*dev, parent) + res_off, v);
+        finish();
+    }
+    __device__ __forceinline__ void finish() {
+        complete_stores();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (c->child_count != 0) GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_TASKWAIT, tid, -1, c->child_count, 0);
+            c->outcome = OUT_FINISHED;
+        }
+        __syncthreads();
+    }
+};
+
+// Thread 0 only: pop the newest published task of this workgroup's deque.
+__device__ __forceinline__ uint32_t block_pop(const GtapDev& dev, BlockCtx* c, uint32_t me) {
+    uint64_t* wp = dq_word_ptr(dev, 0, me);
+    for (int tries = 0; tries < 8; ++tries) {
+        const uint64_t wv = ld64(wp);
+        const uint32_t t = (uint32_t)(wv >> 32), b = (uint32_t)wv;
+        if ((int32_t)(b - t) <= 0) return 0;
+        uint64_t expected = wv;
+        if (atomic_cas(wp, expected, ((uint64_t)t << 32) | (uint64_t)(b - 1))) {
+            const uint32_t id = ld32(dq_slot_ptr(dev, 0, me, b - 1));
+            c->bottom = b - 1;
+            c->tail = b - 1;
+            c->top_cache = t;
+            return id;
+        }
+    }
+    return 0;
+}
+
+// Thread 0 only: steal the oldest task of a random victim.
+__device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me, uint32_t& rng, bool& idle_registered,
+                                                LoopStats& st) {
+    if (dev.nworkers < 2) return 0;
+    for (int probe = 0; probe < 4; ++probe) {
+        uint32_t victim = xorshift32(rng) % (dev.nworkers - 1);
+        if (victim >= me) victim += 1;
+        uint64_t* wp = dq_word_ptr(dev, 0, victim);
+        const uint64_t wv = ld64(wp);
+        const uint32_t t = (uint32_t)(wv >> 32), b = (uint32_t)wv;
+        if ((int32_t)(b - t) <= 0) continue;
+        if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
+            atomic_add(dev.ctl + CTL_IDLE * 32, -1);
+            idle_registered = false;
+        }
+        uint64_t expected = wv;
+        st.attempts += 1;
+        if (atomic_cas(wp, expected, wv + ((uint64_t)1 << 32))) {
+            st.steals += 1;
+            return ld32(dq_slot_ptr(dev, 0, victim, t));
+        }
+        return 0;
+    }
+    return 0;
+}
+
+// Thread 0 only: admit one job from the job table as a root task.
+template <class P>
+__device__ __forceinline__ uint32_t block_admit(const GtapDev& dev, BlockCtx* c, bool& idle_registered) {
+    uint32_t avail = dev.njobs;
+    if (dev.term_mode == GTAP_TERM_SERVE) avail = (uint32_t)ld32_sys(dev.host_ctl + HCTL_JOB_AVAIL * 32);
+    int32_t* nextp = dev.ctl + CTL_JOB_NEXT * 32;
+    const uint32_t next = (uint32_t)ld32(nextp);
+    if (next >= avail) return 0;
+    const uint32_t live = (uint32_t)ld32(dev.ctl + CTL_LIVE_JOBS * 32);
+    if (live >= dev.max_live_jobs) return 0;
+    if (c->free_top == 0) return 0;
+    if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
+        atomic_add(dev.ctl + CTL_IDLE * 32, -1);
+        idle_registered = false;
+    }
+    int32_t expected = (int32_t)next;
+    if (!atomic_cas(nextp, expected, (int32_t)(next + 1))) return 0;
+    atomic_add(dev.ctl + CTL_LIVE_JOBS * 32, 1);
+    const uint32_t slot = next % dev.job_ring;
+    gtap_job* job = dev.jobs + slot;
+    const uint32_t func = ld32_sys(&job->func);
+    const uint32_t anchor = dev.nrecs + slot;
+    c->free_top -= 1;  // free_top >= kBlockMaxSpawn here (refilled before acquisition)
+    const uint32_t id = c->free_ids[c->free_top];
+    uint8_t* arec = rec_ptr(dev, anchor);
+    st64(reinterpret_cast<uint64_t*>(arec), (uint64_t)hdr_word0(kAnchorFunc, 0, 0) | ((uint64_t)kNoParent << 32));
+    st64(reinterpret_cast<uint64_t*>(arec + 8), 1ull);
+    st64(reinterpret_cast<uint64_t*>(arec + kHdrBytes), 0ull);
+    uint8_t* rrec = rec_ptr(dev, id);
+    st64(reinterpret_cast<uint64_t*>(rrec), (uint64_t)hdr_word0(func, 0, 0) | ((uint64_t)anchor << 32));
+    st64(reinterpret_cast<uint64_t*>(rrec + 8), ((uint64_t)kHdrBytes << 32));
+    const uint64_t* ja = reinterpret_cast<const uint64_t*>(job->args);
+    uint64_t* ra = reinterpret_cast<uint64_t*>(rrec + kHdrBytes);
+    constexpr int kArgWords = (P::kFrameWords / 2) < 8 ? (P::kFrameWords / 2) : 8;
+#pragma unroll
+    for (int i = 0; i < kArgWords; ++i) st64(ra + i, ld64_sys(ja + i));
+    st64_sys(&job->t_start, now_ticks());
+    st32_sys(&job->status, (uint32_t)GTAP_JOB_RUNNING);
+    complete_stores();
+    return id;
+}
+
+template <class P>
+__device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args, BlockCtx* c,
+                                typename P::Shared* sh) {
+    const uint32_t t = blk_thread();
+    const uint32_t me = blockIdx.x;
+    const uint64_t t_begin = now_ticks();
+    if (t == 0) {
+        c->cur = 0;
+        c->exit = 0;
+        c->tail = c->bottom = c->top_cache = 0;
+        c->free_top = 0;
+        c->priv_top = 0;
+        c->child_count = 0;
+        c->taken = 0;
+        c->spawns = 0;
+        c->refills = 0;
+    }
+    __syncthreads();
+
+    BlockTask<P> T;
+    T.dev = &dev;
+    T.c = c;
+    T.args = &args;
+    T.sh = sh;
+    T.me = me;
+
+    uint32_t rng = 0x9E3779B9u ^ (me * 0x85EBCA6Bu) ^ (uint32_t)t_begin;
+    if (rng == 0) rng = 1;
+    bool idle_registered = false;
+    uint32_t idle_iter = 0;
+    LoopStats st = {0, 0, 0, 0, 0, 0};
+
+    while (true) {
+        // keep enough ids in LDS for one invocation's spawns (collective)
+        while (c->free_top < kBlockMaxSpawn) block_refill_chunk(dev, c, me);
+        if (t == 0 && c->cur == 0) {
+            uint32_t id = block_pop(dev, c, me);
+            if (id) st.pops += 1;
+            if (!id && dev.jobs != nullptr) {
+                id = block_admit<P>(dev, c, idle_registered);
+                if (id) st.jobs += 1;
+            }
+            if (!id) id = block_steal(dev, me, rng, idle_registered, st);
+            if (!id) {
+                if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered) {
+                    idle_registered = true;
+                    const int32_t cnt = atomic_add(dev.ctl + CTL_IDLE * 32, 1) + 1;
+                    if ((uint32_t)cnt == dev.nworkers && (uint32_t)ld32(dev.ctl + CTL_JOB_NEXT * 32) >= dev.njobs)
+                        st32(dev.ctl + CTL_DONE * 32, 1);
+                }
+                idle_iter += 1;
+                int32_t flags = ld32(dev.ctl + CTL_DONE * 32) | (ld32(dev.ctl + CTL_ABORT * 32) << 1);
+                if ((idle_iter & 63u) == 0 && now_ticks() - t_begin > dev.timeout_ticks) {
+                    GTAP_DEVICE_ERROR(dev, GTAP_ERROR_WATCHDOG, -1, -1, (int)((now_ticks() - t_begin) / 100000u),
+                                      (int)(dev.timeout_ticks / 100000u));
+                    flags |= 2;
+                }
+                if (flags) c->exit = 1;
+                else backoff_sleep(idle_iter < 8 ? 0 : (idle_iter < 64 ? 2 : (idle_iter < 256 ? 4 : 6)));
+            } else {
+                idle_iter = 0;
+                c->cur = id;
+            }
+            if (c->cur) {
+                // load the task header and frame for everyone
+                const uint32_t cur = c->cur;
+                if (cur >= dev.nrecs) {
+                    GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, cur, -1, cur, dev.nrecs);
+                    c->cur = 0;
+                    c->exit = 1;
+                }
+            }
+        }
+        __syncthreads();
+        if (c->exit) break;
+        if (c->cur == 0) continue;
+        if (t == 0) {
+            uint8_t* r = rec_ptr(dev, c->cur);
+            const uint64_t h0 = ld64(reinterpret_cast<const uint64_t*>(r));
+            const uint64_t h1 = ld64(reinterpret_cast<const uint64_t*>(r + 8));
+            c->tid = c->cur;
+            c->func = hdr_func((uint32_t)h0);
+            c->state = hdr_state((uint32_t)h0);
+            c->parent = (uint32_t)(h0 >> 32);
+            c->res_off = (uint32_t)(h1 >> 32);
+            c->outcome = OUT_RUNNING;
+            c->child_count = 0;
+            const uint64_t* fp = reinterpret_cast<const uint64_t*>(r + kHdrBytes);
+#pragma unroll
+            for (int i = 0; i < P::kFrameWords / 2; ++i) {
+                const uint64_t v = ld64(fp + i);
+                c->frame[2 * i] = (uint32_t)v;
+                c->frame[2 * i + 1] = (uint32_t)(v >> 32);
+            }
+        }
+        __syncthreads();
+        T.tid = c->tid;
+        T.func = c->func;
+        T.state = c->state;
+        T.parent = c->parent;
+        T.res_off = c->res_off;
+        T.rec = rec_ptr(dev, T.tid);
+        st.tasks += 1;
+        if (T.func == 0 || T.func > P::kMaxFunc) {
+            if (t == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, T.tid, 0, T.func, P::kMaxFunc);
+            T.finish();
+        } else {
+            P::run(T);
+        }
+        __syncthreads();
+        // ---- epilogue (thread 0): join, release, choose the next task, publish the rest
+        if (t == 0) {
+            if (c->outcome == OUT_RUNNING) {
+                GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, T.tid, 0, T.state, 0);
+                c->outcome = OUT_FINISHED;
+            }
+            uint32_t next = 0;
+            {
+                const uint32_t used = c->taken < c->free_top ? c->taken : c->free_top;
+                c->free_top -= used;
+                c->taken = 0;
+            }
+            if (c->outcome == OUT_FINISHED) {
+                const uint32_t par = c->parent;
+                if (par != kNoParent && par < dev.nrecs + dev.nanchors) {
+                    const uint32_t old = atomic_sub(reinterpret_cast<uint32_t*>(rec_ptr(dev, par) + 8), 1u);
+                    if (old == 1u) {
+                        if (par >= dev.nrecs) complete_job<P>(dev, par);
+                        else next = par;  // resume the parent right here
+                    }
+                }
+                if (T.tid != 0u) {
+                    c->free_ids[c->free_top] = T.tid;  // room is ensured by the spill below
+                    c->free_top += 1;
+                }
+            }
+            if (next == 0 && c->tail != c->bottom) {
+                c->tail -= 1;  // newest child runs next
+                next = ld32(dq_slot_ptr(dev, 0, me, c->tail));
+            }
+            if (c->tail != c->bottom) {
+                const uint64_t old = atomic_add(dq_word_ptr(dev, 0, me), (uint64_t)(c->tail - c->bottom));
+                c->top_cache = (uint32_t)(old >> 32);
+                c->bottom = c->tail;
+            }
+            c->cur = next;
+            if ((st.tasks & 63u) == 0 && ld32(dev.ctl + CTL_ABORT * 32)) c->exit = 1;
+        }
+        __syncthreads();
+        if (c->exit) break;
+        while (c->free_top > kFreeCap - kChunk) block_spill_chunk(dev, c, me);
+    }
+
+    if (t == 0) {
+        uint64_t* s = dev.stats + (size_t)me * ST_COUNT;
+        s[ST_TASKS] = st.tasks;
+        s[ST_SPAWNS] = c->spawns;
+        s[ST_ROUNDS] = st.tasks;
+        s[ST_POPS] = st.pops;
+        s[ST_STEALS] = st.steals;
+        s[ST_STEAL_ATTEMPTS] = st.attempts;
+        s[ST_JOBS] = st.jobs;
+        s[ST_REFILLS] = c->refills;
+    }
+}
+
+template <class P>
+__global__ __launch_bounds__(256) void block_kernel(GtapDev dev, typename P::Args args) {
+    __shared__ BlockCtx ctx;
+    __shared__ typename P::Shared sh;
+    block_scheduler<P>(dev, args, &ctx, &sh);
+}
+
+}  // namespace gtap

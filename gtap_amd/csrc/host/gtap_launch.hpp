@@ -14,7 +14,7 @@
 #include <string>
 #include <vector>
 
-#include "../device/gtap_thread.hpp"
+#include "../device/gtap_block.hpp"
 #include "../gtap_types.h"
 
 extern "C" int gtap_initialize(void);
@@ -131,6 +131,55 @@ int run_single_job(const typename P::Args& args, const gtap_job& job, int64_t* r
     gtap_job out;
     GTAP_HIP_TRY(hipMemcpy(&out, R.jobs, sizeof(gtap_job), hipMemcpyDeviceToHost));
     if (term_mode == GTAP_TERM_JOBS && out.status != GTAP_JOB_DONE) return fail(GTAP_ERROR_INVALID_TASKWAIT, "job did not complete");
+    if (result) *result = out.result;
+    return 0;
+}
+
+// Persistent launch of a block-mode program: one worker (and one deque) per workgroup.
+template <class P>
+int launch_block(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs, uint32_t job_ring,
+                 gtap_run_stats* stats, hipStream_t stream = nullptr) {
+    Runtime& R = rt();
+    if (!R.initialized) {
+        int st = gtap_initialize();
+        if (st) return st;
+    }
+    if (P::kFrameBytes + kHdrBytes > R.stride) return invalid_argument("task frame exceeds the record stride");
+    if (job_ring == 0 || job_ring > R.nanchors) return invalid_argument("job ring exceeds the anchor capacity");
+    if (!stream) stream = R.stream;
+    static uint32_t max_blocks = 0;
+    const void* kfn = reinterpret_cast<const void*>(&block_kernel<P>);
+    if (max_blocks == 0) {
+        int st = occupancy_blocks(kfn, 0, &max_blocks);
+        if (st) return st;
+    }
+    uint32_t blocks = max_blocks;
+    if (R.cfg.block_workers && R.cfg.block_workers < blocks) blocks = R.cfg.block_workers;
+    if (blocks > R.max_workers) blocks = R.max_workers;
+    GtapDev dev = make_dev(blocks, term_mode, njobs, jobs, job_ring);
+    int st = reset_launch_state(dev, stream);
+    if (st) return st;
+    GTAP_HIP_TRY(hipEventRecord(R.ev0, stream));
+    hipLaunchKernelGGL(block_kernel<P>, dim3(blocks), dim3(kBlockThreads), 0, stream, dev, args);
+    GTAP_HIP_TRY(hipGetLastError());
+    GTAP_HIP_TRY(hipEventRecord(R.ev1, stream));
+    GTAP_HIP_TRY(hipStreamSynchronize(stream));
+    return finish_launch(dev, blocks, stats);
+}
+
+template <class P>
+int run_single_block_job(const typename P::Args& args, const gtap_job& job, int64_t* result, gtap_run_stats* stats) {
+    Runtime& R = rt();
+    if (!R.initialized) {
+        int st = gtap_initialize();
+        if (st) return st;
+    }
+    GTAP_HIP_TRY(hipMemcpyAsync(R.jobs, &job, sizeof(gtap_job), hipMemcpyHostToDevice, R.stream));
+    int st = launch_block<P>(args, GTAP_TERM_JOBS, 1, R.jobs, 1, stats);
+    if (st) return st;
+    gtap_job out;
+    GTAP_HIP_TRY(hipMemcpy(&out, R.jobs, sizeof(gtap_job), hipMemcpyDeviceToHost));
+    if (out.status != GTAP_JOB_DONE) return fail(GTAP_ERROR_INVALID_TASKWAIT, "job did not complete");
     if (result) *result = out.result;
     return 0;
 }

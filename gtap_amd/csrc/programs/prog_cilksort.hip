@@ -1,0 +1,318 @@
+// prog_cilksort.hip -- cilksort of uint32 keys as a block-cooperative task program.
+//
+// Task structure of the reference's evaluation/2-comparison/cilksort/gtap_cilksort.cu:89-167:
+//
+//   cilksort(arr, n, tmp):  n <= LEAF        -> the workgroup sorts arr[0..n) in LDS
+//                           else four-way split len12 = n/2, len1 = len12/2, len2 = len12-len1,
+//                           len34 = n-len12, len3 = len34/2, len4 = len34-len3;
+//                           4 x task cilksort; taskwait;
+//                           task cilkmerge(arr,len1, arr+len1,len2, tmp);
+//                           task cilkmerge(arr+len12,len3, arr+len12+len3,len4, tmp+len12); taskwait;
+//                           task cilkmerge(tmp,len12, tmp+len12,len34, arr); taskwait
+//   cilkmerge(a,na,b,nb,dst): make na >= nb; nb == 0 -> copy;
+//                           na + nb <= MERGE_LEAF -> the workgroup merges in LDS tiles;
+//                           else a_split = (na+1)/2,
+//                                b_split = binary_search(b, nb, a[a_split-1]) + 1   (low = -1, :40-50)
+//                           task cilkmerge(a,a_split, b,b_split, dst);
+//                           task cilkmerge(a+a_split,na-a_split, b+b_split,nb-b_split,
+//                                          dst+a_split+b_split); taskwait
+//
+// Differences from the reference's thread-mode program: uint32 keys; the leaf is `n <= 4096`
+// keys sorted by a 256-thread workgroup (bitonic network in LDS) instead of `n < 64` keys by
+// one thread; merges below MERGE_LEAF keys are done by a workgroup with merge-path tiles
+// (4096 outputs per tile, staged through LDS) instead of one thread. The output is the
+// unique sorted permutation of the input whatever the schedule.
+//
+// All key traffic uses sc1 (agent-coherent) accesses: a merge may run on another XCD than
+// the tasks that wrote its inputs.
+#include "../device/gtap_block.hpp"
+#include "../host/gtap_launch.hpp"
+
+namespace gtap {
+namespace cilk {
+
+enum : uint32_t { F_SORT = 1, F_MERGE = 2 };
+constexpr int kTile = 4096;
+constexpr uint32_t kT = kBlockThreads;
+
+struct SortArg {
+    uint32_t* arr;
+    uint32_t* tmp;
+    int64_t n;
+};
+struct MergeArg {
+    const uint32_t* a;
+    const uint32_t* b;
+    uint32_t* dst;
+    int64_t na;
+    int64_t nb;
+};
+struct CilkArgs {
+    int64_t leaf;        // <= 4096
+    int64_t merge_leaf;  // merges of at most this many keys are done by one workgroup
+};
+
+struct Program {
+    static constexpr int kFrameWords = 10;
+    static constexpr uint32_t kMaxFunc = F_MERGE;
+    static constexpr uint32_t kFrameBytes = 40;
+    using Args = CilkArgs;
+    struct Shared {
+        uint32_t keys[kTile];
+        uint32_t out[kTile];
+        int64_t split[18];
+    };
+    template <class B>
+    __device__ static void run(B& blk);
+};
+
+__device__ __forceinline__ uint32_t ldk(const uint32_t* p) { return ld32(p); }
+__device__ __forceinline__ void stk(uint32_t* p, uint32_t v) { st32(p, v); }
+
+// Number of `a` elements among the first d outputs of merge(a, b) when a goes first on ties.
+template <class LDA, class LDB>
+__device__ __forceinline__ int64_t merge_path(LDA lda, int64_t na, LDB ldb, int64_t nb, int64_t d) {
+    int64_t lo = d > nb ? d - nb : 0;
+    int64_t hi = d < na ? d : na;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (lda(mid) <= ldb(d - 1 - mid)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// gtap_cilksort.cu:40-50: largest index with x[index] <= val, or -1.
+__device__ __forceinline__ int64_t binary_search_gm(const uint32_t* x, int64_t len, uint32_t val) {
+    int64_t low = -1, high = len;
+    while (low + 1 < high) {
+        const int64_t mid = (low + high) / 2;
+        if (ldk(x + mid) <= val) low = mid;
+        else high = mid;
+    }
+    return low;
+}
+
+// Workgroup sort of arr[0..n), n <= 4096: bitonic network over 4096 LDS slots (padded with
+// 0xFFFFFFFF, which sorts to the end and is never written back).
+template <class B>
+__device__ void leaf_sort(B& blk, uint32_t* arr, int64_t n) {
+    uint32_t* k = blk.sh->keys;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < (uint32_t)kTile; i += kT) k[i] = (int64_t)i < n ? ldk(arr + i) : 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t size = 2; size <= (uint32_t)kTile; size <<= 1) {
+        for (uint32_t j = size >> 1; j > 0; j >>= 1) {
+            for (uint32_t p = t; p < (uint32_t)kTile / 2; p += kT) {
+                const uint32_t lo = 2 * j * (p / j) + (p % j);
+                const uint32_t hi = lo + j;
+                const uint32_t x = k[lo], y = k[hi];
+                const bool up = (lo & size) == 0;
+                if ((x > y) == up) {
+                    k[lo] = y;
+                    k[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = t; (int64_t)i < n; i += kT) stk(arr + i, k[i]);
+}
+
+// Workgroup copy of n keys.
+template <class B>
+__device__ void block_copy(B& blk, const uint32_t* src, uint32_t* dst, int64_t n) {
+    (void)blk;
+    for (int64_t i = threadIdx.x; i < n; i += kT) stk(dst + i, ldk(src + i));
+}
+
+// Workgroup merge of a[0..na) and b[0..nb) into dst, tile by tile (merge path).
+template <class B>
+__device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t* b, int64_t nb, uint32_t* dst) {
+    typename Program::Shared* sh = blk.sh;
+    const uint32_t t = threadIdx.x;
+    const int64_t total = na + nb;
+    auto ga = [a](int64_t i) { return ldk(a + i); };
+    auto gb = [b](int64_t i) { return ldk(b + i); };
+    for (int64_t base = 0; base < total; base += (int64_t)kTile * 16) {
+        // splits of up to 16 tiles, found in parallel
+        const int64_t ntiles_here = ((total - base) + kTile - 1) / kTile;
+        const int64_t nsplit = (ntiles_here < 16 ? ntiles_here : 16) + 1;
+        if ((int64_t)t < nsplit) {
+            int64_t d = base + (int64_t)t * kTile;
+            if (d > total) d = total;
+            sh->split[t] = merge_path(ga, na, gb, nb, d);
+        }
+        __syncthreads();
+        for (int64_t ti = 0; ti + 1 < nsplit; ++ti) {
+            const int64_t d0 = base + ti * kTile;
+            const int64_t d1 = (d0 + kTile < total) ? d0 + kTile : total;
+            const int64_t i0 = sh->split[ti], i1 = sh->split[ti + 1];
+            const int64_t la = i1 - i0, lb = (d1 - i1) - (d0 - i0);
+            const int64_t j0 = d0 - i0;
+            for (int64_t i = t; i < la; i += kT) sh->keys[i] = ldk(a + i0 + i);
+            for (int64_t i = t; i < lb; i += kT) sh->keys[la + i] = ldk(b + j0 + i);
+            __syncthreads();
+            const uint32_t* A = sh->keys;
+            const uint32_t* Bv = sh->keys + la;
+            const int64_t len = d1 - d0;
+            constexpr int kPer = kTile / (int)kT;  // 16 outputs per thread
+            const int64_t od = (int64_t)t * kPer;
+            if (od < len) {
+                auto la_ = [A](int64_t i) { return A[i]; };
+                auto lb_ = [Bv](int64_t i) { return Bv[i]; };
+                int64_t i = merge_path(la_, la, lb_, lb, od);
+                int64_t j = od - i;
+                const int64_t oe = (od + kPer < len) ? od + kPer : len;
+                for (int64_t o = od; o < oe; ++o) {
+                    const bool takeA = i < la && (j >= lb || A[i] <= Bv[j]);
+                    sh->out[o] = takeA ? A[i] : Bv[j];
+                    if (takeA) ++i;
+                    else ++j;
+                }
+            }
+            __syncthreads();
+            for (int64_t o = t; o < len; o += kT) stk(dst + d0 + o, sh->out[o]);
+            __syncthreads();
+        }
+    }
+}
+
+template <class B>
+__device__ void sort_task(B& blk) {
+    const CilkArgs& g = *blk.args;
+    uint32_t* arr = blk.template arg<uint32_t*>(0);
+    uint32_t* tmp = blk.template arg<uint32_t*>(8);
+    const int64_t n = blk.template arg<int64_t>(16);
+    const int64_t len12 = n / 2, len1 = len12 / 2, len2 = len12 - len1;
+    const int64_t len34 = n - len12, len3 = len34 / 2, len4 = len34 - len3;
+    switch (blk.get_state()) {
+        case 0:
+            if (n <= g.leaf) {
+                leaf_sort(blk, arr, n);
+                blk.finish();
+                return;
+            }
+            if (threadIdx.x == 0) {
+                blk.spawn(F_SORT, 0, SortArg{arr, tmp, len1});
+                blk.spawn(F_SORT, 0, SortArg{arr + len1, tmp + len1, len2});
+                blk.spawn(F_SORT, 0, SortArg{arr + len12, tmp + len12, len3});
+                blk.spawn(F_SORT, 0, SortArg{arr + len12 + len3, tmp + len12 + len3, len4});
+            }
+            if (blk.taskwait(1)) return;
+            [[fallthrough]];
+        case 1:
+            if (threadIdx.x == 0) {
+                blk.spawn(F_MERGE, 0, MergeArg{arr, arr + len1, tmp, len1, len2});
+                blk.spawn(F_MERGE, 0, MergeArg{arr + len12, arr + len12 + len3, tmp + len12, len3, len4});
+            }
+            if (blk.taskwait(2)) return;
+            [[fallthrough]];
+        case 2:
+            if (threadIdx.x == 0) blk.spawn(F_MERGE, 0, MergeArg{tmp, tmp + len12, arr, len12, len34});
+            if (blk.taskwait(3)) return;
+            [[fallthrough]];
+        case 3:
+            blk.finish();
+            return;
+        default:
+            if (threadIdx.x == 0) GTAP_DEVICE_ERROR(*blk.dev, GTAP_ERROR_INVALID_TASKWAIT, blk.tid, 0, blk.state, 3);
+            blk.finish();
+            return;
+    }
+}
+
+template <class B>
+__device__ void merge_task(B& blk) {
+    const CilkArgs& g = *blk.args;
+    const uint32_t* a = blk.template arg<const uint32_t*>(0);
+    const uint32_t* b = blk.template arg<const uint32_t*>(8);
+    uint32_t* dst = blk.template arg<uint32_t*>(16);
+    int64_t na = blk.template arg<int64_t>(24);
+    int64_t nb = blk.template arg<int64_t>(32);
+    if (blk.get_state() != 0) {
+        blk.finish();
+        return;
+    }
+    if (na < nb) {
+        const uint32_t* tp = a;
+        a = b;
+        b = tp;
+        const int64_t tn = na;
+        na = nb;
+        nb = tn;
+    }
+    if (nb == 0) {
+        block_copy(blk, a, dst, na);
+        blk.finish();
+        return;
+    }
+    if (na + nb <= g.merge_leaf) {
+        leaf_merge(blk, a, na, b, nb, dst);
+        blk.finish();
+        return;
+    }
+    if (threadIdx.x == 0) {
+        const int64_t a_split = (na + 1) / 2;
+        const int64_t b_split = binary_search_gm(b, nb, ldk(a + a_split - 1)) + 1;
+        blk.spawn(F_MERGE, 0, MergeArg{a, b, dst, a_split, b_split});
+        blk.spawn(F_MERGE, 0, MergeArg{a + a_split, b + b_split, dst + a_split + b_split, na - a_split, nb - b_split});
+    }
+    if (blk.taskwait(1)) return;
+    blk.finish();
+}
+
+template <class B>
+__device__ void Program::run(B& blk) {
+    if (blk.func == F_SORT) sort_task(blk);
+    else merge_task(blk);
+}
+
+}  // namespace cilk
+}  // namespace gtap
+
+using CilkProgram = gtap::cilk::Program;
+static_assert(CilkProgram::kFrameBytes + gtap::kHdrBytes <= gtap::host::kBuiltinStride, "frame exceeds stride");
+static_assert(CilkProgram::kFrameWords <= 16, "block frame cache holds 16 words");
+
+// Sorts n uint32 keys in place (device buffers keys and tmp, n elements each).
+// leaf <= 4096 keys are sorted by one workgroup in LDS (0 -> 4096); merges of at most
+// merge_leaf keys are done by one workgroup (0 -> 65536).
+// Replaces the program of evaluation/2-comparison/cilksort/gtap_cilksort.cu:169-281.
+extern "C" int gtap_cilksort(uint32_t* keys, uint32_t* tmp, int64_t n, int64_t leaf, int64_t merge_leaf,
+                             gtap_run_stats* stats) {
+    using namespace gtap;
+    if (!keys || !tmp || n < 0) return host::invalid_argument("gtap_cilksort: bad arguments");
+    if (leaf <= 0) leaf = 4096;
+    if (merge_leaf <= 0) merge_leaf = 65536;
+    if (leaf > cilk::kTile) return host::invalid_argument("gtap_cilksort: leaf must be <= 4096");
+    if (merge_leaf < 3) merge_leaf = 3;
+    if (n == 0) {
+        if (stats) memset(stats, 0, sizeof(*stats));
+        return 0;
+    }
+    cilk::SortArg root{keys, tmp, n};
+    gtap_job job = host::make_job(cilk::F_SORT, 0, &root, sizeof(root));
+    return host::run_single_block_job<CilkProgram>(cilk::CilkArgs{leaf, merge_leaf}, job, nullptr, stats);
+}
+
+// Host-buffer convenience: copies keys in, sorts, copies the sorted keys back.
+extern "C" int gtap_cilksort_host(uint32_t* keys, int64_t n, int64_t leaf, int64_t merge_leaf, gtap_run_stats* stats) {
+    if (n < 0 || (n > 0 && !keys)) return gtap::host::invalid_argument("gtap_cilksort_host");
+    if (n == 0) return gtap_cilksort(reinterpret_cast<uint32_t*>(16), reinterpret_cast<uint32_t*>(16), 0, leaf,
+                                     merge_leaf, stats);
+    uint32_t *dk = nullptr, *dt = nullptr;
+    GTAP_HIP_TRY(hipMalloc(&dk, (size_t)n * 4));
+    GTAP_HIP_TRY(hipMalloc(&dt, (size_t)n * 4));
+    int st = 0;
+    hipError_t e = hipMemcpy(dk, keys, (size_t)n * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) st = gtap::host::hip_status(e, "gtap_cilksort_host copy-in");
+    if (!st) st = gtap_cilksort(dk, dt, n, leaf, merge_leaf, stats);
+    if (!st) {
+        e = hipMemcpy(keys, dk, (size_t)n * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = gtap::host::hip_status(e, "gtap_cilksort_host copy-out");
+    }
+    (void)hipFree(dk);
+    (void)hipFree(dt);
+    return st;
+}

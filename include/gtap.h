@@ -104,6 +104,15 @@ int gtap_bfs(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne, int
 int gtap_bfs_host(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne, int32_t source,
                   int32_t* depth_out, int32_t num_queues, int32_t split, gtap_run_stats* stats);
 
+/* Cilksort of n uint32 keys in place, block-cooperative (one task per 256-thread
+ * workgroup): keys and tmp are device buffers of n keys; leaves of <= leaf keys (0 -> 4096,
+ * max 4096) are sorted in LDS by one workgroup; merges of <= merge_leaf keys (0 -> 65536)
+ * are merged by one workgroup. Replaces evaluation/2-comparison/cilksort/gtap_cilksort.cu:89-281. */
+int gtap_cilksort(uint32_t* keys, uint32_t* tmp, int64_t n, int64_t leaf, int64_t merge_leaf,
+                  gtap_run_stats* stats);
+/* The same with a host buffer (copied in and out around the launch). */
+int gtap_cilksort_host(uint32_t* keys, int64_t n, int64_t leaf, int64_t merge_leaf, gtap_run_stats* stats);
+
 /* ------------------------------------------------------------ input generators (host) */
 /* R-MAT edges, CSR by stable counting sort, and the gen_vector key stream
  * (evaluation/util/gen_vector.c:8-18; read format examples/asynchronous_bfs/
