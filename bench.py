@@ -29,6 +29,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 NQ_N = 14
+# Published counts used as the check (OEIS A000170; the task tree of one task per legal
+# placement; F(30) and the 2F(31)-1 tasks of the fib tree).
+NQ14_SOLUTIONS = 365_596
+NQ14_TASKS = 27_358_553
+FIB30 = 832_040
+FIB30_TASKS = 2_692_537
 
 
 def log(*a):
@@ -61,6 +67,76 @@ def cpu_baseline(nq_tasks: int) -> dict:
             "seconds": t}
 
 
+def bench_cilksort(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
+    """cilksort of 2^26 gen_vector keys (block mode), inputs resident on the GPU; the
+    unsorted keys are restored with a device copy before each run (outside the kernel time)."""
+    import ctypes
+    import numpy as np
+    n = 1 << 26
+    keys = gtap_amd.gen_keys(n)
+    src = gtap_amd.DeviceBuffer.from_array(keys)
+    work = gtap_amd.DeviceBuffer(keys.nbytes)
+    tmp = gtap_amd.DeviceBuffer(keys.nbytes)
+    L = gtap_amd.lib()
+    ms = []
+    for _ in range(reps + 1):
+        L.gtap_memcpy_dtod(work.ptr, src.ptr, keys.nbytes)
+        s = gtap_amd.RunStats()
+        st = L.gtap_cilksort(work.ptr, tmp.ptr, n, 4096, 0, ctypes.byref(s))
+        if st != 0:
+            raise RuntimeError(f"gtap_cilksort failed: {st}")
+        ms.append(s.kernel_ms)
+    out = work.to_array(np.empty(n, dtype=np.uint32))
+    ok = bool(np.array_equal(out, np.sort(keys)))
+    for b in (src, work, tmp):
+        b.free()
+    med = statistics.median(ms[1:])
+    gbps = roofline.cilksort_bytes(n) / (med / 1e3) / 1e9
+    log(f"cilksort 2^26: {med:.3f} ms, {gbps:.0f} GB/s algorithmic, sorted={ok}")
+    return {"kernel_ms_median": med, "keys_per_s": n / (med / 1e3), "bit_exact": ok,
+            "tasks_per_s": s.tasks_executed / (med / 1e3),
+            "roofline": {"bound": "hbm", "achieved": gbps, "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
+                         "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas}}
+
+
+def bench_bfs(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
+    """Asynchronous BFS on the R-MAT scale-22 graph (edge factor 16, seed 1), EPAQ with 4
+    degree-bucket queues vs one queue; CSR resident on the GPU."""
+    import ctypes
+    import numpy as np
+    row, col = gtap_amd.rmat_csr(22, 16, 1)
+    nv = row.size - 1
+    drow = gtap_amd.DeviceBuffer.from_array(row)
+    dcol = gtap_amd.DeviceBuffer.from_array(col)
+    ddep = gtap_amd.DeviceBuffer(nv * 4)
+    L = gtap_amd.lib()
+    res = {}
+    depth = np.empty(nv, dtype=np.int32)
+    for q in (4, 1):
+        ms = []
+        for _ in range(reps + 1):
+            s = gtap_amd.RunStats()
+            st = L.gtap_bfs(drow.ptr, dcol.ptr, nv, col.size, 0, ddep.ptr, q, 64, ctypes.byref(s))
+            if st != 0:
+                raise RuntimeError(f"gtap_bfs failed: {st}")
+            ms.append(s.kernel_ms)
+        res[q] = (statistics.median(ms[1:]), s.tasks_executed)
+    ddep.to_array(depth)
+    for b in (drow, dcol, ddep):
+        b.free()
+    reached = depth < gtap_amd.INF_DEPTH
+    vr = int(reached.sum())
+    er = int(np.diff(row)[reached].sum())
+    med4, tasks4 = res[4]
+    gbps = roofline.bfs_bytes(vr, er) / (med4 / 1e3) / 1e9
+    log(f"bfs rmat22: epaq4 {med4:.3f} ms, 1 queue {res[1][0]:.3f} ms, {gbps:.0f} GB/s algorithmic")
+    return {"kernel_ms_median_epaq4": med4, "kernel_ms_median_1queue": res[1][0],
+            "epaq_speedup": res[1][0] / med4, "reached_vertices": vr, "reached_edges": er,
+            "edges_per_s": er / (med4 / 1e3), "tasks_executed_epaq4": tasks4,
+            "roofline": {"bound": "hbm", "achieved": gbps, "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
+                         "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas}}
+
+
 def load_traffic(name: str):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
     p = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
@@ -86,8 +162,7 @@ def main() -> int:
     import gtap_amd
     from gtap_amd import roofline
 
-    import oracle
-    nq_solutions, nq_tasks = oracle.nq(NQ_N)
+    nq_solutions, nq_tasks = NQ14_SOLUTIONS, NQ14_TASKS
 
     gtap_amd.gtap_set_config(timeout_ms=30000)
     st = gtap_amd.gtap_initialize()
@@ -108,8 +183,9 @@ def main() -> int:
     for i in range(args.steps):
         v, s = gtap_amd.nqueens(NQ_N)
         kernel_ms.append(s["kernel_ms"])
-        if v != nq_solutions:
-            raise RuntimeError(f"nqueens({NQ_N}) returned {v}, expected {nq_solutions}")
+        if v != nq_solutions or s["tasks_spawned"] + 1 != nq_tasks:
+            raise RuntimeError(f"nqueens({NQ_N}) returned {v} with {s['tasks_spawned'] + 1} tasks, "
+                               f"expected {nq_solutions} with {nq_tasks}")
     wall = time.perf_counter() - t0
     workers = s["workers"]
     log(f"timed {args.steps} steps: wall {wall*1e3:.3f} ms, kernel median {statistics.median(kernel_ms):.3f} ms")
@@ -121,13 +197,17 @@ def main() -> int:
     atomics_per_s = roofline.ATOMICS_PER_TASK * (nq_tasks - 1) / avg_kernel_s
 
     # fib(30): the other scheduler-bound config
-    fib_tasks = oracle.fib_tasks(30)
+    fib_tasks = FIB30_TASKS
     fib_ms = []
     for _ in range(5):
         fv, fs = gtap_amd.fib(30)
-        assert fv == 832040
+        if fv != FIB30 or fs["tasks_spawned"] + 1 != fib_tasks:
+            raise RuntimeError(f"fib(30) returned {fv} with {fs['tasks_spawned'] + 1} tasks")
         fib_ms.append(fs["kernel_ms"])
     fib_med = statistics.median(fib_ms)
+
+    extra_sort = bench_cilksort(gtap_amd, roofline, hbm_peak_meas)
+    extra_bfs = bench_bfs(gtap_amd, roofline, hbm_peak_meas)
 
     cpu = None if args.no_cpu else cpu_baseline(nq_tasks)
     gtap_amd.gtap_finalize()
@@ -173,6 +253,8 @@ def main() -> int:
             "hbm_measured_copy_gbps": hbm_peak_meas,
             "fib30": {"kernel_ms_median": fib_med, "tasks": fib_tasks,
                       "tasks_per_s": fib_tasks / (fib_med / 1e3)},
+            "cilksort_2e26": extra_sort,
+            "bfs_rmat22": extra_bfs,
             "device": info,
         },
     }

@@ -148,6 +148,15 @@ SIGNATURES: dict[str, tuple[list, object]] = {
                        ctypes.POINTER(RunStats)], _i),
     "gtap_cilksort": ([_vp, _vp, _i64, _i64, _i64, ctypes.POINTER(RunStats)], _i),
     "gtap_cilksort_host": ([_vp, _i64, _i64, _i64, ctypes.POINTER(RunStats)], _i),
+    "gtap_jobs_generate": ([_u64, ctypes.c_int32, _vp, _vp, _vp], _i),
+    "gtap_stream_create": ([_u64, ctypes.c_int32, ctypes.POINTER(_vp)], _i),
+    "gtap_stream_destroy": ([_vp], _i),
+    "gtap_stream_info": ([_vp, _vp, _vp, _vp, ctypes.POINTER(_i64)], _i),
+    "gtap_stream_sorted_keys": ([_vp, ctypes.c_int32, _vp], _i),
+    "gtap_stream_run_batched": ([_vp, _vp, _vp, ctypes.POINTER(RunStats)], _i),
+    "gtap_stream_run_per_job": ([_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)], _i),
+    "gtap_stream_serve": ([_vp, ctypes.c_double, _u64, ctypes.c_int32, ctypes.c_int32, _vp, _vp,
+                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(RunStats)], _i),
     "gtap_rmat_edges": ([_i, _i64, _u64, _u32, _u32, _u32, _vp, _vp], _i),
     "gtap_csr_from_edges": ([ctypes.c_int32, _i64, _vp, _vp, _vp, _vp], _i),
     "gtap_rmat_csr": ([_i, _i, _u64, _u32, _u32, _u32, _vp, _vp], _i),
@@ -341,3 +350,67 @@ def cilksort(keys, leaf: int = 4096, merge_leaf: int = 65536):
     _check(lib().gtap_cilksort_host(a.ctypes.data if a.size else None, a.size, leaf, merge_leaf, ctypes.byref(s)),
            "gtap_cilksort_host")
     return a, s.as_dict()
+
+
+# ---------------------------------------------------------------- job streams
+
+def jobs_generate(seed: int, njobs: int):
+    """(kind, param, key_seed) arrays of the mixed job stream (the library's generator)."""
+    import numpy as np
+    kind = np.empty(njobs, dtype=np.int32)
+    param = np.empty(njobs, dtype=np.int32)
+    kseed = np.empty(njobs, dtype=np.uint64)
+    _check(lib().gtap_jobs_generate(seed, njobs, kind.ctypes.data, param.ctypes.data, kseed.ctypes.data),
+           "gtap_jobs_generate")
+    return kind, param, kseed
+
+
+class Stream:
+    """A generated job stream with device-resident sort inputs (see gtap_stream_* in gtap.h)."""
+
+    def __init__(self, seed: int, njobs: int):
+        h = ctypes.c_void_p(0)
+        _check(lib().gtap_stream_create(seed, njobs, ctypes.byref(h)), "gtap_stream_create")
+        self.h = h.value
+        self.njobs = njobs
+        self.kind, self.param, self.key_seed = jobs_generate(seed, njobs)
+
+    def close(self):
+        if self.h:
+            lib().gtap_stream_destroy(self.h)
+            self.h = None
+
+    def sorted_keys(self, job: int):
+        import numpy as np
+        out = np.empty(int(self.param[job]), dtype=np.uint32)
+        _check(lib().gtap_stream_sorted_keys(self.h, job, out.ctypes.data), "gtap_stream_sorted_keys")
+        return out
+
+    def run_batched(self):
+        import numpy as np
+        res = np.empty(self.njobs, dtype=np.int64)
+        lat = np.empty(self.njobs, dtype=np.float64)
+        s = RunStats()
+        _check(lib().gtap_stream_run_batched(self.h, res.ctypes.data, lat.ctypes.data, ctypes.byref(s)),
+               "gtap_stream_run_batched")
+        return res, lat, s.as_dict()
+
+    def run_per_job(self):
+        import numpy as np
+        res = np.empty(self.njobs, dtype=np.int64)
+        lat = np.empty(self.njobs, dtype=np.float64)
+        tot = ctypes.c_double(0)
+        _check(lib().gtap_stream_run_per_job(self.h, res.ctypes.data, lat.ctypes.data, ctypes.byref(tot)),
+               "gtap_stream_run_per_job")
+        return res, lat, tot.value
+
+    def serve(self, rate: float, arrival_seed: int = 1, per_job_launch: bool = False, max_live: int = 0):
+        import numpy as np
+        res = np.empty(self.njobs, dtype=np.int64)
+        lat = np.empty(self.njobs, dtype=np.float64)
+        tot = ctypes.c_double(0)
+        s = RunStats()
+        _check(lib().gtap_stream_serve(self.h, rate, arrival_seed, 1 if per_job_launch else 0, max_live,
+                                       res.ctypes.data, lat.ctypes.data, ctypes.byref(tot), ctypes.byref(s)),
+               "gtap_stream_serve")
+        return res, lat, tot.value, s.as_dict()

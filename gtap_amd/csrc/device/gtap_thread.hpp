@@ -271,6 +271,41 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtx<Q>* w, 
     }
 }
 
+// ------------------------------------------------------------------ spawn
+
+// Allocate a task id, write the child's header and arguments, append it to the private end
+// of queue q. Called by every lane that spawns (wave-aggregated inside).
+template <class P, typename A>
+__device__ __forceinline__ uint32_t spawn_record(const GtapDev& dev, WaveCtx<P::kQueues>* w, uint32_t me,
+                                                 uint32_t self, uint32_t f, uint32_t q, uint32_t par, uint32_t roff,
+                                                 const A& a) {
+    static_assert(sizeof(A) % 8 == 0, "argument blocks are padded to 8 bytes");
+    static_assert(sizeof(A) / 4 <= P::kFrameWords, "arguments must fit the loaded frame");
+    if (q >= (uint32_t)P::kQueues) {
+        GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_QUEUE_IDX, self, q, q, P::kQueues);
+        q = 0;
+    }
+    const uint32_t id = alloc_id(dev, w, me);
+    uint8_t* crec = rec_ptr(dev, id);
+    st64(reinterpret_cast<uint64_t*>(crec), (uint64_t)hdr_word0(f, 0, q) | ((uint64_t)par << 32));
+    st64(reinterpret_cast<uint64_t*>(crec + 8), ((uint64_t)roff << 32));
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&a);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(crec + kHdrBytes);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(A) / 8); ++i) st64(dst + i, src[i]);
+    queue_append(dev, w, me, q, id);
+    const uint64_t am = active_mask();
+    if (rank_in(am) == 0) vst(&w->spawns, vld(&w->spawns) + (uint32_t)popc(am));
+    return id;
+}
+
+// Memory ordering of LDS between the lanes of one wave (wave-cooperative tasks).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------------ per-lane task context
 
 template <class P>
@@ -324,24 +359,8 @@ struct Lane {
 
     template <typename A>
     __device__ __forceinline__ uint32_t spawn_impl(uint32_t f, uint32_t q, uint32_t par, uint32_t roff, const A& a) {
-        static_assert(sizeof(A) % 8 == 0, "argument blocks are padded to 8 bytes");
-        static_assert(sizeof(A) / 4 <= P::kFrameWords, "arguments must fit the loaded frame");
-        if (q >= (uint32_t)Q) {
-            GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_QUEUE_IDX, tid, q, q, Q);
-            q = 0;
-        }
-        const uint32_t id = alloc_id(*dev, w, me);
-        uint8_t* crec = rec_ptr(*dev, id);
-        st64(reinterpret_cast<uint64_t*>(crec), (uint64_t)hdr_word0(f, 0, q) | ((uint64_t)par << 32));
-        st64(reinterpret_cast<uint64_t*>(crec + 8), ((uint64_t)roff << 32));
-        const uint64_t* src = reinterpret_cast<const uint64_t*>(&a);
-        uint64_t* dst = reinterpret_cast<uint64_t*>(crec + kHdrBytes);
-#pragma unroll
-        for (int i = 0; i < (int)(sizeof(A) / 8); ++i) st64(dst + i, src[i]);
-        queue_append(*dev, w, me, q, id);
+        const uint32_t id = spawn_record<P>(*dev, w, me, tid, f, q, par, roff, a);
         if (par != kNoParent) child_count += 1;
-        const uint64_t am = active_mask();
-        if (rank_in(am) == 0) vst(&w->spawns, vld(&w->spawns) + (uint32_t)popc(am));
         return id;
     }
 
@@ -379,6 +398,132 @@ struct Lane {
         outcome = OUT_FINISHED;
     }
 };
+
+// ------------------------------------------------------------------ wave-cooperative tasks
+// A program may name one EPAQ queue (P::kWaveQueue >= 0) whose tasks run on a whole
+// wavefront: a batch taken from that queue holds one task and all 64 lanes enter
+// P::run_wave with the same task. Spawn, taskwait and return are called by all lanes with
+// uniform arguments; lane 0 does the bookkeeping.
+template <class P>
+struct WaveTask {
+    static constexpr int Q = P::kQueues;
+    const GtapDev* dev;
+    WaveCtx<Q>* w;
+    const typename P::Args* args;
+    typename P::WaveShared* sh;
+    uint8_t* rec;
+    uint32_t me, tid, func, state, queue, parent, res_off, child_count, outcome;
+    int lane;
+    uint32_t frame[P::kFrameWords];
+
+    __device__ __forceinline__ uint32_t get_state() const { return state; }
+    template <typename T>
+    __device__ __forceinline__ T arg(int byte_off) const {
+        if constexpr (sizeof(T) == 4) {
+            return __builtin_bit_cast(T, frame[byte_off / 4]);
+        } else {
+            const uint64_t v = (uint64_t)frame[byte_off / 4] | ((uint64_t)frame[byte_off / 4 + 1] << 32);
+            return __builtin_bit_cast(T, v);
+        }
+    }
+    template <typename T>
+    __device__ __forceinline__ T frame_ld(int byte_off) const { return ldT<T>(rec + kHdrBytes + byte_off); }
+    template <typename T>
+    __device__ __forceinline__ void frame_st(int byte_off, T v) {
+        if (lane == 0) stT<T>(rec + kHdrBytes + byte_off, v);
+    }
+    template <typename A>
+    __device__ __forceinline__ void spawn(uint32_t f, uint32_t q, int res_off_in_frame, const A& a) {
+        if (lane == 0) spawn_record<P>(*dev, w, me, tid, f, q, tid, kHdrBytes + (uint32_t)res_off_in_frame, a);
+        child_count += 1;
+    }
+    template <typename A>
+    __device__ __forceinline__ void spawn_detached(uint32_t f, uint32_t q, const A& a) {
+        if (lane == 0) spawn_record<P>(*dev, w, me, tid, f, q, kNoParent, 0u, a);
+    }
+    __device__ __forceinline__ bool taskwait(uint32_t next_state, uint32_t q_after) {
+        if (q_after >= (uint32_t)Q) q_after = 0;
+        if (child_count == 0) {
+            state = next_state;
+            return false;
+        }
+        if (lane == 0) {
+            st32(reinterpret_cast<uint32_t*>(rec), hdr_word0(func, next_state, q_after));
+            st32(reinterpret_cast<uint32_t*>(rec + 8), child_count);
+        }
+        child_count = 0;
+        outcome = OUT_SUSPENDED;
+        return true;
+    }
+    template <typename T>
+    __device__ __forceinline__ void ret(T v) {
+        if (lane == 0 && parent != kNoParent) stT<T>(rec_ptr(*dev, parent) + res_off, v);
+        finish();
+    }
+    __device__ __forceinline__ void finish() {
+        if (child_count != 0 && lane == 0) GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_TASKWAIT, tid, -1, child_count, 0);
+        outcome = OUT_FINISHED;
+    }
+};
+
+template <class P, class = void>
+struct HasWaveQueue {
+    static constexpr int value = -1;
+};
+template <class P>
+struct HasWaveQueue<P, decltype((void)P::kWaveQueue)> {
+    static constexpr int value = P::kWaveQueue;
+};
+
+// Runs the wave task `tid` on all lanes; returns its outcome (uniform).
+template <class P>
+__device__ __forceinline__ uint32_t run_wave_task(const GtapDev& dev, const typename P::Args& args,
+                                                  WaveCtx<P::kQueues>* w, void* wave_shared, uint32_t me,
+                                                  uint32_t tid, uint32_t& parent_out) {
+    if constexpr (HasWaveQueue<P>::value >= 0) {
+        WaveTask<P> W;
+        W.dev = &dev;
+        W.w = w;
+        W.args = &args;
+        W.sh = reinterpret_cast<typename P::WaveShared*>(wave_shared);
+        W.me = me;
+        W.lane = lane_id();
+        W.tid = tid;
+        W.rec = rec_ptr(dev, tid);
+        const uint64_t h0 = ld64(reinterpret_cast<const uint64_t*>(W.rec));
+        const uint64_t h1 = ld64(reinterpret_cast<const uint64_t*>(W.rec + 8));
+        W.func = hdr_func((uint32_t)h0);
+        W.state = hdr_state((uint32_t)h0);
+        W.queue = hdr_queue((uint32_t)h0);
+        W.parent = (uint32_t)(h0 >> 32);
+        W.res_off = (uint32_t)(h1 >> 32);
+        W.child_count = 0;
+        W.outcome = OUT_RUNNING;
+        const uint64_t* fp = reinterpret_cast<const uint64_t*>(W.rec + kHdrBytes);
+#pragma unroll
+        for (int i = 0; i < P::kFrameWords / 2; ++i) {
+            const uint64_t v = ld64(fp + i);
+            W.frame[2 * i] = (uint32_t)v;
+            W.frame[2 * i + 1] = (uint32_t)(v >> 32);
+        }
+        if (W.func == 0 || W.func > P::kMaxFunc) {
+            if (W.lane == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, tid, W.queue, W.func, P::kMaxFunc);
+            W.outcome = OUT_FINISHED;
+        } else {
+            P::run_wave(W);
+        }
+        if (W.outcome == OUT_RUNNING) {
+            if (W.lane == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, tid, W.queue, W.state, 0);
+            W.outcome = OUT_FINISHED;
+        }
+        parent_out = W.parent;
+        return W.outcome;
+    } else {
+        (void)dev; (void)args; (void)w; (void)wave_shared; (void)me; (void)tid;
+        parent_out = kNoParent;
+        return OUT_FINISHED;
+    }
+}
 
 // ------------------------------------------------------------------ scheduler
 
@@ -500,6 +645,7 @@ __device__ __forceinline__ void complete_job(const GtapDev& dev, uint32_t anchor
 template <class P>
 __device__ __forceinline__ uint32_t try_steal(const GtapDev& dev, uint32_t me, uint32_t q, uint32_t& rng,
                                               uint32_t& tid_out, bool& idle_registered, LoopStats& st) {
+    const uint32_t batch_cap = ((int)q == HasWaveQueue<P>::value) ? 1u : dev.steal_batch;
     const int lane = lane_id();
     if (dev.nworkers < 2) return 0;
     uint32_t victim = 0;
@@ -521,7 +667,7 @@ __device__ __forceinline__ uint32_t try_steal(const GtapDev& dev, uint32_t me, u
     wv = shfl(wv, src);
     sz = shfl(sz, src);
     uint32_t k = (uint32_t)(sz + 1) / 2;
-    if (k > dev.steal_batch) k = dev.steal_batch;
+    if (k > batch_cap) k = batch_cap;
     if (k == 0) k = 1;
     if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
         if (lane == 0) atomic_add(dev.ctl + CTL_IDLE * 32, -1);
@@ -554,7 +700,8 @@ __device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtx<P::kQueu
         const uint32_t t = (uint32_t)(wv >> 32), b = (uint32_t)wv;
         const int32_t sz = (int32_t)(b - t);
         if (sz <= 0) return 0;
-        const uint32_t k = sz < kWave ? (uint32_t)sz : (uint32_t)kWave;
+        const uint32_t cap = ((int)q == HasWaveQueue<P>::value) ? 1u : (uint32_t)kWave;
+        const uint32_t k = (uint32_t)sz < cap ? (uint32_t)sz : cap;
         int ok = 0;
         if (lane == 0) {
             uint64_t expected = wv;
@@ -585,8 +732,10 @@ __device__ __forceinline__ void backoff_sleep(uint32_t level) {
 }
 
 template <class P>
-__device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& args, WaveCtx<P::kQueues>* w) {
+__device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& args, WaveCtx<P::kQueues>* w,
+                                 void* wave_shared) {
     constexpr int Q = P::kQueues;
+    constexpr int WQ = HasWaveQueue<P>::value;
     const int lane = lane_id();
     const uint32_t me = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const uint64_t t_begin = now_ticks();
@@ -616,6 +765,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     if (rng == 0) rng = 1;
     uint32_t n = 0;        // tasks in the current batch (wave-uniform)
     uint32_t my_tid = 0;   // lane's task when lane < n
+    int batch_q = -1;      // queue the batch came from (-1: admitted job roots)
     bool idle_registered = false;
     uint32_t idle_iter = 0;
     uint32_t busy_iter = 0;
@@ -632,17 +782,20 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 if (n) {
                     vst(&w->cur_queue, q);
                     st.pops += 1;
+                    batch_q = (int)q;
                 }
             }
             if (n == 0 && dev.jobs != nullptr) {
                 n = admit_jobs<P>(dev, w, me, my_tid, idle_registered);
                 st.jobs += n;
+                batch_q = -1;
             }
             if (n == 0) {
 #pragma unroll
                 for (int i = 0; i < Q && n == 0; ++i) {
                     const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
                     n = try_steal<P>(dev, me, q, rng, my_tid, idle_registered, st);
+                    if (n) batch_q = (int)q;
                 }
             }
             if (n == 0) {
@@ -692,7 +845,19 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         st.tasks += n;
         uint32_t fin_tid = 0, fin_parent = kNoParent;
         bool fin = false;
-        if ((uint32_t)lane < n) {
+        if (WQ >= 0 && batch_q == WQ) {
+            // one wave-cooperative task on all lanes
+            uint32_t wt = uniform(my_tid);
+            if (wt == 0u || wt >= dev.nrecs) {
+                if (lane == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, wt, -1, wt, dev.nrecs);
+                wt = 0u;
+            }
+            uint32_t par = kNoParent;
+            const uint32_t outc = (wt != 0u) ? run_wave_task<P>(dev, args, w, wave_shared, me, wt, par) : OUT_FINISHED;
+            fin = (lane == 0) && outc == OUT_FINISHED;
+            fin_tid = wt;
+            fin_parent = (wt != 0u && par < dev.nrecs + dev.nanchors) ? par : kNoParent;
+        } else if ((uint32_t)lane < n) {
             if (my_tid == 0u || my_tid >= dev.nrecs) {
                 // a slot never holds the sink or an anchor id; treat as a protocol error
                 GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, my_tid, -1, my_tid, dev.nrecs);
@@ -748,7 +913,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 kmax = (uint32_t)q;
             }
         }
-        const uint32_t keep = cmax < (uint32_t)kWave ? cmax : (uint32_t)kWave;
+        const uint32_t keep_cap = ((int)kmax == WQ) ? 1u : (uint32_t)kWave;
+        const uint32_t keep = cmax < keep_cap ? cmax : keep_cap;
         uint32_t next_tid = 0;
         if (keep) {
             const uint32_t t = uniform(vld(&w->tail[kmax]));
@@ -775,6 +941,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         }
         n = keep;
         my_tid = next_tid;
+        batch_q = (int)kmax;
 
         busy_iter += 1;
         if ((busy_iter & 63u) == 0) {
@@ -806,10 +973,22 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
 }
 
 // Launch shape: 256-thread workgroups (4 waves), one WaveCtx per wave in LDS.
+template <class P, class = void>
+struct WaveSharedOf {
+    struct type {
+        uint32_t unused;
+    };
+};
+template <class P>
+struct WaveSharedOf<P, decltype((void)sizeof(typename P::WaveShared))> {
+    using type = typename P::WaveShared;
+};
+
 template <class P>
 __global__ __launch_bounds__(256) void thread_kernel(GtapDev dev, typename P::Args args) {
     __shared__ WaveCtx<P::kQueues> ctxs[4];
-    thread_scheduler<P>(dev, args, &ctxs[threadIdx.x / kWave]);
+    __shared__ typename WaveSharedOf<P>::type wsh[4];
+    thread_scheduler<P>(dev, args, &ctxs[threadIdx.x / kWave], &wsh[threadIdx.x / kWave]);
 }
 
 }  // namespace gtap

@@ -1,0 +1,715 @@
+// prog_stream.hip -- the mixed job stream: fib, n-queens and mergesort jobs in one resident
+// persistent kernel (batched: all jobs in one launch; served: jobs streamed in by a host
+// loop while the kernel runs), and the plain form, one launch per job.
+//
+// Task program (two EPAQ queues: queue 0 = per-lane tasks, queue 1 = wave-cooperative tasks):
+//   fib(n)                       lane  (evaluation/2-comparison/fib/gtap_fib.cu:10-22, int64)
+//   nq(row, cols, diag, anti, N) lane  (evaluation/2-comparison/nq/gtap_nq.cu:25-55 shape,
+//                                       taskwait + int64 sum, N per job)
+//   sort_root(arr, tmp, n)       lane  spawns msort on queue 1, taskwait, returns n
+//   msort(a, t, n, into_t)       wave  2-way mergesort of examples/mergesort/mergesort.cu:34-50:
+//                                      n <= 1024 -> the wave sorts in LDS; else two msort
+//                                      children (ping-pong buffers), taskwait, then merge
+//   merge(a, na, b, nb, dst)     wave  cilkmerge split of gtap_cilksort.cu:89-127 down to
+//                                      16384 keys, merged by the wave with merge-path tiles
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../device/gtap_abi.hpp"
+#include "../host/gtap_launch.hpp"
+
+namespace gtap {
+namespace stream {
+
+enum : uint32_t { F_FIB = 1, F_NQ = 2, F_SROOT = 3, F_MSORT = 4, F_MERGE = 5 };
+enum : uint32_t { Q_LANE = 0, Q_WAVE = 1 };
+constexpr int kWaveLeaf = 1024;       // keys sorted by one wave in LDS
+constexpr int kWaveMergeLeaf = 16384; // keys merged by one wave
+
+struct FibArg {
+    int32_t n;
+    int32_t pad;
+};
+struct NqArg {
+    int32_t row;
+    uint32_t cols;
+    uint32_t diag;
+    uint32_t anti;
+    int32_t n;
+    int32_t pad;
+};
+struct SortRootArg {
+    uint32_t* arr;
+    uint32_t* tmp;
+    int64_t n;
+};
+struct MsortArg {
+    uint32_t* a;
+    uint32_t* t;
+    int64_t n;
+    int32_t into_t;
+    int32_t pad;
+};
+struct MergeArg {
+    const uint32_t* a;
+    const uint32_t* b;
+    uint32_t* dst;
+    int64_t na;
+    int64_t nb;
+};
+constexpr int kFibA = 8, kFibB = 16;
+constexpr int kNqK = 24, kNqCounts = 32;
+constexpr int kDummySlot = 40;  // result slot of void children (never read)
+
+struct StreamArgs {
+    int32_t unused;
+};
+
+struct Program {
+    static constexpr int kQueues = 2;
+    static constexpr int kWaveQueue = Q_WAVE;
+    static constexpr int kFrameWords = 10;
+    static constexpr uint32_t kMaxFunc = F_MERGE;
+    static constexpr uint32_t kFrameBytes = kNqCounts + 8 * 16;
+    using Args = StreamArgs;
+    struct WaveShared {
+        uint32_t keys[kWaveLeaf];
+        int64_t split[18];
+    };
+    template <class L>
+    __device__ static void run(L& lane);
+    template <class W>
+    __device__ static void run_wave(W& wt);
+};
+
+// ---------------------------------------------------------------- lane tasks
+
+template <class L>
+__device__ __forceinline__ void fib_task(L& lane) {
+    switch (lane.get_state()) {
+        case 0: {
+            const int32_t n = lane.template arg<int32_t>(0);
+            if (n < 2) {
+                lane.ret((int64_t)n);
+                return;
+            }
+            lane.spawn(F_FIB, Q_LANE, kFibA, FibArg{n - 1, 0});
+            lane.spawn(F_FIB, Q_LANE, kFibB, FibArg{n - 2, 0});
+            if (lane.taskwait(1, Q_LANE)) return;
+        }
+            [[fallthrough]];
+        case 1:
+            lane.ret(lane.template frame_ld<int64_t>(kFibA) + lane.template frame_ld<int64_t>(kFibB));
+            return;
+        default:
+            lane.finish();
+            return;
+    }
+}
+
+template <class L>
+__device__ __forceinline__ void nq_task(L& lane) {
+    switch (lane.get_state()) {
+        case 0: {
+            const int32_t row = lane.template arg<int32_t>(0);
+            const uint32_t cols = lane.template arg<uint32_t>(4);
+            const uint32_t diag = lane.template arg<uint32_t>(8);
+            const uint32_t anti = lane.template arg<uint32_t>(12);
+            const int32_t N = lane.template arg<int32_t>(16);
+            if (row == N) {
+                lane.ret((int64_t)1);
+                return;
+            }
+            const uint32_t mask = (N < 32) ? ((1u << N) - 1u) : 0xFFFFFFFFu;
+            uint32_t avail = mask & ~(cols | diag | anti);
+            int32_t k = 0;
+            while (avail) {
+                const uint32_t p = avail & (0u - avail);
+                avail -= p;
+                lane.spawn(F_NQ, Q_LANE, kNqCounts + 8 * k, NqArg{row + 1, cols | p, (diag | p) << 1, (anti | p) >> 1, N, 0});
+                ++k;
+            }
+            if (k == 0) {
+                lane.ret((int64_t)0);
+                return;
+            }
+            lane.template frame_st<int32_t>(kNqK, k);
+            if (lane.taskwait(1, Q_LANE)) return;
+        }
+            [[fallthrough]];
+        case 1: {
+            const int32_t k = lane.template frame_ld<int32_t>(kNqK);
+            int64_t s = 0;
+            for (int32_t i = 0; i < k; ++i) s += lane.template frame_ld<int64_t>(kNqCounts + 8 * i);
+            lane.ret(s);
+            return;
+        }
+        default:
+            lane.finish();
+            return;
+    }
+}
+
+template <class L>
+__device__ __forceinline__ void sort_root_task(L& lane) {
+    uint32_t* arr = lane.template arg<uint32_t*>(0);
+    uint32_t* tmp = lane.template arg<uint32_t*>(8);
+    const int64_t n = lane.template arg<int64_t>(16);
+    if (lane.get_state() == 0 && n > 1) {
+        lane.spawn(F_MSORT, Q_WAVE, kDummySlot, MsortArg{arr, tmp, n, 0, 0});
+        if (lane.taskwait(1, Q_LANE)) return;
+    }
+    lane.ret(n);
+}
+
+template <class L>
+__device__ void Program::run(L& lane) {
+    switch (lane.func) {
+        case F_FIB: fib_task(lane); return;
+        case F_NQ: nq_task(lane); return;
+        case F_SROOT: sort_root_task(lane); return;
+        default:
+            // wave functions never arrive in a lane batch
+            GTAP_DEVICE_ERROR(*lane.dev, GTAP_ERROR_INVALID_FUNCTION, lane.tid, lane.queue, lane.func, F_SROOT);
+            lane.finish();
+            return;
+    }
+}
+
+// ---------------------------------------------------------------- wave tasks
+
+__device__ __forceinline__ uint32_t ldk(const uint32_t* p) { return ld32(p); }
+__device__ __forceinline__ void stk(uint32_t* p, uint32_t v) { st32(p, v); }
+
+template <class LDA, class LDB>
+__device__ __forceinline__ int64_t merge_path(LDA lda, int64_t na, LDB ldb, int64_t nb, int64_t d) {
+    int64_t lo = d > nb ? d - nb : 0;
+    int64_t hi = d < na ? d : na;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (lda(mid) <= ldb(d - 1 - mid)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Sort src[0..n) (n <= 1024) into dst with a bitonic network over 1024 LDS slots.
+template <class W>
+__device__ void wave_leaf_sort(W& wt, const uint32_t* src, uint32_t* dst, int64_t n) {
+    uint32_t* k = wt.sh->keys;
+    const uint32_t l = (uint32_t)wt.lane;
+    for (uint32_t i = l; i < (uint32_t)kWaveLeaf; i += kWave) k[i] = (int64_t)i < n ? ldk(src + i) : 0xFFFFFFFFu;
+    wave_sync();
+    for (uint32_t size = 2; size <= (uint32_t)kWaveLeaf; size <<= 1) {
+        for (uint32_t j = size >> 1; j > 0; j >>= 1) {
+            for (uint32_t p = l; p < (uint32_t)kWaveLeaf / 2; p += kWave) {
+                const uint32_t lo = 2 * j * (p / j) + (p % j);
+                const uint32_t hi = lo + j;
+                const uint32_t x = k[lo], y = k[hi];
+                const bool up = (lo & size) == 0;
+                if ((x > y) == up) {
+                    k[lo] = y;
+                    k[hi] = x;
+                }
+            }
+            wave_sync();
+        }
+    }
+    for (uint32_t i = l; (int64_t)i < n; i += kWave) stk(dst + i, k[i]);
+    wave_sync();
+}
+
+template <class W>
+__device__ void wave_copy(W& wt, const uint32_t* src, uint32_t* dst, int64_t n) {
+    for (int64_t i = wt.lane; i < n; i += kWave) stk(dst + i, ldk(src + i));
+}
+
+// Merge a[0..na) and b[0..nb) into dst by tiles of 1024 outputs (16 per lane).
+template <class W>
+__device__ void wave_leaf_merge(W& wt, const uint32_t* a, int64_t na, const uint32_t* b, int64_t nb, uint32_t* dst) {
+    typename Program::WaveShared* sh = wt.sh;
+    const int l = wt.lane;
+    const int64_t total = na + nb;
+    auto ga = [a](int64_t i) { return ldk(a + i); };
+    auto gb = [b](int64_t i) { return ldk(b + i); };
+    constexpr int kTileOut = kWaveLeaf;
+    constexpr int kPer = kTileOut / kWave;  // 16
+    for (int64_t base = 0; base < total; base += (int64_t)kTileOut * 16) {
+        const int64_t ntiles_here = ((total - base) + kTileOut - 1) / kTileOut;
+        const int64_t nsplit = (ntiles_here < 16 ? ntiles_here : 16) + 1;
+        if (l < nsplit) {
+            int64_t d = base + (int64_t)l * kTileOut;
+            if (d > total) d = total;
+            sh->split[l] = merge_path(ga, na, gb, nb, d);
+        }
+        wave_sync();
+        for (int64_t ti = 0; ti + 1 < nsplit; ++ti) {
+            const int64_t d0 = base + ti * kTileOut;
+            const int64_t d1 = (d0 + kTileOut < total) ? d0 + kTileOut : total;
+            const int64_t i0 = sh->split[ti], i1 = sh->split[ti + 1];
+            const int64_t la = i1 - i0, lb = (d1 - i1) - (d0 - i0);
+            const int64_t j0 = d0 - i0;
+            wave_sync();
+            for (int64_t i = l; i < la; i += kWave) sh->keys[i] = ldk(a + i0 + i);
+            for (int64_t i = l; i < lb; i += kWave) sh->keys[la + i] = ldk(b + j0 + i);
+            wave_sync();
+            const uint32_t* A = sh->keys;
+            const uint32_t* Bv = sh->keys + la;
+            const int64_t len = d1 - d0;
+            const int64_t od = (int64_t)l * kPer;
+            uint32_t outv[kPer];
+            int cnt = 0;
+            if (od < len) {
+                auto la_ = [A](int64_t i) { return A[i]; };
+                auto lb_ = [Bv](int64_t i) { return Bv[i]; };
+                int64_t i = merge_path(la_, la, lb_, lb, od);
+                int64_t j = od - i;
+                const int64_t oe = (od + kPer < len) ? od + kPer : len;
+                cnt = (int)(oe - od);
+#pragma unroll
+                for (int o = 0; o < kPer; ++o) {
+                    if (o < cnt) {
+                        const bool takeA = i < la && (j >= lb || A[i] <= Bv[j]);
+                        outv[o] = takeA ? A[i] : Bv[j];
+                        if (takeA) ++i;
+                        else ++j;
+                    }
+                }
+            }
+            wave_sync();  // every lane has finished reading the inputs
+#pragma unroll
+            for (int o = 0; o < kPer; ++o)
+                if (o < cnt) sh->keys[od + o] = outv[o];
+            wave_sync();
+            for (int64_t o = l; o < len; o += kWave) stk(dst + d0 + o, sh->keys[o]);
+            wave_sync();
+        }
+    }
+}
+
+// gtap_cilksort.cu:40-50
+__device__ __forceinline__ int64_t binary_search_gm(const uint32_t* x, int64_t len, uint32_t val) {
+    int64_t low = -1, high = len;
+    while (low + 1 < high) {
+        const int64_t mid = (low + high) / 2;
+        if (ldk(x + mid) <= val) low = mid;
+        else high = mid;
+    }
+    return low;
+}
+
+template <class W>
+__device__ void msort_task(W& wt) {
+    uint32_t* a = wt.template arg<uint32_t*>(0);
+    uint32_t* t = wt.template arg<uint32_t*>(8);
+    const int64_t n = wt.template arg<int64_t>(16);
+    const int32_t into_t = wt.template arg<int32_t>(24);
+    uint32_t* dst = into_t ? t : a;
+    const int64_t h = n / 2;
+    switch (wt.get_state()) {
+        case 0:
+            if (n <= kWaveLeaf) {
+                wave_leaf_sort(wt, a, dst, n);
+                wt.finish();
+                return;
+            }
+            wt.spawn(F_MSORT, Q_WAVE, kDummySlot, MsortArg{a, t, h, !into_t, 0});
+            wt.spawn(F_MSORT, Q_WAVE, kDummySlot, MsortArg{a + h, t + h, n - h, !into_t, 0});
+            if (wt.taskwait(1, Q_WAVE)) return;
+            [[fallthrough]];
+        case 1: {
+            const uint32_t* src = into_t ? a : t;  // where the sorted halves are
+            if (n <= kWaveMergeLeaf) {
+                wave_leaf_merge(wt, src, h, src + h, n - h, dst);
+                wt.finish();
+                return;
+            }
+            wt.spawn(F_MERGE, Q_WAVE, kDummySlot, MergeArg{src, src + h, dst, h, n - h});
+            if (wt.taskwait(2, Q_WAVE)) return;
+        }
+            [[fallthrough]];
+        default:
+            wt.finish();
+            return;
+    }
+}
+
+template <class W>
+__device__ void merge_task(W& wt) {
+    const uint32_t* a = wt.template arg<const uint32_t*>(0);
+    const uint32_t* b = wt.template arg<const uint32_t*>(8);
+    uint32_t* dst = wt.template arg<uint32_t*>(16);
+    int64_t na = wt.template arg<int64_t>(24);
+    int64_t nb = wt.template arg<int64_t>(32);
+    if (wt.get_state() != 0) {
+        wt.finish();
+        return;
+    }
+    if (na < nb) {
+        const uint32_t* tp = a;
+        a = b;
+        b = tp;
+        const int64_t tn = na;
+        na = nb;
+        nb = tn;
+    }
+    if (nb == 0) {
+        wave_copy(wt, a, dst, na);
+        wt.finish();
+        return;
+    }
+    if (na + nb <= kWaveMergeLeaf) {
+        wave_leaf_merge(wt, a, na, b, nb, dst);
+        wt.finish();
+        return;
+    }
+    const int64_t a_split = (na + 1) / 2;
+    const int64_t b_split = binary_search_gm(b, nb, ldk(a + a_split - 1)) + 1;
+    wt.spawn(F_MERGE, Q_WAVE, kDummySlot, MergeArg{a, b, dst, a_split, b_split});
+    wt.spawn(F_MERGE, Q_WAVE, kDummySlot, MergeArg{a + a_split, b + b_split, dst + a_split + b_split, na - a_split,
+                                                    nb - b_split});
+    if (wt.taskwait(1, Q_WAVE)) return;
+    wt.finish();
+}
+
+template <class W>
+__device__ void Program::run_wave(W& wt) {
+    if (wt.func == F_MSORT) msort_task(wt);
+    else if (wt.func == F_MERGE) merge_task(wt);
+    else {
+        if (wt.lane == 0) GTAP_DEVICE_ERROR(*wt.dev, GTAP_ERROR_INVALID_FUNCTION, wt.tid, wt.queue, wt.func, F_MERGE);
+        wt.finish();
+    }
+}
+
+}  // namespace stream
+}  // namespace gtap
+
+using StreamProgram = gtap::stream::Program;
+static_assert(StreamProgram::kFrameBytes + gtap::kHdrBytes <= gtap::host::kBuiltinStride, "frame exceeds stride");
+
+// ======================================================================== host side
+
+namespace {
+
+using namespace gtap;
+
+uint64_t splitmix64_next(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+uint64_t host_now_ns() {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+
+// Job i of the mixed stream from `seed` (40% fib n in [18,28], 30% n-queens N in [8,12],
+// 30% mergesort of floor(2^(10+10u)) uint32 keys); kind 0 fib, 1 n-queens, 2 sort.
+extern "C" int gtap_jobs_generate(uint64_t seed, int32_t njobs, int32_t* kind, int32_t* param, uint64_t* key_seed) {
+    if (njobs < 0 || (njobs > 0 && (!kind || !param || !key_seed))) return host::invalid_argument("gtap_jobs_generate");
+    for (int32_t i = 0; i < njobs; ++i) {
+        uint64_t s = seed ^ ((uint64_t)(2 * (int64_t)i + 1) * 0xD1B54A32D192ED03ULL);
+        const uint64_t r0 = splitmix64_next(s);
+        const uint64_t r1 = splitmix64_next(s);
+        const uint32_t k = (uint32_t)(r0 >> 32) % 10u;
+        key_seed[i] = 0;
+        if (k < 4) {
+            kind[i] = 0;
+            param[i] = 18 + (int32_t)((uint32_t)(r1 >> 32) % 11u);
+        } else if (k < 7) {
+            kind[i] = 1;
+            param[i] = 8 + (int32_t)((uint32_t)(r1 >> 32) % 5u);
+        } else {
+            const double u = (double)(r1 >> 11) * (1.0 / 9007199254740992.0);
+            double len = std::floor(std::exp2(10.0 + 10.0 * u));
+            len = std::min(std::max(len, 1024.0), 1048576.0);
+            kind[i] = 2;
+            param[i] = (int32_t)len;
+            key_seed[i] = r1 ^ 0x94D049BB133111EBULL;
+        }
+    }
+    return 0;
+}
+
+extern "C" int gtap_gen_keys(uint64_t seed, uint64_t n, uint32_t* out);
+
+struct gtap_stream {
+    int32_t njobs = 0;
+    std::vector<int32_t> kind, param;
+    std::vector<uint64_t> kseed;
+    std::vector<int64_t> key_off;   // offset of each sort job's keys (-1 otherwise)
+    int64_t total_keys = 0;
+    uint32_t* d_src = nullptr;      // pristine unsorted keys of every sort job
+    uint32_t* d_work = nullptr;     // sorted in place
+    uint32_t* d_tmp = nullptr;
+    std::vector<gtap_job> table;    // job table template
+};
+
+extern "C" int gtap_stream_create(uint64_t seed, int32_t njobs, gtap_stream** out) {
+    if (!out || njobs <= 0) return host::invalid_argument("gtap_stream_create");
+    host::Runtime& R = host::rt();
+    if (!R.initialized) {
+        int st = gtap_initialize();
+        if (st) return st;
+    }
+    if ((uint32_t)njobs > R.nanchors) return host::invalid_argument("gtap_stream_create: more jobs than max_jobs");
+    gtap_stream* S = new gtap_stream();
+    S->njobs = njobs;
+    S->kind.resize(njobs);
+    S->param.resize(njobs);
+    S->kseed.resize(njobs);
+    S->key_off.assign(njobs, -1);
+    gtap_jobs_generate(seed, njobs, S->kind.data(), S->param.data(), S->kseed.data());
+    for (int32_t i = 0; i < njobs; ++i) {
+        if (S->kind[i] == 2) {
+            S->key_off[i] = S->total_keys;
+            S->total_keys += S->param[i];
+        }
+    }
+    const size_t bytes = (size_t)std::max<int64_t>(S->total_keys, 1) * sizeof(uint32_t);
+    hipError_t e = hipMalloc(&S->d_src, bytes);
+    if (e == hipSuccess) e = hipMalloc(&S->d_work, bytes);
+    if (e == hipSuccess) e = hipMalloc(&S->d_tmp, bytes);
+    if (e != hipSuccess) {
+        delete S;
+        return host::hip_status(e, "gtap_stream_create alloc");
+    }
+    std::vector<uint32_t> keys((size_t)std::max<int64_t>(S->total_keys, 1));
+    for (int32_t i = 0; i < njobs; ++i)
+        if (S->kind[i] == 2) gtap_gen_keys(S->kseed[i], (uint64_t)S->param[i], keys.data() + S->key_off[i]);
+    GTAP_HIP_TRY(hipMemcpy(S->d_src, keys.data(), bytes, hipMemcpyHostToDevice));
+    S->table.resize(njobs);
+    for (int32_t i = 0; i < njobs; ++i) {
+        gtap_job j;
+        memset(&j, 0, sizeof(j));
+        j.status = GTAP_JOB_READY;
+        j.kind = (uint32_t)S->kind[i];
+        j.queue = stream::Q_LANE;
+        if (S->kind[i] == 0) {
+            j.func = stream::F_FIB;
+            stream::FibArg a{S->param[i], 0};
+            memcpy(j.args, &a, sizeof(a));
+        } else if (S->kind[i] == 1) {
+            j.func = stream::F_NQ;
+            stream::NqArg a{0, 0u, 0u, 0u, S->param[i], 0};
+            memcpy(j.args, &a, sizeof(a));
+        } else {
+            j.func = stream::F_SROOT;
+            stream::SortRootArg a{S->d_work + S->key_off[i], S->d_tmp + S->key_off[i], (int64_t)S->param[i]};
+            memcpy(j.args, &a, sizeof(a));
+        }
+        S->table[i] = j;
+    }
+    *out = S;
+    return 0;
+}
+
+extern "C" int gtap_stream_destroy(gtap_stream* S) {
+    if (!S) return 0;
+    (void)hipFree(S->d_src);
+    (void)hipFree(S->d_work);
+    (void)hipFree(S->d_tmp);
+    delete S;
+    return 0;
+}
+
+extern "C" int gtap_stream_info(gtap_stream* S, int32_t* kind, int32_t* param, uint64_t* key_seed, int64_t* total_keys) {
+    if (!S) return host::invalid_argument("gtap_stream_info");
+    if (kind) memcpy(kind, S->kind.data(), S->kind.size() * sizeof(int32_t));
+    if (param) memcpy(param, S->param.data(), S->param.size() * sizeof(int32_t));
+    if (key_seed) memcpy(key_seed, S->kseed.data(), S->kseed.size() * sizeof(uint64_t));
+    if (total_keys) *total_keys = S->total_keys;
+    return 0;
+}
+
+// Copies the (sorted, after a run) keys of sort job `job` to out.
+extern "C" int gtap_stream_sorted_keys(gtap_stream* S, int32_t job, uint32_t* out) {
+    if (!S || job < 0 || job >= S->njobs || S->kind[job] != 2 || !out) return host::invalid_argument("gtap_stream_sorted_keys");
+    GTAP_HIP_TRY(hipMemcpy(out, S->d_work + S->key_off[job], (size_t)S->param[job] * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int stream_reset_keys(gtap_stream* S) {
+    host::Runtime& R = host::rt();
+    if (S->total_keys > 0)
+        GTAP_HIP_TRY(hipMemcpyAsync(S->d_work, S->d_src, (size_t)S->total_keys * 4, hipMemcpyDeviceToDevice, R.stream));
+    GTAP_HIP_TRY(hipStreamSynchronize(R.stream));
+    return 0;
+}
+
+// All jobs in one persistent launch. results[i] = root return value (sort jobs: n);
+// latency_ms[i] = completion time of job i after the launch start (all jobs arrive at the
+// start). stats->kernel_ms is the launch time.
+extern "C" int gtap_stream_run_batched(gtap_stream* S, int64_t* results, double* latency_ms, gtap_run_stats* stats) {
+    if (!S) return host::invalid_argument("gtap_stream_run_batched");
+    host::Runtime& R = host::rt();
+    int st = stream_reset_keys(S);
+    if (st) return st;
+    GTAP_HIP_TRY(hipMemcpyAsync(R.jobs, S->table.data(), S->table.size() * sizeof(gtap_job), hipMemcpyHostToDevice,
+                                R.stream));
+    st = host::launch_thread<StreamProgram>(StreamProgram::Args{0}, GTAP_TERM_JOBS, (uint32_t)S->njobs, R.jobs,
+                                            (uint32_t)S->njobs, stats);
+    if (st) return st;
+    std::vector<gtap_job> out(S->njobs);
+    GTAP_HIP_TRY(hipMemcpy(out.data(), R.jobs, out.size() * sizeof(gtap_job), hipMemcpyDeviceToHost));
+    uint64_t t0 = UINT64_MAX;
+    for (auto& j : out) t0 = std::min(t0, j.t_start);
+    for (int32_t i = 0; i < S->njobs; ++i) {
+        if (out[i].status != GTAP_JOB_DONE) return host::fail(GTAP_ERROR_INVALID_TASKWAIT, "stream job did not complete");
+        if (results) results[i] = out[i].result;
+        if (latency_ms) latency_ms[i] = (double)(out[i].t_end - t0) * 1e-5;  // 100 MHz ticks
+    }
+    return 0;
+}
+
+// The plain form: one persistent launch per job, in job order. latency_ms[i] = host time
+// from the start of the sequence to the end of job i's launch; *total_ms = whole sequence.
+extern "C" int gtap_stream_run_per_job(gtap_stream* S, int64_t* results, double* latency_ms, double* total_ms) {
+    if (!S) return host::invalid_argument("gtap_stream_run_per_job");
+    host::Runtime& R = host::rt();
+    int st = stream_reset_keys(S);
+    if (st) return st;
+    const uint64_t t0 = host_now_ns();
+    for (int32_t i = 0; i < S->njobs; ++i) {
+        int64_t r = 0;
+        st = host::run_single_job<StreamProgram>(StreamProgram::Args{0}, S->table[i], &r, nullptr, GTAP_TERM_JOBS);
+        if (st) return st;
+        if (results) results[i] = r;
+        if (latency_ms) latency_ms[i] = (double)(host_now_ns() - t0) * 1e-6;
+    }
+    if (total_ms) *total_ms = (double)(host_now_ns() - t0) * 1e-6;
+    (void)R;
+    return 0;
+}
+
+// Poisson arrivals at `rate` jobs/s (exponential gaps from arrival_seed). Resident form
+// (per_job_launch = 0): one served launch; the host loop writes each job into the pinned
+// job table at its arrival time and publishes the count, the kernel admits jobs as they
+// appear (up to max_live concurrently running), the host polls completions. Plain form
+// (per_job_launch = 1): each job is launched on its own when it has arrived and the
+// previous launch has finished. latency_ms[i] = completion seen by the host - arrival.
+extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_seed, int32_t per_job_launch,
+                                 int32_t max_live, int64_t* results, double* latency_ms, double* total_ms,
+                                 gtap_run_stats* stats) {
+    if (!S || !(rate > 0)) return host::invalid_argument("gtap_stream_serve");
+    host::Runtime& R = host::rt();
+    int st = stream_reset_keys(S);
+    if (st) return st;
+    const int32_t n = S->njobs;
+    std::vector<uint64_t> arrival(n);
+    {
+        uint64_t s = arrival_seed ^ 0xA0761D6478BD642FULL;
+        double t = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            const double u = ((double)(splitmix64_next(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+            t += -std::log(u) / rate;
+            arrival[i] = (uint64_t)(t * 1e9);
+        }
+    }
+    std::vector<double> lat(n, 0.0);
+    std::vector<int64_t> res(n, 0);
+    if (per_job_launch) {
+        const uint64_t t0 = host_now_ns();
+        for (int32_t i = 0; i < n; ++i) {
+            while (host_now_ns() - t0 < arrival[i]) std::this_thread::yield();
+            int64_t r = 0;
+            st = host::run_single_job<StreamProgram>(StreamProgram::Args{0}, S->table[i], &r, nullptr, GTAP_TERM_JOBS);
+            if (st) return st;
+            res[i] = r;
+            lat[i] = (double)(host_now_ns() - t0 - arrival[i]) * 1e-6;
+        }
+        if (total_ms) *total_ms = (double)(host_now_ns() - t0) * 1e-6;
+    } else {
+        if (!R.jobs_host) {
+            GTAP_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&R.jobs_host), (size_t)R.nanchors * sizeof(gtap_job),
+                                       hipHostMallocMapped | hipHostMallocCoherent));
+            GTAP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&R.jobs_host_dev), R.jobs_host, 0));
+        }
+        volatile gtap_job* tab = R.jobs_host;
+        for (int32_t i = 0; i < n; ++i) {
+            gtap_job j = S->table[i];
+            j.status = GTAP_JOB_EMPTY;
+            memcpy((void*)&tab[i], &j, sizeof(j));
+        }
+        volatile int32_t* hctl = R.host_ctl;
+        hctl[HCTL_JOB_AVAIL * 32] = 0;
+        hctl[HCTL_STOP * 32] = 0;
+        __atomic_thread_fence(__ATOMIC_SEQ_CST);
+        // the launch itself (asynchronous): served mode, admission by live jobs and free ids
+        static uint32_t max_blocks = 0;
+        const void* kfn = reinterpret_cast<const void*>(&thread_kernel<StreamProgram>);
+        if (max_blocks == 0) {
+            st = host::occupancy_blocks(kfn, 0, &max_blocks);
+            if (st) return st;
+        }
+        uint32_t blocks = max_blocks;
+        if (R.cfg.blocks && R.cfg.blocks < blocks) blocks = R.cfg.blocks;
+        if (blocks * 4 > R.max_workers) blocks = R.max_workers / 4;
+        GtapDev dev = host::make_dev(blocks * 4, GTAP_TERM_SERVE, (uint32_t)n, R.jobs_host_dev, (uint32_t)n);
+        dev.max_live_jobs = max_live > 0 ? (uint32_t)max_live : 1024u;
+        dev.admit_reserve = 4096;  // free task ids required per admitted job
+        const double expected_s = arrival.back() * 1e-9;
+        const uint64_t need = (uint64_t)((expected_s * 3.0 + 30.0) * 1e8);
+        if (dev.timeout_ticks < need) dev.timeout_ticks = need;
+        st = host::reset_launch_state(dev, R.stream);
+        if (st) return st;
+        GTAP_HIP_TRY(hipEventRecord(R.ev0, R.stream));
+        hipLaunchKernelGGL(thread_kernel<StreamProgram>, dim3(blocks), dim3(256), 0, R.stream, dev,
+                           StreamProgram::Args{0});
+        GTAP_HIP_TRY(hipGetLastError());
+        GTAP_HIP_TRY(hipEventRecord(R.ev1, R.stream));
+        const uint64_t t0 = host_now_ns();
+        int32_t submitted = 0, done = 0, first_pending = 0;
+        std::vector<char> seen(n, 0);
+        const uint64_t deadline = t0 + (uint64_t)((expected_s * 3.0 + 60.0) * 1e9);
+        while (done < n) {
+            const uint64_t now = host_now_ns();
+            while (submitted < n && now - t0 >= arrival[submitted]) {
+                tab[submitted].t_submit = now;
+                tab[submitted].status = GTAP_JOB_READY;
+                ++submitted;
+                __atomic_thread_fence(__ATOMIC_SEQ_CST);
+                hctl[HCTL_JOB_AVAIL * 32] = submitted;
+            }
+            while (first_pending < submitted && seen[first_pending]) ++first_pending;
+            for (int32_t i = first_pending; i < submitted; ++i) {
+                if (!seen[i] && tab[i].status == GTAP_JOB_DONE) {
+                    seen[i] = 1;
+                    ++done;
+                    const uint64_t tn = host_now_ns();
+                    lat[i] = (double)(tn - t0 - arrival[i]) * 1e-6;
+                    res[i] = tab[i].result;
+                }
+            }
+            if (hipStreamQuery(R.stream) == hipSuccess) break;  // the kernel ended (error or done)
+            if (host_now_ns() > deadline) break;
+        }
+        hctl[HCTL_STOP * 32] = 1;
+        __atomic_thread_fence(__ATOMIC_SEQ_CST);
+        GTAP_HIP_TRY(hipStreamSynchronize(R.stream));
+        if (total_ms) *total_ms = (double)(host_now_ns() - t0) * 1e-6;
+        st = host::finish_launch(dev, blocks, stats);
+        if (st) return st;
+        for (int32_t i = 0; i < n; ++i) {
+            if (!seen[i]) {
+                if (tab[i].status != GTAP_JOB_DONE) return host::fail(GTAP_ERROR_INVALID_TASKWAIT, "served job did not complete");
+                seen[i] = 1;
+                lat[i] = (double)(host_now_ns() - t0 - arrival[i]) * 1e-6;
+                res[i] = tab[i].result;
+            }
+        }
+    }
+    if (results) memcpy(results, res.data(), n * sizeof(int64_t));
+    if (latency_ms) memcpy(latency_ms, lat.data(), n * sizeof(double));
+    return 0;
+}

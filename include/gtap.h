@@ -113,6 +113,31 @@ int gtap_cilksort(uint32_t* keys, uint32_t* tmp, int64_t n, int64_t leaf, int64_
 /* The same with a host buffer (copied in and out around the launch). */
 int gtap_cilksort_host(uint32_t* keys, int64_t n, int64_t leaf, int64_t merge_leaf, gtap_run_stats* stats);
 
+/* ------------------------------------------------------------ job streams */
+/* A mixed stream of fib / n-queens / mergesort jobs run in one resident persistent kernel
+ * (batched, or served while a host loop streams jobs in) or one launch per job (the plain
+ * form). No reference counterpart: the reference runs one program per launch
+ * (examples/fib/fib.cu:26-57). Kinds: 0 fib(n), 1 n-queens(N), 2 mergesort of `param` keys. */
+typedef struct gtap_stream gtap_stream;
+/* Job i of the stream from `seed`: 40% fib n in [18,28], 30% n-queens N in [8,12], 30%
+ * mergesort of floor(2^(10+10u)) uint32 keys from the per-job key seed. */
+int gtap_jobs_generate(uint64_t seed, int32_t njobs, int32_t* kind, int32_t* param, uint64_t* key_seed);
+/* Generates the jobs, their sort inputs (device resident) and the job table. */
+int gtap_stream_create(uint64_t seed, int32_t njobs, gtap_stream** out);
+int gtap_stream_destroy(gtap_stream* stream);
+int gtap_stream_info(gtap_stream* stream, int32_t* kind, int32_t* param, uint64_t* key_seed, int64_t* total_keys);
+int gtap_stream_sorted_keys(gtap_stream* stream, int32_t job, uint32_t* out);
+/* All jobs in one launch; latency_ms[i] = completion of job i after the launch start. */
+int gtap_stream_run_batched(gtap_stream* stream, int64_t* results, double* latency_ms, gtap_run_stats* stats);
+/* One launch per job, in order; latency_ms[i] = end of job i after the start of the sequence. */
+int gtap_stream_run_per_job(gtap_stream* stream, int64_t* results, double* latency_ms, double* total_ms);
+/* Poisson arrivals at `rate` jobs/s: served by one resident launch (per_job_launch = 0, at
+ * most max_live jobs running at once, 0 -> 1024) or one launch per job (per_job_launch = 1).
+ * latency_ms[i] = completion seen by the host - arrival. */
+int gtap_stream_serve(gtap_stream* stream, double rate, uint64_t arrival_seed, int32_t per_job_launch,
+                      int32_t max_live, int64_t* results, double* latency_ms, double* total_ms,
+                      gtap_run_stats* stats);
+
 /* ------------------------------------------------------------ input generators (host) */
 /* R-MAT edges, CSR by stable counting sort, and the gen_vector key stream
  * (evaluation/util/gen_vector.c:8-18; read format examples/asynchronous_bfs/

@@ -1,0 +1,80 @@
+"""The mixed job stream on the GPU: batched (one resident launch), one launch per job, and
+served (Poisson arrivals streamed into the resident kernel). Every job's result is checked
+against the oracle; each job's result alone equals its result in the mix."""
+import numpy as np
+import pytest
+
+import gtap_amd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def runtime():
+    gtap_amd.gtap_set_config(timeout_ms=60000)
+    assert gtap_amd.gtap_initialize() == 0
+    yield
+    assert gtap_amd.gtap_finalize() == 0
+
+
+def expected_scalar(kind, param):
+    if kind == 0:
+        return oracle.fib(int(param))
+    if kind == 1:
+        return oracle.nq(int(param))[0]
+    return int(param)
+
+
+def check_stream(stream, results, sort_jobs=None):
+    for i in range(stream.njobs):
+        assert results[i] == expected_scalar(stream.kind[i], stream.param[i]), (i, stream.kind[i], stream.param[i])
+    sort_idx = np.where(stream.kind == 2)[0]
+    if sort_jobs is not None:
+        sort_idx = sort_idx[:sort_jobs]
+    for i in sort_idx:
+        keys = oracle.gen_keys_u32(int(stream.param[i]), int(stream.key_seed[i]))
+        assert np.array_equal(stream.sorted_keys(int(i)), np.sort(keys)), i
+
+
+def test_job_generator_matches_oracle():
+    k, p, s = gtap_amd.jobs_generate(2024, 4096)
+    k2, p2, s2 = oracle.jobs(2024, 4096)
+    assert np.array_equal(k, k2) and np.array_equal(p, p2) and np.array_equal(s, s2)
+
+
+def test_stream_small_all_forms():
+    st = gtap_amd.Stream(seed=7, njobs=48)
+    try:
+        print("case stream small batched", flush=True)
+        res_b, lat_b, stats = st.run_batched()
+        check_stream(st, res_b)
+        assert stats["jobs_taken"] == 48
+        print("case stream small per-job", flush=True)
+        res_p, lat_p, tot = st.run_per_job()
+        check_stream(st, res_p)
+        assert np.array_equal(res_b, res_p)  # alone == in the mix
+        print("case stream small served", flush=True)
+        res_s, lat_s, tot_s, stats_s = st.serve(rate=2000.0, arrival_seed=3)
+        check_stream(st, res_s)
+        assert np.all(lat_s > 0)
+        print("case stream small served per-job", flush=True)
+        res_q, lat_q, tot_q, _ = st.serve(rate=2000.0, arrival_seed=3, per_job_launch=True)
+        check_stream(st, res_q)
+    finally:
+        st.close()
+
+
+def test_stream_full_batched():
+    st = gtap_amd.Stream(seed=2024, njobs=4096)
+    try:
+        print("case stream 4096 batched", flush=True)
+        res, lat, stats = st.run_batched()
+        check_stream(st, res, sort_jobs=64)
+        assert stats["jobs_taken"] == 4096
+        # the same jobs alone (one launch each) for a sample give the same results
+        print("case stream 4096 served", flush=True)
+        res_s, lat_s, tot_s, _ = st.serve(rate=40000.0, arrival_seed=11)
+        assert np.array_equal(res, res_s)
+    finally:
+        st.close()
