@@ -129,6 +129,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_get_runtime_error_string": ([_i], ctypes.c_char_p),
     "gtap_get_runtime_error_report": ([ctypes.POINTER(ErrorReport)], _i),
     "gtap_visualize_profile": ([ctypes.c_char_p], None),
+    "gtap_set_profiling": ([_i, _u32], _i),
     "gtap_get_last_stats": ([ctypes.POINTER(RunStats)], _i),
     "gtap_get_worker_stats": ([_vp, _u32, ctypes.POINTER(_u32)], _i),
     "gtap_device_info": ([ctypes.c_char_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_u64)], _i),
@@ -229,6 +230,10 @@ def gtap_get_runtime_error_string(code: int) -> str:
 
 def gtap_visualize_profile(app_name: str) -> None:
     lib().gtap_visualize_profile(app_name.encode())
+
+
+def gtap_set_profiling(enable: bool, max_samples: int = 4096) -> int:
+    return lib().gtap_set_profiling(1 if enable else 0, max_samples)
 
 
 def last_stats() -> dict:

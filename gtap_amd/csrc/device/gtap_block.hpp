@@ -352,6 +352,8 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
     bool idle_registered = false;
     uint32_t idle_iter = 0;
     LoopStats st = {0, 0, 0, 0, 0, 0};
+    uint32_t ph_idx = 0, pw_idx = 0;  // timeline profiling (thread 0)
+    bool had_task = false;
 
     while (true) {
         // keep enough ids in LDS for one invocation's spawns (collective)
@@ -365,6 +367,10 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
             }
             if (!id) id = block_steal(dev, me, rng, idle_registered, st);
             if (!id) {
+                if (had_task) {
+                    had_task = false;
+                    if (dev.prof_max) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, ph_idx, 0);
+                }
                 if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered) {
                     idle_registered = true;
                     const int32_t cnt = atomic_add(dev.ctl + CTL_IDLE * 32, 1) + 1;
@@ -424,6 +430,11 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
         T.res_off = c->res_off;
         T.rec = rec_ptr(dev, T.tid);
         st.tasks += 1;
+        if (t == 0 && dev.prof_max) {
+            if (!had_task) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, ph_idx, 0);
+            prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, 1);
+        }
+        had_task = true;
         if (T.func == 0 || T.func > P::kMaxFunc) {
             if (t == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, T.tid, 0, T.func, P::kMaxFunc);
             T.finish();
@@ -467,6 +478,7 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
                 c->bottom = c->tail;
             }
             c->cur = next;
+            if (dev.prof_max) prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, 1);
             if ((st.tasks & 63u) == 0 && ld32(dev.ctl + CTL_ABORT * 32)) c->exit = 1;
         }
         __syncthreads();

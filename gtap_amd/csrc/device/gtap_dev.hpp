@@ -63,6 +63,9 @@ struct GtapDev {
     gtap_error_report* report; // host-mapped error report
     gtap_job* jobs;            // job table
     uint64_t* stats;           // [worker][ST_COUNT]
+    uint64_t* prof_having;     // [worker][prof_max] ns stamps: even = has tasks, odd = ran dry
+    uint64_t* prof_working;    // [worker][prof_max] ns stamps: even = round start, odd = round end
+    int32_t* prof_counts;      // [worker][prof_max] tasks in the round of each working stamp
     uint64_t timeout_ticks;    // watchdog, wall-clock ticks (100 MHz)
     uint32_t stride;           // record stride in bytes (multiple of 16)
     uint32_t nrecs;            // task ids [0, nrecs); ids [nrecs, nrecs + nanchors) are anchors
@@ -79,7 +82,19 @@ struct GtapDev {
     uint32_t job_ring;         // job-table entries (anchors are job index % job_ring)
     uint32_t admit_reserve;    // free ids required per admitted job
     uint32_t max_live_jobs;    // admission cap on concurrently running jobs
+    uint32_t prof_max;         // samples per worker; 0 = profiling off
 };
+
+// Timeline sample (profiling on): ns since an arbitrary epoch, from the 100 MHz wall clock.
+__device__ __forceinline__ void prof_stamp(uint64_t* base, int32_t* counts, uint32_t max, uint32_t worker,
+                                           uint32_t& idx, int32_t count) {
+    if (idx < max) {
+        const size_t at = (size_t)worker * max + idx;
+        base[at] = wall_clock64() * 10ull;
+        if (counts) counts[at] = count;
+        idx += 1;
+    }
+}
 
 // Record header (16 bytes) at the start of every task record and anchor.
 //   word0: func id [0..11] | state [12..23] | queue [24..27]

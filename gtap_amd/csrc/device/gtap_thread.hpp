@@ -770,6 +770,9 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     uint32_t idle_iter = 0;
     uint32_t busy_iter = 0;
     LoopStats st = {0, 0, 0, 0, 0, 0};
+    // timeline profiling (lane 0, when dev.prof_max > 0)
+    uint32_t ph_idx = 0, pw_idx = 0;
+    bool had_tasks = false;
 
     while (true) {
         if (n == 0) {
@@ -799,6 +802,10 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 }
             }
             if (n == 0) {
+                if (had_tasks) {
+                    had_tasks = false;
+                    if (dev.prof_max && lane == 0) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, ph_idx, 0);
+                }
                 // ---- idle: termination, abort, watchdog, backoff
                 if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered) {
                     idle_registered = true;
@@ -843,6 +850,12 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         // ---- execute one round: lane i runs task my_tid when i < n
         st.rounds += 1;
         st.tasks += n;
+        const uint32_t round_n = n;
+        if (dev.prof_max && lane == 0) {
+            if (!had_tasks) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, ph_idx, 0);
+            prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, (int32_t)round_n);
+        }
+        had_tasks = true;
         uint32_t fin_tid = 0, fin_parent = kNoParent;
         bool fin = false;
         if (WQ >= 0 && batch_q == WQ) {
@@ -942,6 +955,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         n = keep;
         my_tid = next_tid;
         batch_q = (int)kmax;
+        if (dev.prof_max && lane == 0) prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, (int32_t)round_n);
 
         busy_iter += 1;
         if ((busy_iter & 63u) == 0) {

@@ -103,6 +103,10 @@ GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job
     d.job_ring = job_ring;
     d.admit_reserve = 0;
     d.max_live_jobs = R.nanchors;
+    d.prof_max = R.prof_max;
+    d.prof_having = R.prof_having;
+    d.prof_working = R.prof_working;
+    d.prof_counts = R.prof_counts;
     return d;
 }
 
@@ -115,6 +119,13 @@ __global__ void reset_kernel(GtapDev dev) {
     for (uint64_t i = gid; i < (uint64_t)dev.nshards * dev.ncells; i += gs)
         dev.pool_seq[i] = (uint32_t)(i & (dev.ncells - 1));
     for (uint64_t i = gid; i < (uint64_t)dev.nworkers * ST_COUNT; i += gs) dev.stats[i] = 0;
+    if (dev.prof_max) {
+        for (uint64_t i = gid; i < (uint64_t)dev.nworkers * dev.prof_max; i += gs) {
+            dev.prof_having[i] = 0;
+            dev.prof_working[i] = 0;
+            dev.prof_counts[i] = 0;
+        }
+    }
     // ring positions start at 0; the fresh-id counter starts at 64 (ids 0..63 are
     // reserved, id 0 is the sink record)
     for (uint64_t i = gid; i < (uint64_t)dev.nshards * 32 + 32; i += gs)
@@ -165,6 +176,7 @@ int finish_launch(const GtapDev& dev, uint32_t blocks, gtap_run_stats* stats) {
     }
     s.workers = dev.nworkers;
     s.blocks = blocks;
+    R.last_nworkers = dev.nworkers;
     s.error_code = err;
     R.last = s;
     R.last_error_code = err;
@@ -197,6 +209,9 @@ static void free_all() {
     dfree(R.ctl);
     dfree(R.jobs);
     dfree(R.stats);
+    dfree(R.prof_having);
+    dfree(R.prof_working);
+    dfree(R.prof_counts);
     if (R.host_ctl) (void)hipHostFree(R.host_ctl);
     if (R.report) (void)hipHostFree(R.report);
     if (R.jobs_host) (void)hipHostFree(R.jobs_host);
@@ -382,6 +397,30 @@ int gtap_synchronize(void) {
         return (int)e;
     }
     return gtap_check_runtime_error();
+}
+
+// Timeline profiling for the following launches: up to max_samples stamps per worker of
+// each kind (reference: PROFILE / MAX_PROFILE_DATA, runtime/detail/common/gtap_config.cuh:22-26).
+int gtap_set_profiling(int enable, uint32_t max_samples) {
+    Runtime& R = rt();
+    if (!R.initialized) {
+        int st = gtap_initialize();
+        if (st) return st;
+    }
+    if (R.prof_having) (void)hipFree(R.prof_having);
+    if (R.prof_working) (void)hipFree(R.prof_working);
+    if (R.prof_counts) (void)hipFree(R.prof_counts);
+    R.prof_having = R.prof_working = nullptr;
+    R.prof_counts = nullptr;
+    R.prof_max = 0;
+    if (!enable) return 0;
+    if (max_samples == 0) max_samples = 4096;
+    const size_t n = (size_t)R.max_workers * max_samples;
+    GTAP_HIP_TRY(hipMalloc(&R.prof_having, n * sizeof(uint64_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.prof_working, n * sizeof(uint64_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.prof_counts, n * sizeof(int32_t)));
+    R.prof_max = max_samples;
+    return 0;
 }
 
 int gtap_get_last_stats(gtap_run_stats* out) {

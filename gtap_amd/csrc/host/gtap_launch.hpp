@@ -63,6 +63,12 @@ struct Runtime {
     gtap_job* jobs_host_dev = nullptr;
     uint64_t* stats = nullptr;
     std::vector<uint64_t> stats_host;
+    // timeline profiling (gtap_set_profiling)
+    uint32_t prof_max = 0;
+    uint64_t* prof_having = nullptr;
+    uint64_t* prof_working = nullptr;
+    int32_t* prof_counts = nullptr;
+    uint32_t last_nworkers = 0;
     gtap_run_stats last{};
     int last_error_code = 0;
 };

@@ -60,7 +60,7 @@ struct Program {
     static constexpr uint32_t kFrameBytes = 16;
     using Args = BfsArgs;
     template <class L>
-    __device__ static void run(L& lane);
+    __device__ __forceinline__ static void run(L& lane);
 };
 
 template <int NQ>
@@ -110,7 +110,7 @@ __device__ __forceinline__ void bfs_task(L& lane) {
 
 template <int NQ>
 template <class L>
-__device__ void Program<NQ>::run(L& lane) {
+__device__ __forceinline__ void Program<NQ>::run(L& lane) {
     bfs_task<NQ>(lane);
 }
 

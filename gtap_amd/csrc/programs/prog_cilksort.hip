@@ -63,7 +63,7 @@ struct Program {
         int64_t split[18];
     };
     template <class B>
-    __device__ static void run(B& blk);
+    __device__ __forceinline__ static void run(B& blk);
 };
 
 __device__ __forceinline__ uint32_t ldk(const uint32_t* p) { return ld32(p); }
@@ -263,7 +263,7 @@ __device__ void merge_task(B& blk) {
 }
 
 template <class B>
-__device__ void Program::run(B& blk) {
+__device__ __forceinline__ void Program::run(B& blk) {
     if (blk.func == F_SORT) sort_task(blk);
     else merge_task(blk);
 }

@@ -39,7 +39,7 @@ struct Program {
         int32_t unused;
     };
     template <class L>
-    __device__ static void run(L& lane);
+    __device__ __forceinline__ static void run(L& lane);
 };
 
 template <class L>
@@ -70,7 +70,7 @@ __device__ __forceinline__ void fib_task(L& lane) {
 }
 
 template <class L>
-__device__ void Program::run(L& lane) {
+__device__ __forceinline__ void Program::run(L& lane) {
     fib_task(lane);
 }
 

@@ -46,7 +46,7 @@ struct Program {
         int32_t n;
     };
     template <class L>
-    __device__ static void run(L& lane);
+    __device__ __forceinline__ static void run(L& lane);
 };
 
 template <class L>
@@ -95,7 +95,7 @@ __device__ __forceinline__ void nq_task(L& lane) {
 }
 
 template <class L>
-__device__ void Program::run(L& lane) {
+__device__ __forceinline__ void Program::run(L& lane) {
     nq_task(lane);
 }
 

@@ -78,12 +78,12 @@ struct Program {
     using Args = StreamArgs;
     struct WaveShared {
         uint32_t keys[kWaveLeaf];
-        int64_t split[18];
+        int32_t split[18];
     };
     template <class L>
-    __device__ static void run(L& lane);
+    __device__ __forceinline__ static void run(L& lane);
     template <class W>
-    __device__ static void run_wave(W& wt);
+    __device__ __forceinline__ static void run_wave(W& wt);
 };
 
 // ---------------------------------------------------------------- lane tasks
@@ -167,7 +167,7 @@ __device__ __forceinline__ void sort_root_task(L& lane) {
 }
 
 template <class L>
-__device__ void Program::run(L& lane) {
+__device__ __forceinline__ void Program::run(L& lane) {
     switch (lane.func) {
         case F_FIB: fib_task(lane); return;
         case F_NQ: nq_task(lane); return;
@@ -184,18 +184,6 @@ __device__ void Program::run(L& lane) {
 
 __device__ __forceinline__ uint32_t ldk(const uint32_t* p) { return ld32(p); }
 __device__ __forceinline__ void stk(uint32_t* p, uint32_t v) { st32(p, v); }
-
-template <class LDA, class LDB>
-__device__ __forceinline__ int64_t merge_path(LDA lda, int64_t na, LDB ldb, int64_t nb, int64_t d) {
-    int64_t lo = d > nb ? d - nb : 0;
-    int64_t hi = d < na ? d : na;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (lda(mid) <= ldb(d - 1 - mid)) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
 
 // Sort src[0..n) (n <= 1024) into dst with a bitonic network over 1024 LDS slots.
 template <class W>
@@ -228,67 +216,76 @@ __device__ void wave_copy(W& wt, const uint32_t* src, uint32_t* dst, int64_t n) 
     for (int64_t i = wt.lane; i < n; i += kWave) stk(dst + i, ldk(src + i));
 }
 
-// Merge a[0..na) and b[0..nb) into dst by tiles of 1024 outputs (16 per lane).
+// Number of `a` elements among the first d outputs (a first on ties), 32-bit indices.
+template <class LDA, class LDB>
+__device__ __forceinline__ int32_t merge_path32(LDA lda, int32_t na, LDB ldb, int32_t nb, int32_t d) {
+    int32_t lo = d > nb ? d - nb : 0;
+    int32_t hi = d < na ? d : na;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (lda(mid) <= ldb(d - 1 - mid)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Merge a[0..na) and b[0..nb) into dst (na + nb <= kWaveMergeLeaf) by tiles of 1024
+// outputs, 16 per lane, staged through the wave's LDS tile.
 template <class W>
-__device__ void wave_leaf_merge(W& wt, const uint32_t* a, int64_t na, const uint32_t* b, int64_t nb, uint32_t* dst) {
+__device__ void wave_leaf_merge(W& wt, const uint32_t* a, int32_t na, const uint32_t* b, int32_t nb, uint32_t* dst) {
     typename Program::WaveShared* sh = wt.sh;
-    const int l = wt.lane;
-    const int64_t total = na + nb;
-    auto ga = [a](int64_t i) { return ldk(a + i); };
-    auto gb = [b](int64_t i) { return ldk(b + i); };
-    constexpr int kTileOut = kWaveLeaf;
-    constexpr int kPer = kTileOut / kWave;  // 16
-    for (int64_t base = 0; base < total; base += (int64_t)kTileOut * 16) {
-        const int64_t ntiles_here = ((total - base) + kTileOut - 1) / kTileOut;
-        const int64_t nsplit = (ntiles_here < 16 ? ntiles_here : 16) + 1;
-        if (l < nsplit) {
-            int64_t d = base + (int64_t)l * kTileOut;
-            if (d > total) d = total;
-            sh->split[l] = merge_path(ga, na, gb, nb, d);
-        }
+    const int32_t l = wt.lane;
+    const int32_t total = na + nb;
+    auto ga = [a](int32_t i) { return ldk(a + i); };
+    auto gb = [b](int32_t i) { return ldk(b + i); };
+    constexpr int32_t kTileOut = kWaveLeaf;
+    constexpr int32_t kPer = kTileOut / kWave;  // 16
+    const int32_t ntiles = (total + kTileOut - 1) / kTileOut;  // <= 16
+    if (l <= ntiles) {
+        const int32_t d = (l * kTileOut < total) ? l * kTileOut : total;
+        sh->split[l] = merge_path32(ga, na, gb, nb, d);
+    }
+    wave_sync();
+    for (int32_t ti = 0; ti < ntiles; ++ti) {
+        const int32_t d0 = ti * kTileOut;
+        const int32_t d1 = (d0 + kTileOut < total) ? d0 + kTileOut : total;
+        const int32_t i0 = (int32_t)sh->split[ti], i1 = (int32_t)sh->split[ti + 1];
+        const int32_t la = i1 - i0, lb = (d1 - i1) - (d0 - i0);
+        const int32_t j0 = d0 - i0;
         wave_sync();
-        for (int64_t ti = 0; ti + 1 < nsplit; ++ti) {
-            const int64_t d0 = base + ti * kTileOut;
-            const int64_t d1 = (d0 + kTileOut < total) ? d0 + kTileOut : total;
-            const int64_t i0 = sh->split[ti], i1 = sh->split[ti + 1];
-            const int64_t la = i1 - i0, lb = (d1 - i1) - (d0 - i0);
-            const int64_t j0 = d0 - i0;
-            wave_sync();
-            for (int64_t i = l; i < la; i += kWave) sh->keys[i] = ldk(a + i0 + i);
-            for (int64_t i = l; i < lb; i += kWave) sh->keys[la + i] = ldk(b + j0 + i);
-            wave_sync();
-            const uint32_t* A = sh->keys;
-            const uint32_t* Bv = sh->keys + la;
-            const int64_t len = d1 - d0;
-            const int64_t od = (int64_t)l * kPer;
-            uint32_t outv[kPer];
-            int cnt = 0;
-            if (od < len) {
-                auto la_ = [A](int64_t i) { return A[i]; };
-                auto lb_ = [Bv](int64_t i) { return Bv[i]; };
-                int64_t i = merge_path(la_, la, lb_, lb, od);
-                int64_t j = od - i;
-                const int64_t oe = (od + kPer < len) ? od + kPer : len;
-                cnt = (int)(oe - od);
+        for (int32_t i = l; i < la; i += kWave) sh->keys[i] = ldk(a + i0 + i);
+        for (int32_t i = l; i < lb; i += kWave) sh->keys[la + i] = ldk(b + j0 + i);
+        wave_sync();
+        const uint32_t* A = sh->keys;
+        const uint32_t* Bv = sh->keys + la;
+        const int32_t len = d1 - d0;
+        const int32_t od = l * kPer;
+        uint32_t outv[kPer];
+        int32_t cnt = 0;
+        if (od < len) {
+            auto la_ = [A](int32_t i) { return A[i]; };
+            auto lb_ = [Bv](int32_t i) { return Bv[i]; };
+            int32_t i = merge_path32(la_, la, lb_, lb, od);
+            int32_t j = od - i;
+            cnt = (od + kPer < len) ? kPer : len - od;
 #pragma unroll
-                for (int o = 0; o < kPer; ++o) {
-                    if (o < cnt) {
-                        const bool takeA = i < la && (j >= lb || A[i] <= Bv[j]);
-                        outv[o] = takeA ? A[i] : Bv[j];
-                        if (takeA) ++i;
-                        else ++j;
-                    }
+            for (int32_t o = 0; o < kPer; ++o) {
+                if (o < cnt) {
+                    const bool takeA = i < la && (j >= lb || A[i] <= Bv[j]);
+                    outv[o] = takeA ? A[i] : Bv[j];
+                    i += takeA ? 1 : 0;
+                    j += takeA ? 0 : 1;
                 }
             }
-            wave_sync();  // every lane has finished reading the inputs
-#pragma unroll
-            for (int o = 0; o < kPer; ++o)
-                if (o < cnt) sh->keys[od + o] = outv[o];
-            wave_sync();
-            for (int64_t o = l; o < len; o += kWave) stk(dst + d0 + o, sh->keys[o]);
-            wave_sync();
         }
+        wave_sync();  // every lane has finished reading the inputs
+#pragma unroll
+        for (int32_t o = 0; o < kPer; ++o)
+            if (o < cnt) sh->keys[od + o] = outv[o];
+        wave_sync();
+        for (int32_t o = l; o < len; o += kWave) stk(dst + d0 + o, sh->keys[o]);
     }
+    wave_sync();
 }
 
 // gtap_cilksort.cu:40-50
@@ -324,7 +321,7 @@ __device__ void msort_task(W& wt) {
         case 1: {
             const uint32_t* src = into_t ? a : t;  // where the sorted halves are
             if (n <= kWaveMergeLeaf) {
-                wave_leaf_merge(wt, src, h, src + h, n - h, dst);
+                wave_leaf_merge(wt, src, (int32_t)h, src + h, (int32_t)(n - h), dst);
                 wt.finish();
                 return;
             }
@@ -363,7 +360,7 @@ __device__ void merge_task(W& wt) {
         return;
     }
     if (na + nb <= kWaveMergeLeaf) {
-        wave_leaf_merge(wt, a, na, b, nb, dst);
+        wave_leaf_merge(wt, a, (int32_t)na, b, (int32_t)nb, dst);
         wt.finish();
         return;
     }
@@ -377,7 +374,7 @@ __device__ void merge_task(W& wt) {
 }
 
 template <class W>
-__device__ void Program::run_wave(W& wt) {
+__device__ __forceinline__ void Program::run_wave(W& wt) {
     if (wt.func == F_MSORT) msort_task(wt);
     else if (wt.func == F_MERGE) merge_task(wt);
     else {

@@ -56,8 +56,15 @@ int gtap_get_runtime_error_report(gtap_error_report* out);
 /* ------------------------------------------------------------ profiling / statistics */
 
 /* Replaces gtap_visualize_profile (runtime/detail/thread/gtap_profile_thread.cuh:355-358):
- * writes ./profile/<app>_warp_statistics_working.csv for the last launch. */
+ * writes the reference's four CSV files (<app>_warp_timeline_having_task.csv,
+ * _warp_statistics_having_task.csv, _warp_timeline_working.csv, _warp_statistics_working.csv,
+ * same columns) for the last launch when timeline profiling is on, and always
+ * <app>_warp_counters.csv with the scheduler counters per worker. */
 void gtap_visualize_profile(const char* app_name);
+/* Timeline profiling on/off for the following launches, up to max_samples stamps per worker
+ * and kind (0 -> 4096); replaces the PROFILE / MAX_PROFILE_DATA compile-time switches
+ * (runtime/detail/common/gtap_config.cuh:22-26). */
+int gtap_set_profiling(int enable, uint32_t max_samples);
 /* Statistics of the last persistent launch (no reference counterpart; the reference
  * prints "Execution time" around its launch, examples/fib/fib.cu:36-51). */
 int gtap_get_last_stats(gtap_run_stats* out);
