@@ -415,6 +415,7 @@ int gtap_set_profiling(int enable, uint32_t max_samples) {
     R.prof_max = 0;
     if (!enable) return 0;
     if (max_samples == 0) max_samples = 4096;
+    max_samples = (max_samples + 1u) & ~1u;  // start/end stamps come in pairs
     const size_t n = (size_t)R.max_workers * max_samples;
     GTAP_HIP_TRY(hipMalloc(&R.prof_having, n * sizeof(uint64_t)));
     GTAP_HIP_TRY(hipMalloc(&R.prof_working, n * sizeof(uint64_t)));

@@ -1,0 +1,75 @@
+"""Timeline profiling: the four CSV files of the reference's profiler
+(runtime/detail/thread/gtap_profile_thread.cuh:29-108, 188-268), same columns, and their
+task totals agree with the scheduler's own counters."""
+import csv
+import os
+
+import pytest
+
+import gtap_amd as G
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def runtime():
+    G.gtap_set_config(timeout_ms=15000)
+    assert G.gtap_initialize() == 0
+    yield
+    assert G.gtap_finalize() == 0
+
+HEADERS = {
+    "warp_timeline_having_task": "warp_id,time_index,timestamp_ns,relative_time_ms,state,state_description",
+    "warp_statistics_having_task": "warp_id,total_samples,first_activity_ms,last_activity_ms,duration_ms",
+    "warp_timeline_working": "warp_id,time_index,timestamp_ns,relative_time_ms,state,state_description,tasks_in_batch",
+    "warp_statistics_working": "warp_id,total_samples,first_activity_ms,last_activity_ms,duration_ms,total_tasks",
+}
+
+
+def _read(d, app, kind):
+    with open(os.path.join(d, "profile", f"{app}_{kind}.csv")) as f:
+        lines = f.read().splitlines()
+    assert lines[0] == HEADERS[kind]
+    return list(csv.DictReader(lines))
+
+
+@pytest.mark.parametrize("max_samples", [1 << 14, 64])
+def test_profile_fib(tmp_path, monkeypatch, max_samples):
+    monkeypatch.chdir(tmp_path)
+    n = 22
+    print(f"case profile fib({n}) max_samples={max_samples}", flush=True)
+    assert G.gtap_set_profiling(True, max_samples) == 0
+    try:
+        v, st = G.fib(n)
+        assert v == O.fib(n)
+        G.lib().gtap_visualize_profile(b"fib")
+    finally:
+        assert G.gtap_set_profiling(False, 0) == 0
+    work = _read(tmp_path, "fib", "warp_statistics_working")
+    assert len(work) == st["workers"]
+    tl = _read(tmp_path, "fib", "warp_timeline_working")
+    for r in tl:
+        assert r["state_description"] == ("Working" if int(r["time_index"]) % 2 == 0 else "NotWorking")
+        assert int(r["time_index"]) < max_samples
+    total = sum(int(r["total_tasks"]) for r in work)
+    if max_samples >= 1 << 14:  # no worker ran out of samples: every executed task is counted
+        assert total == st["tasks_executed"] == 3 * O.fib(n + 1) - 2
+    else:
+        assert 0 < total <= st["tasks_executed"]
+    hav = _read(tmp_path, "fib", "warp_timeline_having_task")
+    assert hav and all(r["state_description"] in ("Having", "NotHaving") for r in hav)
+    hs = _read(tmp_path, "fib", "warp_statistics_having_task")
+    assert len(hs) == st["workers"]
+    for r in hs:
+        assert float(r["duration_ms"]) >= 0.0
+
+
+def test_profile_off_writes_counters_only(tmp_path, monkeypatch):
+    monkeypatch.chdir(tmp_path)
+    assert G.gtap_set_profiling(False, 0) == 0
+    v, st = G.fib(18)
+    assert v == O.fib(18)
+    G.lib().gtap_visualize_profile(b"fib_off")
+    assert os.path.exists(tmp_path / "profile" / "fib_off_warp_counters.csv")
+    assert not os.path.exists(tmp_path / "profile" / "fib_off_warp_timeline_working.csv")
