@@ -8,9 +8,11 @@ A step is one persistent launch (gtap_nqueens(14)) with everything resident on t
 each step ends with a stream synchronize). The roofline uses the average kernel duration
 from hipEvents recorded on the launch stream.
 
-Also reported (outside `value`): fib(30) tasks/s, the measured HBM-copy and atomic peaks of
-this GPU, and a CPU baseline (the reference's own sequential n-queens, built from its
-sources into oracle/_ref/, timed on this host).
+Also reported (outside `value`): fib(30) tasks/s; cilksort of 2^26 uint32 keys; BFS on an
+R-MAT scale-22 graph; the 4096-job mixed stream (batched, per-job launches, served); the
+measured HBM-copy and atomic peaks of this GPU and the cost of fence vs store-wait publication;
+and a CPU baseline (the reference's own sequential n-queens, built from its sources into
+oracle/_ref/, timed on this host).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -137,6 +139,42 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
                          "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas}}
 
 
+def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024, serve_rate: float = 16000.0) -> dict:
+    """The 4096-job mixed stream (fib / n-queens / sort jobs, fixed seed): one resident batched
+    launch against one launch per job, and served arrivals (Poisson at `serve_rate` jobs/s)
+    into the running kernel. Results of the three forms must agree job for job."""
+    import numpy as np
+    st = gtap_amd.Stream(seed=seed, njobs=njobs)
+    try:
+        bat = []
+        for _ in range(3):
+            res, lat, s = st.run_batched()
+            bat.append((s["kernel_ms"], res, lat))
+        kms = statistics.median(b[0] for b in bat)
+        res_b, lat_b = bat[-1][1], bat[-1][2]
+        res_p, lat_p, tot_p = st.run_per_job()
+        if not np.array_equal(res_b, res_p):
+            raise RuntimeError("stream: batched and per-job results differ")
+        res_s, lat_s, tot_s, _ = st.serve(rate=serve_rate, arrival_seed=5)
+        if not np.array_equal(res_b, res_s):
+            raise RuntimeError("stream: served and batched results differ")
+        out = {
+            "jobs": njobs,
+            "batched": {"kernel_ms_median": kms, "jobs_per_s": njobs / (kms / 1e3),
+                        "p50_ms": float(np.percentile(lat_b, 50)), "p99_ms": float(np.percentile(lat_b, 99))},
+            "per_job_launch": {"total_ms": tot_p, "jobs_per_s": njobs / (tot_p / 1e3),
+                               "p50_ms": float(np.percentile(lat_p, 50)), "p99_ms": float(np.percentile(lat_p, 99))},
+            "served": {"rate_jobs_per_s": serve_rate, "total_ms": tot_s,
+                       "p50_ms": float(np.percentile(lat_s, 50)), "p99_ms": float(np.percentile(lat_s, 99))},
+        }
+        out["batched_over_per_job"] = out["batched"]["jobs_per_s"] / out["per_job_launch"]["jobs_per_s"]
+        log(f"stream: batched {out['batched']['jobs_per_s']:.0f} jobs/s, per-job {out['per_job_launch']['jobs_per_s']:.0f}"
+            f" jobs/s, served p99 {out['served']['p99_ms']:.3f} ms")
+        return out
+    finally:
+        st.close()
+
+
 def load_traffic(name: str):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
     p = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
@@ -208,6 +246,8 @@ def main() -> int:
 
     extra_sort = bench_cilksort(gtap_amd, roofline, hbm_peak_meas)
     extra_bfs = bench_bfs(gtap_amd, roofline, hbm_peak_meas)
+    extra_stream = bench_stream(gtap_amd)
+    fences = gtap_amd.bench_fences(max(4, workers // 4 * 4), 64)
 
     cpu = None if args.no_cpu else cpu_baseline(nq_tasks)
     gtap_amd.gtap_finalize()
@@ -255,6 +295,8 @@ def main() -> int:
                       "tasks_per_s": fib_tasks / (fib_med / 1e3)},
             "cilksort_2e26": extra_sort,
             "bfs_rmat22": extra_bfs,
+            "stream_4096": extra_stream,
+            "publication_order_ns_per_wave": fences,
             "device": info,
         },
     }

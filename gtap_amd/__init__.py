@@ -130,6 +130,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_get_runtime_error_report": ([ctypes.POINTER(ErrorReport)], _i),
     "gtap_visualize_profile": ([ctypes.c_char_p], None),
     "gtap_set_profiling": ([_i, _u32], _i),
+    "gtap_bench_fences": ([_u32, _i, ctypes.POINTER(ctypes.c_double)], _i),
     "gtap_get_last_stats": ([ctypes.POINTER(RunStats)], _i),
     "gtap_get_worker_stats": ([_vp, _u32, ctypes.POINTER(_u32)], _i),
     "gtap_device_info": ([ctypes.c_char_p, _i, ctypes.POINTER(_i), ctypes.POINTER(_u64)], _i),
@@ -266,6 +267,13 @@ def nqueens(n: int) -> tuple[int, dict]:
     s = RunStats()
     _check(lib().gtap_nqueens(n, ctypes.byref(r), ctypes.byref(s)), f"gtap_nqueens({n})")
     return r.value, s.as_dict()
+
+
+def bench_fences(waves: int = 8192, iters: int = 64) -> dict:
+    """ns per publication per wave: release fence, acquire fence, store-completion wait."""
+    out = (ctypes.c_double * 3)()
+    _check(lib().gtap_bench_fences(waves, iters, out), "gtap_bench_fences")
+    return {"waves": waves, "release_fence_ns": out[0], "acquire_fence_ns": out[1], "store_wait_ns": out[2]}
 
 
 def bench_hbm_copy(nbytes: int = 2 << 30, reps: int = 3) -> float:

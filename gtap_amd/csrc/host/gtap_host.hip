@@ -132,9 +132,14 @@ __global__ void reset_kernel(GtapDev dev) {
         dev.pool_ctl[i] = (i == (uint64_t)dev.nshards * 32) ? 64u : 0u;
 }
 
-int reset_launch_state(const GtapDev& dev, hipStream_t stream) {
+int enqueue_reset(const GtapDev& dev, hipStream_t stream) {
     hipLaunchKernelGGL(reset_kernel, dim3(1024), dim3(256), 0, stream, dev);
     GTAP_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int reset_launch_state(const GtapDev& dev, hipStream_t stream) {
+    if (int st = enqueue_reset(dev, stream)) return st;
     Runtime& R = rt();
     if (R.report) memset(R.report, 0, sizeof(gtap_error_report));
     return 0;

@@ -80,6 +80,8 @@ int hip_status(hipError_t e, const char* what);
 gtap_job make_job(uint32_t func, uint32_t queue, const void* args, size_t nbytes);
 GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job* jobs, uint32_t job_ring);
 int reset_launch_state(const GtapDev& dev, hipStream_t stream);
+// Enqueues the per-launch state reset kernel only (usable under stream capture).
+int enqueue_reset(const GtapDev& dev, hipStream_t stream);
 int finish_launch(const GtapDev& dev, uint32_t blocks, gtap_run_stats* stats);
 int occupancy_blocks(const void* kernel, uint32_t requested, uint32_t* blocks_out);
 
@@ -90,9 +92,10 @@ int occupancy_blocks(const void* kernel, uint32_t requested, uint32_t* blocks_ou
     } while (0)
 
 // Persistent launch of a thread-mode program over `njobs` jobs of the job table `jobs`.
+// Checks a thread-mode launch against the provisioned runtime and picks its workgroup count
+// (every co-resident workgroup unless gtap_config.blocks asks for fewer).
 template <class P>
-int launch_thread(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs,
-                  uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream = nullptr) {
+int thread_launch_shape(uint32_t job_ring, uint32_t* blocks_out) {
     Runtime& R = rt();
     if (!R.initialized) {
         int st = gtap_initialize();
@@ -101,7 +104,6 @@ int launch_thread(const typename P::Args& args, uint32_t term_mode, uint32_t njo
     if ((uint32_t)P::kQueues > R.nqueues) return invalid_argument("program needs more EPAQ queues than provisioned");
     if (P::kFrameBytes + kHdrBytes > R.stride) return invalid_argument("task frame exceeds the record stride");
     if (job_ring == 0 || job_ring > R.nanchors) return invalid_argument("job ring exceeds the anchor capacity");
-    if (!stream) stream = R.stream;
     static uint32_t max_blocks = 0;
     const void* kfn = reinterpret_cast<const void*>(&thread_kernel<P>);
     if (max_blocks == 0) {
@@ -111,6 +113,17 @@ int launch_thread(const typename P::Args& args, uint32_t term_mode, uint32_t njo
     uint32_t blocks = max_blocks;
     if (R.cfg.blocks && R.cfg.blocks < blocks) blocks = R.cfg.blocks;
     if (blocks * 4 > R.max_workers) blocks = R.max_workers / 4;
+    *blocks_out = blocks;
+    return 0;
+}
+
+template <class P>
+int launch_thread(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs,
+                  uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream = nullptr) {
+    Runtime& R = rt();
+    uint32_t blocks = 0;
+    if (int st = thread_launch_shape<P>(job_ring, &blocks)) return st;
+    if (!stream) stream = R.stream;
     GtapDev dev = make_dev(blocks * 4, term_mode, njobs, jobs, job_ring);
     int st = reset_launch_state(dev, stream);
     if (st) return st;
@@ -120,6 +133,70 @@ int launch_thread(const typename P::Args& args, uint32_t term_mode, uint32_t njo
     GTAP_HIP_TRY(hipEventRecord(R.ev1, stream));
     GTAP_HIP_TRY(hipStreamSynchronize(stream));
     return finish_launch(dev, blocks, stats);
+}
+
+// A thread-mode launch held as an instantiated HIP graph: [copy of the job table from a
+// device-resident template into the runtime's job table, state reset kernel, persistent
+// kernel]. Replaying it costs one hipGraphLaunch. `dev` is the launch's argument block: the
+// graph is valid while make_dev() returns the same block (same buffers, same configuration).
+struct ThreadGraph {
+    hipGraphExec_t exec = nullptr;
+    GtapDev dev{};
+    uint32_t blocks = 0;
+};
+
+inline void destroy_graph(ThreadGraph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+}
+
+template <class P>
+int capture_thread(ThreadGraph& g, const typename P::Args& args, uint32_t term_mode, uint32_t njobs,
+                   const gtap_job* jobs_template_dev, uint32_t job_ring) {
+    Runtime& R = rt();
+    uint32_t blocks = 0;
+    if (int st = thread_launch_shape<P>(job_ring, &blocks)) return st;
+    GtapDev dev = make_dev(blocks * 4, term_mode, njobs, R.jobs, job_ring);
+    if (g.exec && g.blocks == blocks && memcmp(&g.dev, &dev, sizeof(GtapDev)) == 0) return 0;
+    destroy_graph(g);
+    hipStream_t s = R.stream;
+    GTAP_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipMemcpyAsync(R.jobs, jobs_template_dev, (size_t)njobs * sizeof(gtap_job),
+                                  hipMemcpyDeviceToDevice, s);
+    int st = (e == hipSuccess) ? enqueue_reset(dev, s) : hip_status(e, "hipMemcpyAsync (capture)");
+    if (st == 0) {
+        hipLaunchKernelGGL(thread_kernel<P>, dim3(blocks), dim3(256), 0, s, dev, args);
+        e = hipGetLastError();
+        if (e != hipSuccess) st = hip_status(e, "thread_kernel (capture)");
+    }
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(s, &graph);
+    if (st) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return hip_status(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        g.exec = nullptr;
+        return hip_status(e, "hipGraphInstantiate");
+    }
+    g.dev = dev;
+    g.blocks = blocks;
+    return 0;
+}
+
+// Replays a captured launch; stats->kernel_ms covers the whole graph (copy, reset, kernel).
+inline int replay_thread(const ThreadGraph& g, gtap_run_stats* stats) {
+    Runtime& R = rt();
+    if (!g.exec) return invalid_argument("replay_thread: no captured graph");
+    if (R.report) memset(R.report, 0, sizeof(gtap_error_report));
+    GTAP_HIP_TRY(hipEventRecord(R.ev0, R.stream));
+    GTAP_HIP_TRY(hipGraphLaunch(g.exec, R.stream));
+    GTAP_HIP_TRY(hipEventRecord(R.ev1, R.stream));
+    GTAP_HIP_TRY(hipStreamSynchronize(R.stream));
+    return finish_launch(g.dev, g.blocks, stats);
 }
 
 // One job in slot 0 of the device job table; the root's return value goes to *result.

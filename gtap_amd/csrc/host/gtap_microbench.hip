@@ -1,6 +1,7 @@
 // gtap_microbench.hip -- measured peaks of this GPU, for the roofline fractions the
 // benchmark reports: HBM streaming-copy bandwidth and agent-scope atomic throughput
-// (relaxed fetch-add, each lane on its own 64-byte-aligned word of a 256 MiB region).
+// (relaxed fetch-add, each lane on its own 64-byte-aligned word of a 256 MiB region);
+// and the cost of the publication orders the scheduler could use (gtap_bench_fences).
 #include <hip/hip_runtime.h>
 
 #include "gtap_launch.hpp"
@@ -18,6 +19,28 @@ __global__ void atomic_kernel(uint32_t* buf, int iters, uint32_t nslots) {
     for (int i = 0; i < iters; ++i) {
         const uint32_t slot = (g * 16u + (uint32_t)i * 977u) % nslots;
         __hip_atomic_fetch_add(buf + (size_t)slot * 16, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// One publication per iteration per wave: a data store, then an order, then a relaxed
+// agent-scope flag store. MODE 0: agent-scope release fence; 1: relaxed flag load followed by an
+// agent-scope acquire fence; 2: wait for the wave's own stores to complete (s_waitcnt vmcnt(0)),
+// the order the scheduler uses.
+template <int MODE>
+__global__ void __launch_bounds__(256) fence_kernel(uint32_t* buf, int iters) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if ((threadIdx.x & 63u) != 0u) return;
+    uint32_t* p = buf + (size_t)wave * 16u;
+    uint32_t acc = 0;
+    for (int i = 0; i < iters; ++i) {
+        __hip_atomic_store(p + 1, (uint32_t)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if constexpr (MODE == 0) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if constexpr (MODE == 1) {
+            acc += __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        if constexpr (MODE == 2) __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __hip_atomic_store(p, (uint32_t)i + acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -81,5 +104,38 @@ extern "C" int gtap_bench_atomics(int reps, double* gops) {
     (void)hipEventDestroy(e1);
     (void)hipFree(buf);
     *gops = best;
+    return 0;
+}
+
+// Time per publication per wave, in ns, with `waves` waves each doing `iters` publications
+// concurrently: out_ns[0] release fence, out_ns[1] acquire fence, out_ns[2] store completion wait.
+extern "C" int gtap_bench_fences(uint32_t waves, int iters, double* out_ns) {
+    if (waves < 4 || waves > (1u << 20) || (waves & 3u) || iters < 1 || !out_ns)
+        return gtap::host::invalid_argument("gtap_bench_fences: waves must be a multiple of 4 in [4, 2^20]");
+    uint32_t* buf = nullptr;
+    GTAP_HIP_TRY(hipMalloc(&buf, (size_t)waves * 64));
+    GTAP_HIP_TRY(hipMemset(buf, 0, (size_t)waves * 64));
+    hipEvent_t e0, e1;
+    GTAP_HIP_TRY(hipEventCreate(&e0));
+    GTAP_HIP_TRY(hipEventCreate(&e1));
+    const dim3 grid(waves / 4), block(256);
+    for (int mode = 0; mode < 3; ++mode) {
+        double best = 1e300;
+        for (int r = 0; r < 3; ++r) {
+            GTAP_HIP_TRY(hipEventRecord(e0, nullptr));
+            if (mode == 0) hipLaunchKernelGGL(fence_kernel<0>, grid, block, 0, nullptr, buf, iters);
+            if (mode == 1) hipLaunchKernelGGL(fence_kernel<1>, grid, block, 0, nullptr, buf, iters);
+            if (mode == 2) hipLaunchKernelGGL(fence_kernel<2>, grid, block, 0, nullptr, buf, iters);
+            GTAP_HIP_TRY(hipEventRecord(e1, nullptr));
+            GTAP_HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0;
+            GTAP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (r > 0 && ms < best) best = ms;
+        }
+        out_ns[mode] = best * 1e6 / iters;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
     return 0;
 }

@@ -450,7 +450,11 @@ struct gtap_stream {
     uint32_t* d_work = nullptr;     // sorted in place
     uint32_t* d_tmp = nullptr;
     std::vector<gtap_job> table;    // job table template
+    gtap_job* d_table = nullptr;    // the same, device-resident (source of the batched graph's copy)
+    host::ThreadGraph graph;        // batched launch, captured on first use
 };
+
+extern "C" int gtap_stream_destroy(gtap_stream* S);
 
 extern "C" int gtap_stream_create(uint64_t seed, int32_t njobs, gtap_stream** out) {
     if (!out || njobs <= 0) return host::invalid_argument("gtap_stream_create");
@@ -484,7 +488,12 @@ extern "C" int gtap_stream_create(uint64_t seed, int32_t njobs, gtap_stream** ou
     std::vector<uint32_t> keys((size_t)std::max<int64_t>(S->total_keys, 1));
     for (int32_t i = 0; i < njobs; ++i)
         if (S->kind[i] == 2) gtap_gen_keys(S->kseed[i], (uint64_t)S->param[i], keys.data() + S->key_off[i]);
-    GTAP_HIP_TRY(hipMemcpy(S->d_src, keys.data(), bytes, hipMemcpyHostToDevice));
+    e = hipMemcpy(S->d_src, keys.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&S->d_table, (size_t)njobs * sizeof(gtap_job));
+    if (e != hipSuccess) {
+        gtap_stream_destroy(S);
+        return host::hip_status(e, "gtap_stream_create upload");
+    }
     S->table.resize(njobs);
     for (int32_t i = 0; i < njobs; ++i) {
         gtap_job j;
@@ -507,12 +516,19 @@ extern "C" int gtap_stream_create(uint64_t seed, int32_t njobs, gtap_stream** ou
         }
         S->table[i] = j;
     }
+    e = hipMemcpy(S->d_table, S->table.data(), (size_t)njobs * sizeof(gtap_job), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        gtap_stream_destroy(S);
+        return host::hip_status(e, "gtap_stream_create upload");
+    }
     *out = S;
     return 0;
 }
 
 extern "C" int gtap_stream_destroy(gtap_stream* S) {
     if (!S) return 0;
+    host::destroy_graph(S->graph);
+    (void)hipFree(S->d_table);
     (void)hipFree(S->d_src);
     (void)hipFree(S->d_work);
     (void)hipFree(S->d_tmp);
@@ -544,18 +560,20 @@ static int stream_reset_keys(gtap_stream* S) {
     return 0;
 }
 
-// All jobs in one persistent launch. results[i] = root return value (sort jobs: n);
-// latency_ms[i] = completion time of job i after the launch start (all jobs arrive at the
-// start). stats->kernel_ms is the launch time.
+// All jobs in one persistent launch, replayed from a HIP graph captured on the first call
+// (job-table copy, state reset, persistent kernel; re-captured when the runtime's buffers or
+// configuration change). results[i] = root return value (sort jobs: n); latency_ms[i] =
+// completion time of job i after the launch start (all jobs arrive at the start).
+// stats->kernel_ms is the time of the whole graph.
 extern "C" int gtap_stream_run_batched(gtap_stream* S, int64_t* results, double* latency_ms, gtap_run_stats* stats) {
     if (!S) return host::invalid_argument("gtap_stream_run_batched");
     host::Runtime& R = host::rt();
     int st = stream_reset_keys(S);
     if (st) return st;
-    GTAP_HIP_TRY(hipMemcpyAsync(R.jobs, S->table.data(), S->table.size() * sizeof(gtap_job), hipMemcpyHostToDevice,
-                                R.stream));
-    st = host::launch_thread<StreamProgram>(StreamProgram::Args{0}, GTAP_TERM_JOBS, (uint32_t)S->njobs, R.jobs,
-                                            (uint32_t)S->njobs, stats);
+    st = host::capture_thread<StreamProgram>(S->graph, StreamProgram::Args{0}, GTAP_TERM_JOBS, (uint32_t)S->njobs,
+                                             S->d_table, (uint32_t)S->njobs);
+    if (st) return st;
+    st = host::replay_thread(S->graph, stats);
     if (st) return st;
     std::vector<gtap_job> out(S->njobs);
     GTAP_HIP_TRY(hipMemcpy(out.data(), R.jobs, out.size() * sizeof(gtap_job), hipMemcpyDeviceToHost));

@@ -77,6 +77,10 @@ int gtap_device_info(char* name, int name_len, int* num_cus, uint64_t* hbm_bytes
  * fetch-add throughput on distinct 64-byte words (Gop/s). Best of `reps` launches. */
 int gtap_bench_hbm_copy(uint64_t bytes, int reps, double* gbps);
 int gtap_bench_atomics(int reps, double* gops);
+/* Time per publication per wave (ns) with `waves` concurrent waves: out_ns[0] an agent-scope
+ * release fence, out_ns[1] an agent-scope acquire fence, out_ns[2] a store-completion wait
+ * (the order this runtime publishes with). No reference counterpart; a measurement. */
+int gtap_bench_fences(uint32_t waves, int iters, double* out_ns);
 
 /* ------------------------------------------------------------ device memory helpers */
 /* For callers that only hold host pointers (ctypes, plain C). */

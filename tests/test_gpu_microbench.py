@@ -1,0 +1,37 @@
+"""The measurement entry points return finite, positive figures (they feed the benchmark's
+roofline fractions and the publication-order choice of the scheduler)."""
+import math
+
+import pytest
+
+import gtap_amd
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def runtime():
+    assert gtap_amd.gtap_initialize() == 0
+    yield
+    assert gtap_amd.gtap_finalize() == 0
+
+
+def test_hbm_copy_and_atomics():
+    print("case bench_hbm_copy/bench_atomics", flush=True)
+    g = gtap_amd.bench_hbm_copy(256 << 20, 2)
+    a = gtap_amd.bench_atomics(2)
+    assert 100.0 < g < 20000.0
+    assert a > 1.0 and math.isfinite(a)
+
+
+@pytest.mark.parametrize("waves", [64, 8192])
+def test_fences(waves):
+    print(f"case bench_fences waves={waves}", flush=True)
+    f = gtap_amd.bench_fences(waves, 16)
+    for k in ("release_fence_ns", "acquire_fence_ns", "store_wait_ns"):
+        assert f[k] > 0.0 and math.isfinite(f[k])
+
+
+def test_fences_rejects_bad_sizes():
+    with pytest.raises(Exception):
+        gtap_amd.bench_fences(6, 16)

@@ -78,3 +78,38 @@ def test_stream_full_batched():
         assert np.array_equal(res, res_s)
     finally:
         st.close()
+
+
+def test_stream_graph_replays():
+    """The batched form is a captured HIP graph: replays with eager launches and host
+    synchronisation in between each give the oracle's results, sorted keys included."""
+    st = gtap_amd.Stream(seed=99, njobs=96)
+    try:
+        for rep in range(4):
+            print(f"case stream graph replay {rep}", flush=True)
+            res, lat, stats = st.run_batched()
+            check_stream(st, res)
+            assert stats["jobs_taken"] == 96
+            v, _ = gtap_amd.fib(15 + rep)  # eager work on the same stream between replays
+            assert v == oracle.fib(15 + rep)
+    finally:
+        st.close()
+
+
+def test_stream_graph_recaptured_after_config_change():
+    st = gtap_amd.Stream(seed=5, njobs=32)
+    try:
+        print("case stream graph before profiling switch", flush=True)
+        res0, _, _ = st.run_batched()
+        assert gtap_amd.gtap_set_profiling(True, 256) == 0
+        try:
+            print("case stream graph with profiling on", flush=True)
+            res1, _, _ = st.run_batched()
+        finally:
+            assert gtap_amd.gtap_set_profiling(False, 0) == 0
+        print("case stream graph after profiling switch", flush=True)
+        res2, _, _ = st.run_batched()
+        check_stream(st, res2)
+        assert np.array_equal(res0, res1) and np.array_equal(res0, res2)
+    finally:
+        st.close()
