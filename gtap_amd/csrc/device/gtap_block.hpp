@@ -283,7 +283,10 @@ __device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me,
 template <class P>
 __device__ __forceinline__ uint32_t block_admit(const GtapDev& dev, BlockCtx* c, bool& idle_registered) {
     uint32_t avail = dev.njobs;
-    if (dev.term_mode == GTAP_TERM_SERVE) avail = (uint32_t)ld32_sys(dev.host_ctl + HCTL_JOB_AVAIL * 32);
+    if (dev.term_mode == GTAP_TERM_SERVE) {
+        host_poll(dev.ctl, dev.host_ctl);
+        avail = (uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32);
+    }
     int32_t* nextp = dev.ctl + CTL_JOB_NEXT * 32;
     const uint32_t next = (uint32_t)ld32(nextp);
     if (next >= avail) return 0;

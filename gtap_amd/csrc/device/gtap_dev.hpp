@@ -33,7 +33,10 @@ enum CtlIndex : int {
     CTL_JOB_NEXT = 4,   // next job index to admit
     CTL_JOBS_DONE = 5,  // completed jobs
     CTL_LIVE_JOBS = 6,  // admitted and not yet completed
-    CTL_COUNT = 8
+    CTL_HOST_AVAIL = 7,   // serve mode: device mirror of HCTL_JOB_AVAIL
+    CTL_HOST_STOP = 8,    // serve mode: device mirror of HCTL_STOP
+    CTL_HOST_POLL_T = 9,  // serve mode: wall-clock tick of the last host poll (64-bit)
+    CTL_COUNT = 12
 };
 
 // Host-visible control (pinned host memory, read with system-scope loads).
@@ -228,6 +231,26 @@ template <typename T>
 __device__ __forceinline__ void vst(T* p, T v) { *reinterpret_cast<volatile T*>(p) = v; }
 
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
+
+// Serve mode. The host's control words sit in pinned host memory; a system-scope read of
+// them crosses the host bus and takes microseconds. Thousands of idle waves reading them on
+// every idle iteration would queue on that bus and stall the idle waves that should be
+// stealing, so at most one caller per kHostPollTicks (claimed by a CAS on the last poll
+// time) reads them and mirrors them into device control words, which every wave reads.
+// Call from one lane.
+constexpr uint64_t kHostPollTicks = 100;  // 1 us of the 100 MHz wall clock
+__device__ __forceinline__ void host_poll(int32_t* ctl, const int32_t* host_ctl) {
+    uint64_t* lastp = reinterpret_cast<uint64_t*>(ctl + CTL_HOST_POLL_T * 32);
+    const uint64_t now = now_ticks();
+    uint64_t last = ld64(lastp);
+    if (now - last < kHostPollTicks) return;
+    if (!atomic_cas(lastp, last, now)) return;
+    const int32_t stop = ld32_sys(host_ctl + HCTL_STOP * 32);
+    complete_stores();  // the stop flag is read before the job count (the host writes them in the other order)
+    const int32_t avail = ld32_sys(host_ctl + HCTL_JOB_AVAIL * 32);
+    __hip_atomic_fetch_max(ctl + CTL_HOST_AVAIL * 32, avail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (stop) st32(ctl + CTL_HOST_STOP * 32, 1);
+}
 
 __device__ __forceinline__ uint32_t xorshift32(uint32_t& s) {
     s ^= s << 13;

@@ -562,7 +562,10 @@ __device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtx<P::kQ
                                               uint32_t& tid_out, bool& idle_registered) {
     const int lane = lane_id();
     uint32_t avail = dev.njobs;
-    if (dev.term_mode == GTAP_TERM_SERVE) avail = (uint32_t)ld32_sys(dev.host_ctl + HCTL_JOB_AVAIL * 32);
+    if (dev.term_mode == GTAP_TERM_SERVE) {
+        if (lane == 0) host_poll(dev.ctl, dev.host_ctl);
+        avail = uniform((uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32));
+    }
     int32_t* nextp = dev.ctl + CTL_JOB_NEXT * 32;
     uint32_t next = uniform((uint32_t)ld32(nextp));
     if (next >= avail) return 0;
@@ -822,9 +825,10 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 if (lane == 0) {
                     flags = ld32(dev.ctl + CTL_DONE * 32) | (ld32(dev.ctl + CTL_ABORT * 32) << 1);
                     if (dev.term_mode == GTAP_TERM_SERVE && (idle_iter & 15u) == 0) {
-                        const int32_t stop = ld32_sys(dev.host_ctl + HCTL_STOP * 32);
+                        host_poll(dev.ctl, dev.host_ctl);
+                        const int32_t stop = ld32(dev.ctl + CTL_HOST_STOP * 32);
                         if (stop) {
-                            const uint32_t avail = (uint32_t)ld32_sys(dev.host_ctl + HCTL_JOB_AVAIL * 32);
+                            const uint32_t avail = (uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32);
                             const uint32_t next = (uint32_t)ld32(dev.ctl + CTL_JOB_NEXT * 32);
                             const int32_t live = ld32(dev.ctl + CTL_LIVE_JOBS * 32);
                             if (next >= avail && live == 0) {
