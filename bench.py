@@ -191,6 +191,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="time the n-queens workload only (no fib/cilksort/BFS/stream extras); for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
@@ -237,17 +239,19 @@ def main() -> int:
     # fib(30): the other scheduler-bound config
     fib_tasks = FIB30_TASKS
     fib_ms = []
-    for _ in range(5):
+    for _ in range(0 if args.headline_only else 5):
         fv, fs = gtap_amd.fib(30)
         if fv != FIB30 or fs["tasks_spawned"] + 1 != fib_tasks:
             raise RuntimeError(f"fib(30) returned {fv} with {fs['tasks_spawned'] + 1} tasks")
         fib_ms.append(fs["kernel_ms"])
-    fib_med = statistics.median(fib_ms)
+    fib_med = statistics.median(fib_ms) if fib_ms else None
 
-    extra_sort = bench_cilksort(gtap_amd, roofline, hbm_peak_meas)
-    extra_bfs = bench_bfs(gtap_amd, roofline, hbm_peak_meas)
-    extra_stream = bench_stream(gtap_amd)
-    fences = gtap_amd.bench_fences(max(4, workers // 4 * 4), 64)
+    extra_sort = extra_bfs = extra_stream = fences = None
+    if not args.headline_only:
+        extra_sort = bench_cilksort(gtap_amd, roofline, hbm_peak_meas)
+        extra_bfs = bench_bfs(gtap_amd, roofline, hbm_peak_meas)
+        extra_stream = bench_stream(gtap_amd)
+        fences = gtap_amd.bench_fences(max(4, workers // 4 * 4), 64)
 
     cpu = None if args.no_cpu else cpu_baseline(nq_tasks)
     gtap_amd.gtap_finalize()
@@ -291,8 +295,8 @@ def main() -> int:
                 "frac": atomics_per_s / 1e9 / atomic_peak_meas if atomic_peak_meas else None,
             },
             "hbm_measured_copy_gbps": hbm_peak_meas,
-            "fib30": {"kernel_ms_median": fib_med, "tasks": fib_tasks,
-                      "tasks_per_s": fib_tasks / (fib_med / 1e3)},
+            "fib30": None if fib_med is None else {"kernel_ms_median": fib_med, "tasks": fib_tasks,
+                                                   "tasks_per_s": fib_tasks / (fib_med / 1e3)},
             "cilksort_2e26": extra_sort,
             "bfs_rmat22": extra_bfs,
             "stream_4096": extra_stream,
