@@ -174,6 +174,79 @@ __device__ __forceinline__ bool atomic_cas(T* p, T& expected, T desired) {
 __device__ __forceinline__ void complete_stores() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 __device__ __forceinline__ void compiler_barrier() { __asm__ volatile("" ::: "memory"); }
 
+// 16-byte device-coherent access to task records. A raw buffer access over the record pool
+// with the sc1 cache policy is the same device-scope coherence as the 8-byte agent-scope
+// relaxed atomics above (global_load/store ... sc1), at twice the width: one transaction per
+// 16 bytes instead of one per 8. Offsets are 32-bit bytes from the pool base, so the pool
+// (records and job anchors) stays below 4 GiB. Each call is followed by a compiler barrier so
+// no global access is moved across it.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kCachePolicySc1 = 16;            // buffer instruction cache-policy operand: sc1
+constexpr uint32_t kBufferRsrcWord3 = 0x00020000u;  // raw (untyped) buffer, gfx9 family
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pool_rsrc(uint8_t* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)0xFFFFFFFF, (int)kBufferRsrcWord3);
+}
+__device__ __forceinline__ u32x4 ld128(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, kCachePolicySc1);
+    compiler_barrier();
+    return v;
+}
+__device__ __forceinline__ void st128(__amdgpu_buffer_rsrc_t rs, uint32_t off, u32x4 v) {
+    compiler_barrier();
+    __builtin_amdgcn_raw_buffer_store_b128(v, rs, (int)off, 0, kCachePolicySc1);
+    compiler_barrier();
+}
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u32x2 ld64b(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, kCachePolicySc1);
+    compiler_barrier();
+    return v;
+}
+__device__ __forceinline__ void st64b(__amdgpu_buffer_rsrc_t rs, uint32_t off, u32x2 v) {
+    compiler_barrier();
+    __builtin_amdgcn_raw_buffer_store_b64(v, rs, (int)off, 0, kCachePolicySc1);
+    compiler_barrier();
+}
+__device__ __forceinline__ uint32_t ld32b(__amdgpu_buffer_rsrc_t rs, uint32_t off) {
+    const uint32_t v = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, kCachePolicySc1);
+    compiler_barrier();
+    return v;
+}
+__device__ __forceinline__ void st32b(__amdgpu_buffer_rsrc_t rs, uint32_t off, uint32_t v) {
+    compiler_barrier();
+    __builtin_amdgcn_raw_buffer_store_b32(v, rs, (int)off, 0, kCachePolicySc1);
+    compiler_barrier();
+}
+
+// The first FW frame words of the record at byte offset `rec_off`, to and from registers,
+// in 16-byte pieces.
+template <int FW>
+__device__ __forceinline__ void frame_words_st(__amdgpu_buffer_rsrc_t rs, uint32_t rec_off, const uint32_t* f) {
+#pragma unroll
+    for (int i = 0; i + 4 <= FW; i += 4) st128(rs, rec_off + kHdrBytes + 4 * i, u32x4{f[i], f[i + 1], f[i + 2], f[i + 3]});
+    constexpr int base = FW - FW % 4;
+    if constexpr (FW % 4 >= 2) st64b(rs, rec_off + kHdrBytes + 4 * base, u32x2{f[base], f[base + 1]});
+    if constexpr (FW % 2 == 1) st32b(rs, rec_off + kHdrBytes + 4 * (FW - 1), f[FW - 1]);
+}
+template <int FW>
+__device__ __forceinline__ void frame_words_ld(__amdgpu_buffer_rsrc_t rs, uint32_t rec_off, uint32_t* f) {
+#pragma unroll
+    for (int i = 0; i + 4 <= FW; i += 4) {
+        const u32x4 v = ld128(rs, rec_off + kHdrBytes + 4 * i);
+        f[i] = v.x;
+        f[i + 1] = v.y;
+        f[i + 2] = v.z;
+        f[i + 3] = v.w;
+    }
+    constexpr int base = FW - FW % 4;
+    if constexpr (FW % 4 >= 2) {
+        const u32x2 v = ld64b(rs, rec_off + kHdrBytes + 4 * base);
+        f[base] = v.x;
+        f[base + 1] = v.y;
+    }
+    if constexpr (FW % 2 == 1) f[FW - 1] = ld32b(rs, rec_off + kHdrBytes + 4 * (FW - 1));
+}
+
 // Typed coherent loads/stores of 4- or 8-byte aligned fields inside records.
 template <typename T>
 __device__ __forceinline__ T ldT(const void* p) {
@@ -214,6 +287,11 @@ __device__ __forceinline__ int rank_in(uint64_t mask) {
 __device__ __forceinline__ int popc(uint64_t m) { return __popcll(m); }
 __device__ __forceinline__ int first_lane(uint64_t m) { return __ffsll((long long)m) - 1; }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
 __device__ __forceinline__ int32_t shfl(int32_t v, int src) { return __shfl(v, src, kWave); }
 __device__ __forceinline__ uint32_t shfl(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, kWave); }
 __device__ __forceinline__ uint64_t shfl(uint64_t v, int src) {

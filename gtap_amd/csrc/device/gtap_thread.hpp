@@ -43,7 +43,32 @@ constexpr uint32_t kMargin = 1024;          // deque slots kept free beyond the 
 
 enum Outcome : uint32_t { OUT_RUNNING = 0, OUT_FINISHED = 1, OUT_SUSPENDED = 2 };
 
-template <int Q>
+// A program may name one EPAQ queue whose tasks run on a whole wavefront (P::kWaveQueue).
+template <class P, class = void>
+struct HasWaveQueue {
+    static constexpr int value = -1;
+};
+template <class P>
+struct HasWaveQueue<P, decltype((void)P::kWaveQueue)> {
+    static constexpr int value = P::kWaveQueue;
+};
+
+// Per-wave scheduler state in LDS.
+//
+// The private end of each queue (positions [bottom, tail)) is mirrored in a ring of 64
+// slots indexed by position & 63. A child spawned this round is held there *only in LDS*:
+// its descriptor (header words and the first FW frame words) sits in desc[q][slot] and
+// nothing is written to HBM. After the round the wave keeps the newest <= 64 entries of
+// one queue for its next round and reads their descriptors back from LDS; only the
+// entries it publishes, the ones pushed out of the 64-slot window by later appends, and
+// the tasks that suspend at a taskwait are written to their HBM records. Most tasks of a
+// fork-join tree never touch HBM for their spawn and start.
+//   valid[q] bit s: slot s holds a private entry not yet published or taken;
+//   inlds[q] bit s: that entry's record exists only as desc[q][s].
+// The wave queue (run on a whole wave) keeps its records in HBM: QL = queues with
+// descriptors. A descriptor is { word0, parent, result offset, frame[0..FW) } padded to an
+// odd word count (LDS bank spread).
+template <int Q, int QL, int D>
 struct WaveCtx {
     uint32_t free_top;
     uint32_t priv_top;
@@ -53,9 +78,25 @@ struct WaveCtx {
     uint32_t tail[Q];       // owner's private tail per queue (absolute position)
     uint32_t bottom[Q];     // published bottom per queue (low word of the deque word)
     uint32_t top_cache[Q];  // last top observed per queue
+    uint64_t valid[Q];
+    uint64_t inlds[Q];
     uint32_t free_ids[kFreeCap];
-    uint32_t mirror[Q][kWave];  // the last 64 ids appended per queue, by position & 63
+    uint32_t mirror[Q][kWave];  // ids of the last 64 entries appended per queue, by position & 63
+    uint32_t desc[QL][kWave][D];
 };
+
+template <class P>
+struct CtxShape {
+    static constexpr int kQ = P::kQueues;
+    static constexpr int kWQ = HasWaveQueue<P>::value;
+    static_assert(kWQ < 0 || kWQ == kQ - 1, "the wave queue must be the last EPAQ queue");
+    static constexpr int kQL = kWQ >= 0 ? kQ - 1 : kQ;  // queues whose entries may live in LDS
+    static constexpr int kFW = P::kFrameWords;
+    static constexpr int kD = (3 + kFW) | 1;
+    using type = WaveCtx<kQ, (kQL > 0 ? kQL : 1), kD>;
+};
+template <class P>
+using WaveCtxOf = typename CtxShape<P>::type;
 
 __device__ __forceinline__ uint8_t* rec_ptr(const GtapDev& dev, uint32_t id) {
     return dev.recs + (size_t)id * dev.stride;
@@ -127,8 +168,8 @@ __device__ __forceinline__ bool ring_put_claim(const GtapDev& dev, uint32_t s, u
 }
 
 // Move 64 ids into the wave's LDS stack. Called by all active lanes.
-template <int Q>
-__device__ __forceinline__ void refill_chunk(const GtapDev& dev, WaveCtx<Q>* w, uint32_t me) {
+template <class W>
+__device__ __forceinline__ void refill_chunk(const GtapDev& dev, W* w, uint32_t me) {
     const uint64_t m = active_mask();
     const uint32_t cnt = (uint32_t)popc(m);
     const uint32_t r = (uint32_t)rank_in(m);
@@ -174,8 +215,8 @@ __device__ __forceinline__ void refill_chunk(const GtapDev& dev, WaveCtx<Q>* w, 
 }
 
 // Move the 64 most recently freed ids out of the wave's LDS stack.
-template <int Q>
-__device__ __forceinline__ void spill_chunk(const GtapDev& dev, WaveCtx<Q>* w, uint32_t me) {
+template <class W>
+__device__ __forceinline__ void spill_chunk(const GtapDev& dev, W* w, uint32_t me) {
     const uint64_t m = active_mask();
     const uint32_t cnt = (uint32_t)popc(m);
     const uint32_t r = (uint32_t)rank_in(m);
@@ -205,8 +246,8 @@ __device__ __forceinline__ void spill_chunk(const GtapDev& dev, WaveCtx<Q>* w, u
     vst(&w->free_top, top - kChunk);
 }
 
-template <int Q>
-__device__ __forceinline__ uint32_t alloc_id(const GtapDev& dev, WaveCtx<Q>* w, uint32_t me) {
+template <class W>
+__device__ __forceinline__ uint32_t alloc_id(const GtapDev& dev, W* w, uint32_t me) {
     const uint64_t m = active_mask();
     const uint32_t cnt = (uint32_t)popc(m);
     const uint32_t r = (uint32_t)rank_in(m);
@@ -222,8 +263,8 @@ __device__ __forceinline__ uint32_t alloc_id(const GtapDev& dev, WaveCtx<Q>* w, 
     return id;
 }
 
-template <int Q>
-__device__ __forceinline__ void free_id(const GtapDev& dev, WaveCtx<Q>* w, uint32_t me, uint32_t id) {
+template <class W>
+__device__ __forceinline__ void free_id(const GtapDev& dev, W* w, uint32_t me, uint32_t id) {
     const uint64_t m = active_mask();
     const uint32_t cnt = (uint32_t)popc(m);
     const uint32_t r = (uint32_t)rank_in(m);
@@ -240,10 +281,36 @@ __device__ __forceinline__ void free_id(const GtapDev& dev, WaveCtx<Q>* w, uint3
 
 // ------------------------------------------------------------------ deque append
 
-// Append `id` to the private (unpublished) end of queue `q` of this wave.
-// Called by all active lanes; lanes may target different queues.
-template <int Q>
-__device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtx<Q>* w, uint32_t me, uint32_t q, uint32_t id) {
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, uint32_t r) {
+    r &= 63u;
+    return r ? ((x << r) | (x >> (64u - r))) : x;
+}
+__device__ __forceinline__ uint64_t low_bits(uint32_t n) { return n >= 64u ? ~0ull : ((1ull << n) - 1ull); }
+
+// Writes the HBM record of an entry held in LDS: the 16-byte header (join counter 0) and the
+// first FW frame words.
+template <class P>
+__device__ __forceinline__ void record_from_desc(const GtapDev& dev, __amdgpu_buffer_rsrc_t rs, uint32_t id,
+                                                 const uint32_t* d) {
+    constexpr int FW = CtxShape<P>::kFW;
+    const uint32_t off = id * dev.stride;
+    uint32_t f[FW];
+#pragma unroll
+    for (int i = 0; i < FW; ++i) f[i] = vld(&d[3 + i]);
+    st128(rs, off, u32x4{vld(&d[0]), vld(&d[1]), 0u, vld(&d[2])});
+    frame_words_st<FW>(rs, off, f);
+}
+
+// Append `id` to the private (unpublished) end of queue `q` of this wave. Called by all
+// active lanes; lanes may target different queues. kDesc: the entry's record is held in
+// LDS (the D descriptor words `dw`), else it is already in HBM. An older private entry that
+// the append pushes out of the 64-slot window is written out: its record if it was held in
+// LDS, and its deque slot.
+template <class P, bool kDesc>
+__device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t q,
+                                             uint32_t id, const uint32_t* dw) {
+    constexpr int Q = CtxShape<P>::kQ, QL = CtxShape<P>::kQL, D = CtxShape<P>::kD;
+    const __amdgpu_buffer_rsrc_t rs = pool_rsrc(dev.recs);
 #pragma unroll
     for (int qq = 0; qq < Q; ++qq) {
         const uint64_t mq = ballot(q == (uint32_t)qq);
@@ -263,9 +330,28 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtx<Q>* w, 
                 if (r == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, id, qq, t + cnt - top, dev.cap - kMargin);
             } else {
                 const uint32_t pos = t + r;
-                st32(dq_slot_ptr(dev, qq, me, pos), id);
-                vst(&w->mirror[qq][pos & (kWave - 1)], id);
-                vst(&w->tail[qq], t + cnt);
+                const uint32_t slot = pos & (kWave - 1);
+                const uint64_t vmask = uniform(vld(&w->valid[qq]));
+                const uint64_t lmask = uniform(vld(&w->inlds[qq]));
+                if ((vmask >> slot) & 1ull) {
+                    // the slot still holds the private entry at pos - 64
+                    const uint32_t oid = vld(&w->mirror[qq][slot]);
+                    if (qq < QL && ((lmask >> slot) & 1ull)) record_from_desc<P>(dev, rs, oid, &w->desc[qq < QL ? qq : 0][slot][0]);
+                    st32(dq_slot_ptr(dev, qq, me, pos - kWave), oid);
+                }
+                vst(&w->mirror[qq][slot], id);
+                if constexpr (kDesc) {
+                    if (qq < QL) {
+#pragma unroll
+                        for (int i = 0; i < D; ++i) vst(&w->desc[qq < QL ? qq : 0][slot][i], dw[i]);
+                    }
+                }
+                const uint64_t range = rotl64(low_bits(cnt), t);
+                if (r == 0) {
+                    vst(&w->valid[qq], vmask | range);
+                    vst(&w->inlds[qq], (kDesc && qq < QL) ? (lmask | range) : (lmask & ~range));
+                    vst(&w->tail[qq], t + cnt);
+                }
             }
         }
     }
@@ -273,27 +359,39 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtx<Q>* w, 
 
 // ------------------------------------------------------------------ spawn
 
-// Allocate a task id, write the child's header and arguments, append it to the private end
-// of queue q. Called by every lane that spawns (wave-aggregated inside).
+// Allocate a task id and append the child to the private end of queue q: held in LDS (a
+// descriptor) on a per-lane queue, written to its HBM record on the wave queue. Called by
+// every lane that spawns (wave-aggregated inside).
 template <class P, typename A>
-__device__ __forceinline__ uint32_t spawn_record(const GtapDev& dev, WaveCtx<P::kQueues>* w, uint32_t me,
+__device__ __forceinline__ uint32_t spawn_record(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me,
                                                  uint32_t self, uint32_t f, uint32_t q, uint32_t par, uint32_t roff,
                                                  const A& a) {
     static_assert(sizeof(A) % 8 == 0, "argument blocks are padded to 8 bytes");
     static_assert(sizeof(A) / 4 <= P::kFrameWords, "arguments must fit the loaded frame");
+    constexpr int D = CtxShape<P>::kD, WQ = CtxShape<P>::kWQ, FW = CtxShape<P>::kFW;
     if (q >= (uint32_t)P::kQueues) {
         GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_QUEUE_IDX, self, q, q, P::kQueues);
         q = 0;
     }
     const uint32_t id = alloc_id(dev, w, me);
-    uint8_t* crec = rec_ptr(dev, id);
-    st64(reinterpret_cast<uint64_t*>(crec), (uint64_t)hdr_word0(f, 0, q) | ((uint64_t)par << 32));
-    st64(reinterpret_cast<uint64_t*>(crec + 8), ((uint64_t)roff << 32));
-    const uint64_t* src = reinterpret_cast<const uint64_t*>(&a);
-    uint64_t* dst = reinterpret_cast<uint64_t*>(crec + kHdrBytes);
+    uint32_t d[D];
+    d[0] = hdr_word0(f, 0, q);
+    d[1] = par;
+    d[2] = roff;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&a);
 #pragma unroll
-    for (int i = 0; i < (int)(sizeof(A) / 8); ++i) st64(dst + i, src[i]);
-    queue_append(dev, w, me, q, id);
+    for (int i = 0; i < FW; ++i) d[3 + i] = (i < (int)(sizeof(A) / 4)) ? src[i < (int)(sizeof(A) / 4) ? i : 0] : 0u;
+#pragma unroll
+    for (int i = 3 + FW; i < D; ++i) d[i] = 0u;
+    if (WQ >= 0 && q == (uint32_t)WQ) {
+        const __amdgpu_buffer_rsrc_t rs = pool_rsrc(dev.recs);
+        const uint32_t off = id * dev.stride;
+        st128(rs, off, u32x4{d[0], d[1], 0u, d[2]});
+        frame_words_st<FW>(rs, off, &d[3]);
+        queue_append<P, false>(dev, w, me, q, id, nullptr);
+    } else {
+        queue_append<P, true>(dev, w, me, q, id, d);
+    }
     const uint64_t am = active_mask();
     if (rank_in(am) == 0) vst(&w->spawns, vld(&w->spawns) + (uint32_t)popc(am));
     return id;
@@ -311,9 +409,11 @@ __device__ __forceinline__ void wave_sync() {
 template <class P>
 struct Lane {
     static constexpr int Q = P::kQueues;
+    static constexpr int FW = P::kFrameWords;
     const GtapDev* dev;
-    WaveCtx<Q>* w;
+    WaveCtxOf<P>* w;
     const typename P::Args* args;
+    __amdgpu_buffer_rsrc_t rs;  // the record pool
     uint8_t* rec;           // this task's record
     uint32_t me;            // worker (wave) index
     uint32_t tid;
@@ -324,7 +424,8 @@ struct Lane {
     uint32_t res_off;
     uint32_t child_count;
     uint32_t outcome;
-    uint32_t frame[P::kFrameWords];  // the first frame words, loaded when the round starts
+    bool lds;               // the record's header and first FW frame words are not in HBM yet
+    uint32_t frame[FW];     // the first frame words, loaded when the round starts
 
     // -- reads of the loaded frame (arguments written by the spawner)
     template <typename T>
@@ -337,11 +438,26 @@ struct Lane {
             return __builtin_bit_cast(T, v);
         }
     }
+    // A task started from LDS writes its first FW frame words to HBM before its frame is
+    // accessed there (the header follows at taskwait).
+    __device__ __forceinline__ void materialize_frame() {
+        frame_words_st<FW>(rs, tid * dev->stride, frame);
+        lds = false;
+    }
     // -- coherent access to any part of this task's frame (offsets relative to the frame)
     template <typename T>
-    __device__ __forceinline__ T frame_ld(int byte_off) const { return ldT<T>(rec + kHdrBytes + byte_off); }
+    __device__ __forceinline__ T frame_ld(int byte_off) {
+        if (lds) {
+            materialize_frame();
+            complete_stores();
+        }
+        return ldT<T>(rec + kHdrBytes + byte_off);
+    }
     template <typename T>
-    __device__ __forceinline__ void frame_st(int byte_off, T v) { stT<T>(rec + kHdrBytes + byte_off, v); }
+    __device__ __forceinline__ void frame_st(int byte_off, T v) {
+        if (lds) materialize_frame();
+        stT<T>(rec + kHdrBytes + byte_off, v);
+    }
 
     __device__ __forceinline__ uint32_t get_state() const { return state; }
 
@@ -365,7 +481,8 @@ struct Lane {
     }
 
     // taskwait: returns true when the task must return now (it has children to wait for);
-    // the task resumes at `next_state`, re-enqueued on queue `q_after`.
+    // the task resumes at `next_state`, re-enqueued on queue `q_after`. The whole 16-byte
+    // header (resume state, parent, join counter, result offset) is written in one store.
     // (reference: __gtap_set_state_for_join, gtap_runtime_thread.cuh:672-696)
     __device__ __forceinline__ bool taskwait(uint32_t next_state, uint32_t q_after) {
         if (q_after >= (uint32_t)Q) {
@@ -376,8 +493,8 @@ struct Lane {
             state = next_state;
             return false;
         }
-        st32(reinterpret_cast<uint32_t*>(rec), hdr_word0(func, next_state, q_after));
-        st32(reinterpret_cast<uint32_t*>(rec + 8), child_count);
+        if (lds) materialize_frame();
+        st128(rs, tid * dev->stride, u32x4{hdr_word0(func, next_state, q_after), parent, child_count, res_off});
         child_count = 0;
         outcome = OUT_SUSPENDED;
         return true;
@@ -408,7 +525,7 @@ template <class P>
 struct WaveTask {
     static constexpr int Q = P::kQueues;
     const GtapDev* dev;
-    WaveCtx<Q>* w;
+    WaveCtxOf<P>* w;
     const typename P::Args* args;
     typename P::WaveShared* sh;
     uint8_t* rec;
@@ -447,10 +564,9 @@ struct WaveTask {
             state = next_state;
             return false;
         }
-        if (lane == 0) {
-            st32(reinterpret_cast<uint32_t*>(rec), hdr_word0(func, next_state, q_after));
-            st32(reinterpret_cast<uint32_t*>(rec + 8), child_count);
-        }
+        if (lane == 0)
+            st128(pool_rsrc(dev->recs), tid * dev->stride,
+                  u32x4{hdr_word0(func, next_state, q_after), parent, child_count, res_off});
         child_count = 0;
         outcome = OUT_SUSPENDED;
         return true;
@@ -466,19 +582,10 @@ struct WaveTask {
     }
 };
 
-template <class P, class = void>
-struct HasWaveQueue {
-    static constexpr int value = -1;
-};
-template <class P>
-struct HasWaveQueue<P, decltype((void)P::kWaveQueue)> {
-    static constexpr int value = P::kWaveQueue;
-};
-
 // Runs the wave task `tid` on all lanes; returns its outcome (uniform).
 template <class P>
 __device__ __forceinline__ uint32_t run_wave_task(const GtapDev& dev, const typename P::Args& args,
-                                                  WaveCtx<P::kQueues>* w, void* wave_shared, uint32_t me,
+                                                  WaveCtxOf<P>* w, void* wave_shared, uint32_t me,
                                                   uint32_t tid, uint32_t& parent_out) {
     if constexpr (HasWaveQueue<P>::value >= 0) {
         WaveTask<P> W;
@@ -536,29 +643,41 @@ __device__ __forceinline__ void load_task(Lane<P>& L, uint32_t id) {
     const GtapDev& dev = *L.dev;
     L.tid = id;
     L.rec = rec_ptr(dev, id);
-    const uint64_t h0 = ld64(reinterpret_cast<const uint64_t*>(L.rec));
-    const uint64_t h1 = ld64(reinterpret_cast<const uint64_t*>(L.rec + 8));
-    const uint32_t w0 = (uint32_t)h0;
+    const uint32_t off = id * dev.stride;
+    const u32x4 h = ld128(L.rs, off);
+    L.func = hdr_func(h.x);
+    L.state = hdr_state(h.x);
+    L.queue = hdr_queue(h.x);
+    L.parent = h.y;
+    L.res_off = h.w;
+    L.child_count = 0;
+    L.outcome = OUT_RUNNING;
+    L.lds = false;
+    frame_words_ld<P::kFrameWords>(L.rs, off, L.frame);
+}
+
+// A task held in LDS: its descriptor { word0, parent, result offset, frame words }.
+template <class P>
+__device__ __forceinline__ void load_task_lds(Lane<P>& L, uint32_t id, const uint32_t* d) {
+    L.tid = id;
+    L.rec = rec_ptr(*L.dev, id);
+    const uint32_t w0 = vld(&d[0]);
     L.func = hdr_func(w0);
     L.state = hdr_state(w0);
     L.queue = hdr_queue(w0);
-    L.parent = (uint32_t)(h0 >> 32);
-    L.res_off = (uint32_t)(h1 >> 32);
+    L.parent = vld(&d[1]);
+    L.res_off = vld(&d[2]);
     L.child_count = 0;
     L.outcome = OUT_RUNNING;
-    const uint64_t* fp = reinterpret_cast<const uint64_t*>(L.rec + kHdrBytes);
+    L.lds = true;
 #pragma unroll
-    for (int i = 0; i < P::kFrameWords / 2; ++i) {
-        const uint64_t v = ld64(fp + i);
-        L.frame[2 * i] = (uint32_t)v;
-        L.frame[2 * i + 1] = (uint32_t)(v >> 32);
-    }
+    for (int i = 0; i < P::kFrameWords; ++i) L.frame[i] = vld(&d[3 + i]);
 }
 
 // Admit up to 64 jobs from the job table as root tasks. Returns the number admitted; lane
 // i < count receives the root id in `tid_out`.
 template <class P>
-__device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtx<P::kQueues>* w, uint32_t me,
+__device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me,
                                               uint32_t& tid_out, bool& idle_registered) {
     const int lane = lane_id();
     uint32_t avail = dev.njobs;
@@ -692,7 +811,7 @@ __device__ __forceinline__ uint32_t try_steal(const GtapDev& dev, uint32_t me, u
 
 // Pop up to 64 of this wave's newest published tasks from queue q.
 template <class P>
-__device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtx<P::kQueues>* w, uint32_t me, uint32_t q,
+__device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t q,
                                             uint32_t& tid_out) {
     const int lane = lane_id();
     uint64_t* wp = dq_word_ptr(dev, q, me);
@@ -735,7 +854,7 @@ __device__ __forceinline__ void backoff_sleep(uint32_t level) {
 }
 
 template <class P>
-__device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& args, WaveCtx<P::kQueues>* w,
+__device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& args, WaveCtxOf<P>* w,
                                  void* wave_shared) {
     constexpr int Q = P::kQueues;
     constexpr int WQ = HasWaveQueue<P>::value;
@@ -754,6 +873,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             vst(&w->tail[q], 0u);
             vst(&w->bottom[q], 0u);
             vst(&w->top_cache[q], 0u);
+            vst(&w->valid[q], (uint64_t)0);
+            vst(&w->inlds[q], (uint64_t)0);
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -763,11 +884,15 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     L.w = w;
     L.args = &args;
     L.me = me;
+    L.rs = pool_rsrc(dev.recs);
+    constexpr int QL = CtxShape<P>::kQL;
 
     uint32_t rng = 0x9E3779B9u ^ (me * 0x85EBCA6Bu) ^ ((uint32_t)lane * 0xC2B2AE35u) ^ (uint32_t)t_begin;
     if (rng == 0) rng = 1;
     uint32_t n = 0;        // tasks in the current batch (wave-uniform)
     uint32_t my_tid = 0;   // lane's task when lane < n
+    bool my_lds = false;   // the lane's task is held in LDS: desc[batch_q][my_slot]
+    uint32_t my_slot = 0;
     int batch_q = -1;      // queue the batch came from (-1: admitted job roots)
     bool idle_registered = false;
     uint32_t idle_iter = 0;
@@ -779,7 +904,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
 
     while (true) {
         if (n == 0) {
-            // ---- acquire work: own deque, then the job table, then steal
+            // ---- acquire work: own deque, then the job table, then steal (records in HBM)
+            my_lds = false;
             const uint32_t cq = uniform(vld(&w->cur_queue));
 #pragma unroll
             for (int i = 0; i < Q && n == 0; ++i) {
@@ -880,7 +1006,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, my_tid, -1, my_tid, dev.nrecs);
                 my_tid = 0u;
             }
-            load_task<P>(L, my_tid);
+            if (my_lds && batch_q >= 0 && batch_q < QL) load_task_lds<P>(L, my_tid, &w->desc[batch_q < QL ? batch_q : 0][my_slot][0]);
+            else load_task<P>(L, my_tid);
             if (my_tid == 0u) {
                 L.outcome = OUT_FINISHED;
                 L.parent = kNoParent;
@@ -915,7 +1042,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             if (ready != kNoParent && ready >= dev.nrecs) complete_job<P>(dev, ready);
             if (ready != kNoParent && ready < dev.nrecs) {
                 const uint32_t w0 = ld32(reinterpret_cast<const uint32_t*>(rec_ptr(dev, ready)));
-                queue_append(dev, w, me, hdr_queue(w0) < (uint32_t)Q ? hdr_queue(w0) : 0u, ready);
+                queue_append<P, false>(dev, w, me, hdr_queue(w0) < (uint32_t)Q ? hdr_queue(w0) : 0u, ready, nullptr);
             }
         }
         if (fin && fin_tid != 0u) free_id(dev, w, me, fin_tid);  // id 0 is the sink record
@@ -932,32 +1059,68 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         }
         const uint32_t keep_cap = ((int)kmax == WQ) ? 1u : (uint32_t)kWave;
         const uint32_t keep = cmax < keep_cap ? cmax : keep_cap;
-        uint32_t next_tid = 0;
+        uint32_t next_tid = 0, next_slot = 0;
+        bool next_lds = false;
         if (keep) {
             const uint32_t t = uniform(vld(&w->tail[kmax]));
-            if ((uint32_t)lane < keep) next_tid = vld(&w->mirror[kmax][(t - keep + (uint32_t)lane) & (kWave - 1)]);
-            vst(&w->tail[kmax], t - keep);
-            vst(&w->cur_queue, kmax);
+            const uint64_t vmask = uniform(vld(&w->valid[kmax]));
+            const uint64_t lmask = uniform(vld(&w->inlds[kmax]));
+            if ((uint32_t)lane < keep) {
+                next_slot = (t - keep + (uint32_t)lane) & (kWave - 1);
+                next_tid = vld(&w->mirror[kmax][next_slot]);
+                next_lds = (lmask >> next_slot) & 1ull;
+            }
+            const uint64_t taken = rotl64(low_bits(keep), t - keep);
+            if (lane == 0) {
+                vst(&w->valid[kmax], vmask & ~taken);
+                vst(&w->inlds[kmax], lmask & ~taken);
+                vst(&w->tail[kmax], t - keep);
+                vst(&w->cur_queue, kmax);
+            }
         }
-        bool waited = false;
+        // write out what is published: entries still in the 64-slot window (lane i takes
+        // slot i) get their deque slot, and their record if it is held in LDS
+        const __amdgpu_buffer_rsrc_t rs = L.rs;
+        bool any_publish = false;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const uint32_t t = uniform(vld(&w->tail[q]));
             const uint32_t b = uniform(vld(&w->bottom[q]));
             if (t != b) {
-                if (!waited) {
-                    complete_stores();  // slots of continuations appended above
-                    waited = true;
+                any_publish = true;
+                const uint64_t vmask = uniform(vld(&w->valid[q]));
+                const uint64_t lmask = uniform(vld(&w->inlds[q]));
+                if ((vmask >> lane) & 1ull) {
+                    const uint32_t pos = t - kWave + (((uint32_t)lane - t) & (kWave - 1));
+                    const uint32_t oid = vld(&w->mirror[q][lane]);
+                    if (q < QL && ((lmask >> lane) & 1ull)) record_from_desc<P>(dev, rs, oid, &w->desc[q < QL ? q : 0][lane][0]);
+                    st32(dq_slot_ptr(dev, q, me, pos), oid);
                 }
-                uint64_t old = 0;
-                if (lane == 0) old = atomic_add(dq_word_ptr(dev, q, me), (uint64_t)(t - b));
-                old = shfl(old, 0);
-                vst(&w->bottom[q], t);
-                vst(&w->top_cache[q], (uint32_t)(old >> 32));
+                if (lane == 0) {
+                    vst(&w->valid[q], (uint64_t)0);
+                    vst(&w->inlds[q], (uint64_t)0);
+                }
+            }
+        }
+        if (any_publish) {
+            complete_stores();  // records and slots of everything published below
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const uint32_t t = uniform(vld(&w->tail[q]));
+                const uint32_t b = uniform(vld(&w->bottom[q]));
+                if (t != b) {
+                    uint64_t old = 0;
+                    if (lane == 0) old = atomic_add(dq_word_ptr(dev, q, me), (uint64_t)(t - b));
+                    old = shfl(old, 0);
+                    vst(&w->bottom[q], t);
+                    vst(&w->top_cache[q], (uint32_t)(old >> 32));
+                }
             }
         }
         n = keep;
         my_tid = next_tid;
+        my_slot = next_slot;
+        my_lds = next_lds;
         batch_q = (int)kmax;
         if (dev.prof_max && lane == 0) prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, (int32_t)round_n);
 
@@ -1004,7 +1167,7 @@ struct WaveSharedOf<P, decltype((void)sizeof(typename P::WaveShared))> {
 
 template <class P>
 __global__ __launch_bounds__(256) void thread_kernel(GtapDev dev, typename P::Args args) {
-    __shared__ WaveCtx<P::kQueues> ctxs[4];
+    __shared__ WaveCtxOf<P> ctxs[4];
     __shared__ typename WaveSharedOf<P>::type wsh[4];
     thread_scheduler<P>(dev, args, &ctxs[threadIdx.x / kWave], &wsh[threadIdx.x / kWave]);
 }

@@ -290,6 +290,8 @@ int gtap_initialize(void) {
     n = (n / 64) * 64;
     if (n < 4096) n = 4096;
     if (n + R.nanchors > 0x7FFFFFFFull) n = ((0x7FFFFFFFull - R.nanchors) / 64) * 64;
+    // task records are addressed by 32-bit byte offsets (16-byte buffer accesses)
+    if (n * R.stride > 0xFFFFFFFFull) n = ((0xFFFFFFFFull / R.stride) / 64) * 64;
     R.nrecs = (uint32_t)n;
     R.recs_bytes = (uint64_t)(R.nrecs + R.nanchors) * R.stride;
     GTAP_HIP_TRY(hipMalloc(&R.recs, R.recs_bytes));
