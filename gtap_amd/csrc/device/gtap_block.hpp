@@ -179,8 +179,8 @@ struct BlockTask {
             id = c->free_ids[avail - 1 - k];
         }
         uint8_t* crec = rec_ptr(*dev, id);
-        st64(reinterpret_cast<uint64_t*>(crec), (uint64_t)hdr_word0(f, 0, 0) | ((uint64_t)par << 32));
-        st64(reinterpret_cast<uint64_t*>(crec + 8), ((uint64_t)roff << 32));
+        st64(reinterpret_cast<uint64_t*>(crec), hdr_lo(par, roff));
+        st64(reinterpret_cast<uint64_t*>(crec + kHdrJoin), hdr_hi(0u, hdr_word0(f, 0, 0)));
         const uint64_t* src = reinterpret_cast<const uint64_t*>(&a);
         uint64_t* dst = reinterpret_cast<uint64_t*>(crec + kHdrBytes);
 #pragma unroll
@@ -209,8 +209,7 @@ struct BlockTask {
             return false;
         }
         if (threadIdx.x == 0) {
-            st32(reinterpret_cast<uint32_t*>(rec), hdr_word0(func, next_state, 0));
-            st32(reinterpret_cast<uint32_t*>(rec + 8), cc);
+            st64(reinterpret_cast<uint64_t*>(rec + kHdrJoin), hdr_hi(cc, hdr_word0(func, next_state, 0)));
             complete_stores();
             c->outcome = OUT_SUSPENDED;
         }
@@ -307,12 +306,12 @@ __device__ __forceinline__ uint32_t block_admit(const GtapDev& dev, BlockCtx* c,
     c->free_top -= 1;  // free_top >= kBlockMaxSpawn here (refilled before acquisition)
     const uint32_t id = c->free_ids[c->free_top];
     uint8_t* arec = rec_ptr(dev, anchor);
-    st64(reinterpret_cast<uint64_t*>(arec), (uint64_t)hdr_word0(kAnchorFunc, 0, 0) | ((uint64_t)kNoParent << 32));
-    st64(reinterpret_cast<uint64_t*>(arec + 8), 1ull);
+    st64(reinterpret_cast<uint64_t*>(arec), hdr_lo(kNoParent, 0u));
+    st64(reinterpret_cast<uint64_t*>(arec + kHdrJoin), hdr_hi(1u, hdr_word0(kAnchorFunc, 0, 0)));
     st64(reinterpret_cast<uint64_t*>(arec + kHdrBytes), 0ull);
     uint8_t* rrec = rec_ptr(dev, id);
-    st64(reinterpret_cast<uint64_t*>(rrec), (uint64_t)hdr_word0(func, 0, 0) | ((uint64_t)anchor << 32));
-    st64(reinterpret_cast<uint64_t*>(rrec + 8), ((uint64_t)kHdrBytes << 32));
+    st64(reinterpret_cast<uint64_t*>(rrec), hdr_lo(anchor, kHdrBytes));
+    st64(reinterpret_cast<uint64_t*>(rrec + kHdrJoin), hdr_hi(0u, hdr_word0(func, 0, 0)));
     const uint64_t* ja = reinterpret_cast<const uint64_t*>(job->args);
     uint64_t* ra = reinterpret_cast<uint64_t*>(rrec + kHdrBytes);
     constexpr int kArgWords = (P::kFrameWords / 2) < 8 ? (P::kFrameWords / 2) : 8;
@@ -409,12 +408,12 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
         if (t == 0) {
             uint8_t* r = rec_ptr(dev, c->cur);
             const uint64_t h0 = ld64(reinterpret_cast<const uint64_t*>(r));
-            const uint64_t h1 = ld64(reinterpret_cast<const uint64_t*>(r + 8));
+            const uint64_t h1 = ld64(reinterpret_cast<const uint64_t*>(r + kHdrJoin));
             c->tid = c->cur;
-            c->func = hdr_func((uint32_t)h0);
-            c->state = hdr_state((uint32_t)h0);
-            c->parent = (uint32_t)(h0 >> 32);
-            c->res_off = (uint32_t)(h1 >> 32);
+            c->func = hdr_func((uint32_t)(h1 >> 32));
+            c->state = hdr_state((uint32_t)(h1 >> 32));
+            c->parent = (uint32_t)h0;
+            c->res_off = (uint32_t)(h0 >> 32);
             c->outcome = OUT_RUNNING;
             c->child_count = 0;
             const uint64_t* fp = reinterpret_cast<const uint64_t*>(r + kHdrBytes);
@@ -460,7 +459,7 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
             if (c->outcome == OUT_FINISHED) {
                 const uint32_t par = c->parent;
                 if (par != kNoParent && par < dev.nrecs + dev.nanchors) {
-                    const uint32_t old = atomic_sub(reinterpret_cast<uint32_t*>(rec_ptr(dev, par) + 8), 1u);
+                    const uint32_t old = atomic_sub(reinterpret_cast<uint32_t*>(rec_ptr(dev, par) + kHdrJoin), 1u);
                     if (old == 1u) {
                         if (par >= dev.nrecs) complete_job<P>(dev, par);
                         else next = par;  // resume the parent right here

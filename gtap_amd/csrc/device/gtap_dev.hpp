@@ -100,11 +100,14 @@ __device__ __forceinline__ void prof_stamp(uint64_t* base, int32_t* counts, uint
 }
 
 // Record header (16 bytes) at the start of every task record and anchor.
-//   word0: func id [0..11] | state [12..23] | queue [24..27]
-//   word1: parent id (int32, -1 = detached / none)
+//   word0: parent id (-1 = detached / none)
+//   word1: byte offset of the result slot inside the parent record
 //   word2: join counter (children still running)
-//   word3: byte offset of the result slot inside the parent record
+//   word3: control word = func id [0..11] | state [12..23] | queue [24..27]
+// The join counter and the control word share one 8-byte word, so the child that takes the
+// counter to zero learns the parent's resume queue from the same atomic.
 constexpr uint32_t kHdrBytes = 16;
+constexpr uint32_t kHdrJoin = 8;  // byte offset of the {join, control} 8-byte word
 constexpr uint32_t kNoParent = 0xFFFFFFFFu;
 constexpr uint32_t kMaxFunc = 0xFFF;
 constexpr uint32_t kMaxState = 0xFFF;
@@ -113,6 +116,10 @@ __device__ __forceinline__ uint32_t hdr_word0(uint32_t func, uint32_t state, uin
     return (func & 0xFFFu) | ((state & 0xFFFu) << 12) | ((queue & 0xFu) << 24);
 }
 __device__ __forceinline__ uint32_t hdr_func(uint32_t w0) { return w0 & 0xFFFu; }
+__device__ __forceinline__ uint64_t hdr_lo(uint32_t parent, uint32_t res_off) {
+    return (uint64_t)parent | ((uint64_t)res_off << 32);
+}
+__device__ __forceinline__ uint64_t hdr_hi(uint32_t join, uint32_t ctl) { return (uint64_t)join | ((uint64_t)ctl << 32); }
 __device__ __forceinline__ uint32_t hdr_state(uint32_t w0) { return (w0 >> 12) & 0xFFFu; }
 __device__ __forceinline__ uint32_t hdr_queue(uint32_t w0) { return (w0 >> 24) & 0xFu; }
 
@@ -158,6 +165,11 @@ __device__ __forceinline__ void st64_sys(uint64_t* p, uint64_t v) {
 template <typename T>
 __device__ __forceinline__ T atomic_add(T* p, T v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Fetch-add whose result is not used: the wave does not wait for it to return.
+template <typename T>
+__device__ __forceinline__ void atomic_add_noret(T* p, T v) {
+    (void)__hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ T atomic_sub(T* p, T v) {

@@ -297,7 +297,7 @@ __device__ __forceinline__ void record_from_desc(const GtapDev& dev, __amdgpu_bu
     uint32_t f[FW];
 #pragma unroll
     for (int i = 0; i < FW; ++i) f[i] = vld(&d[3 + i]);
-    st128(rs, off, u32x4{vld(&d[0]), vld(&d[1]), 0u, vld(&d[2])});
+    st128(rs, off, u32x4{vld(&d[1]), vld(&d[2]), 0u, vld(&d[0])});
     frame_words_st<FW>(rs, off, f);
 }
 
@@ -386,7 +386,7 @@ __device__ __forceinline__ uint32_t spawn_record(const GtapDev& dev, WaveCtxOf<P
     if (WQ >= 0 && q == (uint32_t)WQ) {
         const __amdgpu_buffer_rsrc_t rs = pool_rsrc(dev.recs);
         const uint32_t off = id * dev.stride;
-        st128(rs, off, u32x4{d[0], d[1], 0u, d[2]});
+        st128(rs, off, u32x4{d[1], d[2], 0u, d[0]});
         frame_words_st<FW>(rs, off, &d[3]);
         queue_append<P, false>(dev, w, me, q, id, nullptr);
     } else {
@@ -453,10 +453,37 @@ struct Lane {
         }
         return ldT<T>(rec + kHdrBytes + byte_off);
     }
+    // A store to a field of the loaded frame words (constant offset) also updates the
+    // register copy, so the task reads it back with arg() when it resumes; an LDS-held task
+    // writes it to HBM with the rest of its frame words at taskwait.
     template <typename T>
     __device__ __forceinline__ void frame_st(int byte_off, T v) {
+        static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4/8-byte fields");
+        if (__builtin_constant_p(byte_off) && byte_off % 4 == 0 && byte_off + (int)sizeof(T) <= FW * 4) {
+            if constexpr (sizeof(T) == 4) {
+                frame[byte_off / 4] = __builtin_bit_cast(uint32_t, v);
+            } else {
+                const uint64_t u = __builtin_bit_cast(uint64_t, v);
+                frame[byte_off / 4] = (uint32_t)u;
+                frame[byte_off / 4 + 1] = (uint32_t)(u >> 32);
+            }
+            if (!lds) stT<T>(rec + kHdrBytes + byte_off, v);
+            return;
+        }
         if (lds) materialize_frame();
         stT<T>(rec + kHdrBytes + byte_off, v);
+    }
+    // Two adjacent 8-byte fields at a 16-byte aligned frame offset, with one 16-byte load.
+    template <typename T>
+    __device__ __forceinline__ void frame_ld2(int byte_off, T& a, T& b) {
+        static_assert(sizeof(T) == 8, "8-byte fields");
+        if (lds) {
+            materialize_frame();
+            complete_stores();
+        }
+        const u32x4 v = ld128(rs, tid * dev->stride + kHdrBytes + (uint32_t)byte_off);
+        a = __builtin_bit_cast(T, (uint64_t)v.x | ((uint64_t)v.y << 32));
+        b = __builtin_bit_cast(T, (uint64_t)v.z | ((uint64_t)v.w << 32));
     }
 
     __device__ __forceinline__ uint32_t get_state() const { return state; }
@@ -494,7 +521,7 @@ struct Lane {
             return false;
         }
         if (lds) materialize_frame();
-        st128(rs, tid * dev->stride, u32x4{hdr_word0(func, next_state, q_after), parent, child_count, res_off});
+        st128(rs, tid * dev->stride, u32x4{parent, res_off, child_count, hdr_word0(func, next_state, q_after)});
         child_count = 0;
         outcome = OUT_SUSPENDED;
         return true;
@@ -566,7 +593,7 @@ struct WaveTask {
         }
         if (lane == 0)
             st128(pool_rsrc(dev->recs), tid * dev->stride,
-                  u32x4{hdr_word0(func, next_state, q_after), parent, child_count, res_off});
+                  u32x4{parent, res_off, child_count, hdr_word0(func, next_state, q_after)});
         child_count = 0;
         outcome = OUT_SUSPENDED;
         return true;
@@ -597,13 +624,12 @@ __device__ __forceinline__ uint32_t run_wave_task(const GtapDev& dev, const type
         W.lane = lane_id();
         W.tid = tid;
         W.rec = rec_ptr(dev, tid);
-        const uint64_t h0 = ld64(reinterpret_cast<const uint64_t*>(W.rec));
-        const uint64_t h1 = ld64(reinterpret_cast<const uint64_t*>(W.rec + 8));
-        W.func = hdr_func((uint32_t)h0);
-        W.state = hdr_state((uint32_t)h0);
-        W.queue = hdr_queue((uint32_t)h0);
-        W.parent = (uint32_t)(h0 >> 32);
-        W.res_off = (uint32_t)(h1 >> 32);
+        const u32x4 h = ld128(pool_rsrc(dev.recs), tid * dev.stride);
+        W.func = hdr_func(h.w);
+        W.state = hdr_state(h.w);
+        W.queue = hdr_queue(h.w);
+        W.parent = h.x;
+        W.res_off = h.y;
         W.child_count = 0;
         W.outcome = OUT_RUNNING;
         const uint64_t* fp = reinterpret_cast<const uint64_t*>(W.rec + kHdrBytes);
@@ -645,11 +671,11 @@ __device__ __forceinline__ void load_task(Lane<P>& L, uint32_t id) {
     L.rec = rec_ptr(dev, id);
     const uint32_t off = id * dev.stride;
     const u32x4 h = ld128(L.rs, off);
-    L.func = hdr_func(h.x);
-    L.state = hdr_state(h.x);
-    L.queue = hdr_queue(h.x);
-    L.parent = h.y;
-    L.res_off = h.w;
+    L.func = hdr_func(h.w);
+    L.state = hdr_state(h.w);
+    L.queue = hdr_queue(h.w);
+    L.parent = h.x;
+    L.res_off = h.y;
     L.child_count = 0;
     L.outcome = OUT_RUNNING;
     L.lds = false;
@@ -728,13 +754,12 @@ __device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtxOf<P>*
         const uint32_t anchor = dev.nrecs + slot;
         const uint32_t id = alloc_id(dev, w, me);
         uint8_t* arec = rec_ptr(dev, anchor);
-        st64(reinterpret_cast<uint64_t*>(arec), (uint64_t)hdr_word0(kAnchorFunc, 0, 0) | ((uint64_t)kNoParent << 32));
-        st64(reinterpret_cast<uint64_t*>(arec + 8), 1ull);  // join = 1 (the root), res_off = 0
+        st64(reinterpret_cast<uint64_t*>(arec), hdr_lo(kNoParent, 0u));
+        st64(reinterpret_cast<uint64_t*>(arec + kHdrJoin), hdr_hi(1u, hdr_word0(kAnchorFunc, 0, 0)));  // join = 1 (the root)
         st64(reinterpret_cast<uint64_t*>(arec + kHdrBytes), 0ull);
         uint8_t* rrec = rec_ptr(dev, id);
-        st64(reinterpret_cast<uint64_t*>(rrec), (uint64_t)hdr_word0(func, 0, q < (uint32_t)P::kQueues ? q : 0) |
-                                                    ((uint64_t)anchor << 32));
-        st64(reinterpret_cast<uint64_t*>(rrec + 8), ((uint64_t)kHdrBytes << 32));
+        st64(reinterpret_cast<uint64_t*>(rrec), hdr_lo(anchor, kHdrBytes));
+        st64(reinterpret_cast<uint64_t*>(rrec + kHdrJoin), hdr_hi(0u, hdr_word0(func, 0, q < (uint32_t)P::kQueues ? q : 0)));
         const uint64_t* ja = reinterpret_cast<const uint64_t*>(job->args);
         uint64_t* ra = reinterpret_cast<uint64_t*>(rrec + kHdrBytes);
         constexpr int kArgWords = (P::kFrameWords / 2) < 8 ? (P::kFrameWords / 2) : 8;
@@ -1032,18 +1057,21 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
 
         // ---- epilogue: joins, continuations, id release, next batch, publish
         complete_stores();  // results, child headers/arguments and slots of this round
-        uint32_t ready = kNoParent;
+        uint32_t ready = kNoParent, ready_ctl = 0;
         if (fin && fin_parent != kNoParent) {
-            const uint32_t old = atomic_sub(reinterpret_cast<uint32_t*>(rec_ptr(dev, fin_parent) + 8), 1u);
-            if (old == 1u) ready = fin_parent;
-            else if (old == 0u) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, fin_tid, -1, 0, 0);
+            // {join, control} of the parent: the last child also reads the resume queue
+            const uint64_t old = atomic_sub(reinterpret_cast<uint64_t*>(rec_ptr(dev, fin_parent) + kHdrJoin), (uint64_t)1);
+            if ((uint32_t)old == 1u) {
+                ready = fin_parent;
+                ready_ctl = (uint32_t)(old >> 32);
+            } else if ((uint32_t)old == 0u) {
+                GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, fin_tid, -1, 0, 0);
+            }
         }
         if (ballot(ready != kNoParent)) {
             if (ready != kNoParent && ready >= dev.nrecs) complete_job<P>(dev, ready);
-            if (ready != kNoParent && ready < dev.nrecs) {
-                const uint32_t w0 = ld32(reinterpret_cast<const uint32_t*>(rec_ptr(dev, ready)));
-                queue_append<P, false>(dev, w, me, hdr_queue(w0) < (uint32_t)Q ? hdr_queue(w0) : 0u, ready, nullptr);
-            }
+            if (ready != kNoParent && ready < dev.nrecs)
+                queue_append<P, false>(dev, w, me, hdr_queue(ready_ctl) < (uint32_t)Q ? hdr_queue(ready_ctl) : 0u, ready, nullptr);
         }
         if (fin && fin_tid != 0u) free_id(dev, w, me, fin_tid);  // id 0 is the sink record
 
@@ -1109,11 +1137,9 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 const uint32_t t = uniform(vld(&w->tail[q]));
                 const uint32_t b = uniform(vld(&w->bottom[q]));
                 if (t != b) {
-                    uint64_t old = 0;
-                    if (lane == 0) old = atomic_add(dq_word_ptr(dev, q, me), (uint64_t)(t - b));
-                    old = shfl(old, 0);
+                    // top_cache stays a lower bound of the top; the capacity check re-reads it
+                    if (lane == 0) atomic_add_noret(dq_word_ptr(dev, q, me), (uint64_t)(t - b));
                     vst(&w->bottom[q], t);
-                    vst(&w->top_cache[q], (uint32_t)(old >> 32));
                 }
             }
         }

@@ -19,7 +19,10 @@
 //       return s;
 //   }
 //
-// Lowered frame: { int32 row; uint32 cols, diag, anti; int32 k; int32 pad; int64 counts[16]; }.
+// Lowered frame: { int32 row; uint32 cols, diag, anti; int32 k; int32 pad[3]; int64 counts[16]; }.
+// The first six words (the arguments and k) are the frame words loaded when the task starts,
+// so the resumed task knows k without another load; counts[] is 16-byte aligned and read
+// two at a time.
 #include "../device/gtap_abi.hpp"
 #include "../host/gtap_launch.hpp"
 
@@ -34,12 +37,12 @@ struct NqArg {
     uint32_t anti;
 };
 constexpr int kOffK = 16;
-constexpr int kOffCounts = 24;
+constexpr int kOffCounts = 32;
 constexpr int kMaxN = 16;
 
 struct Program {
     static constexpr int kQueues = 1;
-    static constexpr int kFrameWords = 4;  // row, cols, diag, anti
+    static constexpr int kFrameWords = 6;  // row, cols, diag, anti, k, pad
     static constexpr uint32_t kMaxFunc = F_NQ;
     static constexpr uint32_t kFrameBytes = kOffCounts + 8 * kMaxN;
     struct Args {
@@ -81,9 +84,13 @@ __device__ __forceinline__ void nq_task(L& lane) {
         }
             [[fallthrough]];
         case 1: {
-            const int32_t k = lane.template frame_ld<int32_t>(kOffK);
+            const int32_t k = lane.template arg<int32_t>(kOffK);
             int64_t s = 0;
-            for (int32_t i = 0; i < k; ++i) s += lane.template frame_ld<int64_t>(kOffCounts + 8 * i);
+            for (int32_t i = 0; i < k; i += 2) {
+                int64_t a, b;
+                lane.template frame_ld2<int64_t>(kOffCounts + 8 * i, a, b);
+                s += a + (i + 1 < k ? b : 0);
+            }
             __gtap_return(lane, s);
             return;
         }
