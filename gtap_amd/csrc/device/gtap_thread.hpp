@@ -68,7 +68,7 @@ struct HasWaveQueue<P, decltype((void)P::kWaveQueue)> {
 // The wave queue (run on a whole wave) keeps its records in HBM: QL = queues with
 // descriptors. A descriptor is { word0, parent, result offset, frame[0..FW) } padded to an
 // odd word count (LDS bank spread).
-template <int Q, int QL, int D>
+template <int Q, int QL, int D, int S>
 struct WaveCtx {
     uint32_t free_top;
     uint32_t priv_top;
@@ -83,6 +83,11 @@ struct WaveCtx {
     uint32_t free_ids[kFreeCap];
     uint32_t mirror[Q][kWave];  // ids of the last 64 entries appended per queue, by position & 63
     uint32_t desc[QL][kWave][D];
+    // records written out together after a round (suspended tasks, published entries): the
+    // 16-byte header and S/4 - 1 frame pieces each, spread over the lanes so that the pieces
+    // of one record leave in one store instruction and merge into one memory transaction
+    uint32_t stage[kWave][S];
+    uint32_t stage_id[kWave];
 };
 
 template <class P>
@@ -92,8 +97,12 @@ struct CtxShape {
     static_assert(kWQ < 0 || kWQ == kQ - 1, "the wave queue must be the last EPAQ queue");
     static constexpr int kQL = kWQ >= 0 ? kQ - 1 : kQ;  // queues whose entries may live in LDS
     static constexpr int kFW = P::kFrameWords;
+    static constexpr int kFWr = (kFW + 3) & ~3;     // frame words kept in registers (16-byte pieces)
+    static constexpr int kPieces = 1 + kFWr / 4;    // 16-byte pieces of header + frame words
+    static_assert(kFWr <= 12, "at most 12 frame words are loaded with a task (one 64-byte line)");
+    static_assert(P::kFrameBytes >= 4 * kFWr, "the frame must cover the loaded frame words");
     static constexpr int kD = (3 + kFW) | 1;
-    using type = WaveCtx<kQ, (kQL > 0 ? kQL : 1), kD>;
+    using type = WaveCtx<kQ, (kQL > 0 ? kQL : 1), kD, 4 * kPieces>;
 };
 template <class P>
 using WaveCtxOf = typename CtxShape<P>::type;
@@ -404,12 +413,35 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Writes the `count` records staged in w->stage / w->stage_id: piece k (16 bytes) of entry e
+// goes out from lane e * kPieces + k, so one store instruction carries 64 / kPieces whole
+// records and the pieces of a record merge into one memory transaction. An entry whose id
+// carries kHeaderOnly writes its header piece only. All lanes of the wave call it.
+constexpr uint32_t kHeaderOnly = 0x80000000u;
+template <class P, class W>
+__device__ __forceinline__ void write_staged(const GtapDev& dev, __amdgpu_buffer_rsrc_t rs, W* w, uint32_t count) {
+    constexpr uint32_t PC = CtxShape<P>::kPieces;
+    constexpr uint32_t E = kWave / PC;  // records per store instruction
+    const uint32_t lane = (uint32_t)lane_id();
+    const uint32_t k = lane % PC;
+    for (uint32_t base = 0; base < count; base += E) {
+        const uint32_t e = base + lane / PC;
+        if (lane < E * PC && e < count) {
+            const uint32_t v = w->stage_id[e];
+            if (k == 0 || !(v & kHeaderOnly)) {
+                const uint32_t* src = &w->stage[e][4 * k];
+                st128(rs, (v & ~kHeaderOnly) * dev.stride + 16u * k, u32x4{src[0], src[1], src[2], src[3]});
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ per-lane task context
 
 template <class P>
 struct Lane {
     static constexpr int Q = P::kQueues;
-    static constexpr int FW = P::kFrameWords;
+    static constexpr int FW = CtxShape<P>::kFWr;
     const GtapDev* dev;
     WaveCtxOf<P>* w;
     const typename P::Args* args;
@@ -423,6 +455,7 @@ struct Lane {
     uint32_t parent;
     uint32_t res_off;
     uint32_t child_count;
+    uint32_t join_count;    // children to wait for, after a taskwait that suspends
     uint32_t outcome;
     bool lds;               // the record's header and first FW frame words are not in HBM yet
     uint32_t frame[FW];     // the first frame words, loaded when the round starts
@@ -443,6 +476,10 @@ struct Lane {
     __device__ __forceinline__ void materialize_frame() {
         frame_words_st<FW>(rs, tid * dev->stride, frame);
         lds = false;
+    }
+    // The 16-byte header this task leaves in its record when it suspends.
+    __device__ __forceinline__ u32x4 suspended_header() const {
+        return u32x4{parent, res_off, join_count, hdr_word0(func, state, queue)};
     }
     // -- coherent access to any part of this task's frame (offsets relative to the frame)
     template <typename T>
@@ -508,8 +545,9 @@ struct Lane {
     }
 
     // taskwait: returns true when the task must return now (it has children to wait for);
-    // the task resumes at `next_state`, re-enqueued on queue `q_after`. The whole 16-byte
-    // header (resume state, parent, join counter, result offset) is written in one store.
+    // the task resumes at `next_state`, re-enqueued on queue `q_after`. The record (header,
+    // and the frame words of a task held in LDS) is written after the round, together with
+    // the other suspended tasks of the wave.
     // (reference: __gtap_set_state_for_join, gtap_runtime_thread.cuh:672-696)
     __device__ __forceinline__ bool taskwait(uint32_t next_state, uint32_t q_after) {
         if (q_after >= (uint32_t)Q) {
@@ -520,8 +558,9 @@ struct Lane {
             state = next_state;
             return false;
         }
-        if (lds) materialize_frame();
-        st128(rs, tid * dev->stride, u32x4{parent, res_off, child_count, hdr_word0(func, next_state, q_after)});
+        state = next_state;
+        queue = q_after;
+        join_count = child_count;
         child_count = 0;
         outcome = OUT_SUSPENDED;
         return true;
@@ -679,7 +718,7 @@ __device__ __forceinline__ void load_task(Lane<P>& L, uint32_t id) {
     L.child_count = 0;
     L.outcome = OUT_RUNNING;
     L.lds = false;
-    frame_words_ld<P::kFrameWords>(L.rs, off, L.frame);
+    frame_words_ld<CtxShape<P>::kFWr>(L.rs, off, L.frame);
 }
 
 // A task held in LDS: its descriptor { word0, parent, result offset, frame words }.
@@ -698,6 +737,8 @@ __device__ __forceinline__ void load_task_lds(Lane<P>& L, uint32_t id, const uin
     L.lds = true;
 #pragma unroll
     for (int i = 0; i < P::kFrameWords; ++i) L.frame[i] = vld(&d[3 + i]);
+#pragma unroll
+    for (int i = P::kFrameWords; i < CtxShape<P>::kFWr; ++i) L.frame[i] = 0u;
 }
 
 // Admit up to 64 jobs from the job table as root tasks. Returns the number admitted; lane
@@ -1012,7 +1053,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         }
         had_tasks = true;
         uint32_t fin_tid = 0, fin_parent = kNoParent;
-        bool fin = false;
+        bool fin = false, sus = false;
         if (WQ >= 0 && batch_q == WQ) {
             // one wave-cooperative task on all lanes
             uint32_t wt = uniform(my_tid);
@@ -1047,6 +1088,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 L.outcome = OUT_FINISHED;
             }
             fin = (L.outcome == OUT_FINISHED);
+            sus = (L.outcome == OUT_SUSPENDED);
             fin_tid = L.tid;
             fin_parent = L.parent;
             if (fin_parent != kNoParent && fin_parent >= dev.nrecs + dev.nanchors) {
@@ -1055,8 +1097,28 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             }
         }
 
-        // ---- epilogue: joins, continuations, id release, next batch, publish
-        complete_stores();  // results, child headers/arguments and slots of this round
+        // ---- epilogue: suspended records, joins, continuations, id release, next batch, publish
+        {
+            const uint64_t sm = ballot(sus);
+            if (sm) {
+                const uint32_t r = (uint32_t)rank_in(sm);
+                if (sus) {
+                    const u32x4 h = L.suspended_header();
+                    uint32_t* dst = &w->stage[r][0];
+                    dst[0] = h.x;
+                    dst[1] = h.y;
+                    dst[2] = h.z;
+                    dst[3] = h.w;
+#pragma unroll
+                    for (int i = 0; i < CtxShape<P>::kFWr; ++i) dst[4 + i] = L.frame[i];
+                    w->stage_id[r] = L.tid | (L.lds ? 0u : kHeaderOnly);  // frame already in HBM
+                }
+                wave_sync();
+                write_staged<P>(dev, L.rs, w, (uint32_t)popc(sm));
+                wave_sync();
+            }
+        }
+        complete_stores();  // results, suspended records and child records of this round
         uint32_t ready = kNoParent, ready_ctl = 0;
         if (fin && fin_parent != kNoParent) {
             // {join, control} of the parent: the last child also reads the resume queue
@@ -1118,11 +1180,34 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 any_publish = true;
                 const uint64_t vmask = uniform(vld(&w->valid[q]));
                 const uint64_t lmask = uniform(vld(&w->inlds[q]));
-                if ((vmask >> lane) & 1ull) {
+                const bool mine = (vmask >> lane) & 1ull;
+                uint32_t oid = 0;
+                if (mine) {
                     const uint32_t pos = t - kWave + (((uint32_t)lane - t) & (kWave - 1));
-                    const uint32_t oid = vld(&w->mirror[q][lane]);
-                    if (q < QL && ((lmask >> lane) & 1ull)) record_from_desc<P>(dev, rs, oid, &w->desc[q < QL ? q : 0][lane][0]);
+                    oid = vld(&w->mirror[q][lane]);
                     st32(dq_slot_ptr(dev, q, me, pos), oid);
+                }
+                if (q < QL) {
+                    // records held in LDS: staged in record order, written together
+                    const bool stg = mine && ((lmask >> lane) & 1ull);
+                    const uint64_t sm = ballot(stg);
+                    if (sm) {
+                        if (stg) {
+                            const uint32_t r = (uint32_t)rank_in(sm);
+                            const uint32_t* d = &w->desc[q < QL ? q : 0][lane][0];
+                            uint32_t* dst = &w->stage[r][0];
+                            dst[0] = d[1];
+                            dst[1] = d[2];
+                            dst[2] = 0u;
+                            dst[3] = d[0];
+#pragma unroll
+                            for (int i = 0; i < CtxShape<P>::kFWr; ++i) dst[4 + i] = (i < P::kFrameWords) ? d[3 + (i < P::kFrameWords ? i : 0)] : 0u;
+                            w->stage_id[r] = oid;
+                        }
+                        wave_sync();
+                        write_staged<P>(dev, rs, w, (uint32_t)popc(sm));
+                        wave_sync();
+                    }
                 }
                 if (lane == 0) {
                     vst(&w->valid[q], (uint64_t)0);
