@@ -56,6 +56,9 @@ class RunStats(ctypes.Structure):
         ("steal_attempts", ctypes.c_uint64),
         ("jobs_taken", ctypes.c_uint64),
         ("pool_refills", ctypes.c_uint64),
+        ("published", ctypes.c_uint64),
+        ("overflowed", ctypes.c_uint64),
+        ("suspended", ctypes.c_uint64),
         ("workers", ctypes.c_uint32),
         ("blocks", ctypes.c_uint32),
         ("error_code", ctypes.c_int32),

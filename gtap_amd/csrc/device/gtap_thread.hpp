@@ -75,6 +75,7 @@ struct WaveCtx {
     uint32_t cur_queue;
     uint32_t spawns;
     uint32_t refills;
+    uint32_t overflowed;    // LDS-held records written out by an append that pushed them out
     uint32_t tail[Q];       // owner's private tail per queue (absolute position)
     uint32_t bottom[Q];     // published bottom per queue (low word of the deque word)
     uint32_t top_cache[Q];  // last top observed per queue
@@ -342,12 +343,16 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w
                 const uint32_t slot = pos & (kWave - 1);
                 const uint64_t vmask = uniform(vld(&w->valid[qq]));
                 const uint64_t lmask = uniform(vld(&w->inlds[qq]));
-                if ((vmask >> slot) & 1ull) {
+                const bool spill = (vmask >> slot) & 1ull;
+                const bool spill_rec = spill && qq < QL && ((lmask >> slot) & 1ull);
+                const uint64_t om = ballot(spill_rec);
+                if (spill) {
                     // the slot still holds the private entry at pos - 64
                     const uint32_t oid = vld(&w->mirror[qq][slot]);
-                    if (qq < QL && ((lmask >> slot) & 1ull)) record_from_desc<P>(dev, rs, oid, &w->desc[qq < QL ? qq : 0][slot][0]);
+                    if (spill_rec) record_from_desc<P>(dev, rs, oid, &w->desc[qq < QL ? qq : 0][slot][0]);
                     st32(dq_slot_ptr(dev, qq, me, pos - kWave), oid);
                 }
+                if (om && r == 0) vst(&w->overflowed, vld(&w->overflowed) + (uint32_t)popc(om));
                 vst(&w->mirror[qq][slot], id);
                 if constexpr (kDesc) {
                     if (qq < QL) {
@@ -700,7 +705,7 @@ __device__ __forceinline__ uint32_t run_wave_task(const GtapDev& dev, const type
 // ------------------------------------------------------------------ scheduler
 
 struct LoopStats {
-    uint64_t tasks, rounds, pops, steals, attempts, jobs;
+    uint64_t tasks, rounds, pops, steals, attempts, jobs, published, suspended;
 };
 
 template <class P>
@@ -934,6 +939,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         vst(&w->cur_queue, 0u);
         vst(&w->spawns, 0u);
         vst(&w->refills, 0u);
+        vst(&w->overflowed, 0u);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             vst(&w->tail[q], 0u);
@@ -963,7 +969,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     bool idle_registered = false;
     uint32_t idle_iter = 0;
     uint32_t busy_iter = 0;
-    LoopStats st = {0, 0, 0, 0, 0, 0};
+    LoopStats st = {0, 0, 0, 0, 0, 0, 0, 0};
     // timeline profiling (lane 0, when dev.prof_max > 0)
     uint32_t ph_idx = 0, pw_idx = 0;
     bool had_tasks = false;
@@ -1113,6 +1119,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                     for (int i = 0; i < CtxShape<P>::kFWr; ++i) dst[4 + i] = L.frame[i];
                     w->stage_id[r] = L.tid | (L.lds ? 0u : kHeaderOnly);  // frame already in HBM
                 }
+                st.suspended += (uint32_t)popc(sm);
                 wave_sync();
                 write_staged<P>(dev, L.rs, w, (uint32_t)popc(sm));
                 wave_sync();
@@ -1204,6 +1211,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                             for (int i = 0; i < CtxShape<P>::kFWr; ++i) dst[4 + i] = (i < P::kFrameWords) ? d[3 + (i < P::kFrameWords ? i : 0)] : 0u;
                             w->stage_id[r] = oid;
                         }
+                        st.published += (uint32_t)popc(sm);
                         wave_sync();
                         write_staged<P>(dev, rs, w, (uint32_t)popc(sm));
                         wave_sync();
@@ -1261,6 +1269,9 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         s[ST_STEAL_ATTEMPTS] = st.attempts;
         s[ST_JOBS] = st.jobs;
         s[ST_REFILLS] = vld(&w->refills);
+        s[ST_PUBLISHED] = st.published;
+        s[ST_OVERFLOWED] = vld(&w->overflowed);
+        s[ST_SUSPENDED] = st.suspended;
     }
 }
 

@@ -83,6 +83,9 @@ typedef struct gtap_run_stats {
     uint64_t steal_attempts;  /* CAS attempts on victims */
     uint64_t jobs_taken;      /* roots admitted from the job table */
     uint64_t pool_refills;    /* id chunks taken from the global pool */
+    uint64_t published;       /* task records written out when published (thread mode) */
+    uint64_t overflowed;      /* task records written out when pushed out of the LDS window */
+    uint64_t suspended;       /* taskwaits that suspended a task */
     uint32_t workers;         /* wavefront workers in the launch */
     uint32_t blocks;          /* workgroups in the launch */
     int32_t error_code;
@@ -92,7 +95,7 @@ typedef struct gtap_run_stats {
 /* Tunables of the scheduler (worker count, deque capacity, steal batch width, ...).
  * Zero fields take the defaults. */
 typedef struct gtap_config {
-    uint64_t pool_bytes;      /* bytes of task records (default 6 GiB) */
+    uint64_t pool_bytes;      /* bytes of task records (default and maximum 4 GiB) */
     uint32_t record_stride;   /* bytes per task record (default: max over built-in programs) */
     uint32_t deque_capacity;  /* slots per worker per queue, power of two (default 16384) */
     uint32_t num_queues;      /* EPAQ queues provisioned (default 4) */

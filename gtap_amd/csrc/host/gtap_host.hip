@@ -178,6 +178,9 @@ int finish_launch(const GtapDev& dev, uint32_t blocks, gtap_run_stats* stats) {
         s.steal_attempts += p[ST_STEAL_ATTEMPTS];
         s.jobs_taken += p[ST_JOBS];
         s.pool_refills += p[ST_REFILLS];
+        s.published += p[ST_PUBLISHED];
+        s.overflowed += p[ST_OVERFLOWED];
+        s.suspended += p[ST_SUSPENDED];
     }
     s.workers = dev.nworkers;
     s.blocks = blocks;
