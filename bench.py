@@ -43,18 +43,36 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nq_tasks: int) -> dict:
-    """Sequential CPU n-queens(14) on this host: the reference's seq_nq when it was built
-    (oracle/_ref), else the oracle's restatement."""
+def _ref_ms(binary: str, args: list, runs: int, env=None, timeout: int = 300) -> list:
+    """Run one of the reference's CPU programs `runs` times; its own "Execution time: X ms"."""
+    out = []
+    for _ in range(runs):
+        r = subprocess.run([binary, *map(str, args)], capture_output=True, text=True, timeout=timeout, env=env)
+        m = re.search(r"Execution time: ([0-9.]+) ms", r.stdout)
+        if m:
+            out.append(float(m.group(1)))
+    return out
+
+
+def _median_iqr(xs: list) -> dict:
+    xs = sorted(xs)
+    n = len(xs)
+    rank = lambda q: xs[min(n - 1, max(0, int(-(-q * n // 1)) - 1))]  # nearest rank
+    return {"median_ms": statistics.median(xs), "q1_ms": rank(0.25), "q3_ms": rank(0.75), "runs": n}
+
+
+def cpu_baseline(nq_tasks: int, headline_only: bool = False) -> dict:
+    """CPU baselines timed on this host.
+
+    value: sequential n-queens(14) -- the reference's seq_nq when it was built (oracle/_ref),
+    else the oracle's restatement. `programs`: the reference's own sequential and OpenMP
+    programs on the other configs (fib(30), cilksort of 2^26 keys) and the mixed job stream
+    on a bounded sample, with the thread count used."""
     import oracle
     ref = oracle.ref_binary("seq_nq")
     times = []
     if ref:
-        for _ in range(3):
-            out = subprocess.run([ref, str(NQ_N)], capture_output=True, text=True, timeout=300).stdout
-            m = re.search(r"Execution time: ([0-9.]+) ms", out)
-            if m:
-                times.append(float(m.group(1)) / 1e3)
+        times = [t / 1e3 for t in _ref_ms(ref, [NQ_N], 3)]
         kind = "reference"
         sample = f"reference seq_nq {NQ_N} (evaluation/2-comparison/nq/seq_nq.c), full workload, median of 3 runs"
     if not times:
@@ -65,8 +83,55 @@ def cpu_baseline(nq_tasks: int) -> dict:
         kind = "port"
         sample = f"oracle nq({NQ_N}) (restatement of seq_nq.c), full workload, median of 3 runs"
     t = statistics.median(times)
-    return {"value": nq_tasks / t, "unit": "tasks/s", "cores": 1, "kind": kind, "sample": sample,
-            "seconds": t}
+    out = {"value": nq_tasks / t, "unit": "tasks/s", "cores": 1, "kind": kind, "sample": sample, "seconds": t}
+    if headline_only:
+        return out
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="64M")
+    progs = {"threads_openmp": threads, "cpu_model": _cpu_model()}
+    runs = [("seq_fib", [30], 5, None), ("omp_fib", [30], 1, env), ("seq_nq", [NQ_N], 3, None),
+            ("omp_nq", [NQ_N, 7], 3, env)]
+    for name, args, n, e in runs:
+        b = oracle.ref_binary(name)
+        if b:
+            ms = _ref_ms(b, args, n, env=e)
+            if ms:
+                progs[f"{name} {' '.join(map(str, args))}"] = _median_iqr(ms)
+    gen = oracle.ref_binary("gen_vector")
+    if gen and oracle.ref_binary("seq_cilksort"):
+        path = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"gtap_keys_{os.getpid()}.bin")
+        try:
+            subprocess.run([gen, str(1 << 26), path], capture_output=True, timeout=300, check=True)
+            for name, n, e in (("seq_cilksort", 1, None), ("omp_cilksort", 1, env)):
+                b = oracle.ref_binary(name)
+                ms = _ref_ms(b, [path], n, env=e) if b else []
+                if ms:
+                    progs[f"{name} 2^26"] = _median_iqr(ms)
+        finally:
+            if os.path.exists(path):
+                os.remove(path)
+    # mixed job stream: the oracle's sequential restatement over the first 256 jobs
+    kinds, params, kseeds = oracle.jobs(2024, 4096)
+    t0 = time.perf_counter()
+    nj = 256
+    for i in range(nj):
+        oracle.job_result(int(kinds[i]), int(params[i]), int(kseeds[i]))
+    dt = time.perf_counter() - t0
+    progs["stream sequential (oracle), first 256 of 4096 jobs"] = {"seconds": dt, "jobs_per_s": nj / dt}
+    out["programs"] = progs
+    return out
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def bench_cilksort(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
@@ -253,7 +318,7 @@ def main() -> int:
         extra_stream = bench_stream(gtap_amd)
         fences = gtap_amd.bench_fences(max(4, workers // 4 * 4), 64)
 
-    cpu = None if args.no_cpu else cpu_baseline(nq_tasks)
+    cpu = None if args.no_cpu else cpu_baseline(nq_tasks, args.headline_only)
     gtap_amd.gtap_finalize()
 
     line = {
@@ -285,7 +350,8 @@ def main() -> int:
             "frac": achieved_gbps / roofline.MI355X_HBM_GBPS,
             "traffic": traffic,
         },
-        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+        "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample", "programs")
+                                                  if k in cpu},
         "extra": {
             "kernel_ms_median": statistics.median(kernel_ms),
             "kernel_tasks_per_s": nq_tasks / (statistics.median(kernel_ms) / 1e3),

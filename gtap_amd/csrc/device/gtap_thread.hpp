@@ -969,6 +969,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     bool idle_registered = false;
     uint32_t idle_iter = 0;
     uint32_t busy_iter = 0;
+    bool hungry_reg = false;    // this wave is counted in CTL_HUNGRY
+    uint32_t hungry_seen = 0;   // CTL_HUNGRY as last read (every 4th round)
     LoopStats st = {0, 0, 0, 0, 0, 0, 0, 0};
     // timeline profiling (lane 0, when dev.prof_max > 0)
     uint32_t ph_idx = 0, pw_idx = 0;
@@ -1001,6 +1003,14 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                     n = try_steal<P>(dev, me, q, rng, my_tid, idle_registered, st);
                     if (n) batch_q = (int)q;
                 }
+            }
+            if (n == 0 && !hungry_reg) {
+                hungry_reg = true;
+                if (lane == 0) atomic_add_noret(dev.ctl + CTL_HUNGRY * 32, 1);
+            }
+            if (n != 0 && hungry_reg) {
+                hungry_reg = false;
+                if (lane == 0) atomic_add_noret(dev.ctl + CTL_HUNGRY * 32, -1);
             }
             if (n == 0) {
                 if (had_tasks) {
@@ -1155,7 +1165,11 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             }
         }
         const uint32_t keep_cap = ((int)kmax == WQ) ? 1u : (uint32_t)kWave;
-        const uint32_t keep = cmax < keep_cap ? cmax : keep_cap;
+        uint32_t keep = cmax < keep_cap ? cmax : keep_cap;
+        // Share when many waves are hungry (start-up, drain): keep half of a batch that
+        // would otherwise stay private and publish the rest for the thieves.
+        if ((busy_iter & 3u) == 0) hungry_seen = uniform((uint32_t)ld32(dev.ctl + CTL_HUNGRY * 32));
+        if (keep == cmax && cmax >= 2 && hungry_seen > (dev.nworkers >> 4)) keep = (cmax + 1) / 2;
         uint32_t next_tid = 0, next_slot = 0;
         bool next_lds = false;
         if (keep) {
