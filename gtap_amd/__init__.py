@@ -236,6 +236,28 @@ def gtap_visualize_profile(app_name: str) -> None:
     lib().gtap_visualize_profile(app_name.encode())
 
 
+_LOWERED: dict = {}
+
+
+def lowered(name: str) -> ctypes.CDLL:
+    """A program lowered from `#pragma gtap` source by gtap_amd.lower and built by
+    gtap_amd.build (gtap_amd/_lib/libgtap_lowered_<name>.so). Its entry points are
+    `gtap_entry_<kernel>(<kernel params>, [<result>*,] gtap_run_stats*)` and
+    `gtap_lowered_<name>_record_bytes()`; their argument types come from the source, so the
+    caller sets `argtypes` itself."""
+    if name not in _LOWERED:
+        lib()  # the runtime library first: the lowered library links against it
+        path = os.path.join(HERE, "_lib", f"libgtap_lowered_{name}.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run `python -m gtap_amd.build`")
+        L = ctypes.CDLL(path)
+        rb = getattr(L, f"gtap_lowered_{name}_record_bytes")
+        rb.argtypes = []
+        rb.restype = ctypes.c_uint32
+        _LOWERED[name] = L
+    return _LOWERED[name]
+
+
 def gtap_set_profiling(enable: bool, max_samples: int = 4096) -> int:
     return lib().gtap_set_profiling(1 if enable else 0, max_samples)
 

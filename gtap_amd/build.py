@@ -69,7 +69,41 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_lowered(jobs)
     return LIB
+
+
+EXAMPLES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lower", "examples")
+GEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_gen")
+
+
+def lowered_lib(name: str) -> str:
+    return os.path.join(LIBDIR, f"libgtap_lowered_{name}.so")
+
+
+def _build_one_lowered(src: str) -> str:
+    from gtap_amd import lower
+    name = os.path.basename(src).split(".")[0]
+    gen = os.path.join(GEN, f"{name}.lowered.hip")
+    lower.lower_file(src, gen, name)
+    out = lowered_lib(name)
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *CFLAGS, "-I", CSRC, "-shared", gen, "-L", LIBDIR, "-lgtap_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
+        raise RuntimeError(f"hipcc failed for the lowered {src}:\n{msgs}")
+    os.replace(tmp, out)
+    return out
+
+
+def build_lowered(jobs: int | None = None) -> list[str]:
+    """Lowers every `#pragma gtap` example (gtap_amd/lower/examples/*.gtap.cu) with the
+    lowering tool and builds each into gtap_amd/_lib/libgtap_lowered_<name>.so."""
+    os.makedirs(GEN, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(EXAMPLES, "*.gtap.cu")))
+    with cf.ThreadPoolExecutor(max_workers=jobs or min(4, max(1, len(srcs)))) as ex:
+        return list(ex.map(_build_one_lowered, srcs))
 
 
 if __name__ == "__main__":
