@@ -36,7 +36,6 @@ enum CtlIndex : int {
     CTL_HOST_AVAIL = 7,   // serve mode: device mirror of HCTL_JOB_AVAIL
     CTL_HOST_STOP = 8,    // serve mode: device mirror of HCTL_STOP
     CTL_HOST_POLL_T = 9,  // serve mode: wall-clock tick of the last host poll (64-bit)
-    CTL_HUNGRY = 10,      // waves that found no work on their last try (thread mode)
     CTL_COUNT = 12
 };
 

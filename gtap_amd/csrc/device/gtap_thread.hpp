@@ -87,8 +87,9 @@ struct WaveCtx {
     // records written out together after a round (suspended tasks, published entries): the
     // 16-byte header and S/4 - 1 frame pieces each, spread over the lanes so that the pieces
     // of one record leave in one store instruction and merge into one memory transaction
-    uint32_t stage[kWave][S];
-    uint32_t stage_id[kWave];
+    uint32_t stage[2 * kWave][S];
+    uint32_t stage_id[2 * kWave];
+    uint32_t nstage;        // records staged so far this round (entries pushed out of the window)
 };
 
 template <class P>
@@ -311,6 +312,20 @@ __device__ __forceinline__ void record_from_desc(const GtapDev& dev, __amdgpu_bu
     frame_words_st<FW>(rs, off, f);
 }
 
+// Copies an LDS-held entry into staging entry `at` in record order (header, frame words).
+template <class P, class W>
+__device__ __forceinline__ void stage_from_desc(W* w, uint32_t at, uint32_t id, const uint32_t* d) {
+    constexpr int FW = CtxShape<P>::kFW, FWr = CtxShape<P>::kFWr;
+    volatile uint32_t* dst = &w->stage[at][0];
+    dst[0] = vld(&d[1]);
+    dst[1] = vld(&d[2]);
+    dst[2] = 0u;
+    dst[3] = vld(&d[0]);
+#pragma unroll
+    for (int i = 0; i < FWr; ++i) dst[4 + i] = (i < FW) ? vld(&d[3 + (i < FW ? i : 0)]) : 0u;
+    vst(&w->stage_id[at], id);
+}
+
 // Append `id` to the private (unpublished) end of queue `q` of this wave. Called by all
 // active lanes; lanes may target different queues. kDesc: the entry's record is held in
 // LDS (the D descriptor words `dw`), else it is already in HBM. An older private entry that
@@ -347,12 +362,27 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w
                 const bool spill_rec = spill && qq < QL && ((lmask >> slot) & 1ull);
                 const uint64_t om = ballot(spill_rec);
                 if (spill) {
-                    // the slot still holds the private entry at pos - 64
+                    // the slot still holds the private entry at pos - 64: its deque slot now,
+                    // its record (if held in LDS) staged for the writes after the round, or
+                    // written at once when the staging area is full
                     const uint32_t oid = vld(&w->mirror[qq][slot]);
-                    if (spill_rec) record_from_desc<P>(dev, rs, oid, &w->desc[qq < QL ? qq : 0][slot][0]);
+                    if (spill_rec) {
+                        const uint32_t base = uniform(vld(&w->nstage));
+                        const uint32_t at = base + (uint32_t)rank_in(om);
+                        const uint32_t* d = &w->desc[qq < QL ? qq : 0][slot][0];
+                        if (base + (uint32_t)popc(om) <= 2u * kWave) {
+                            stage_from_desc<P>(w, at, oid, d);
+                        } else {
+                            record_from_desc<P>(dev, rs, oid, d);
+                        }
+                    }
                     st32(dq_slot_ptr(dev, qq, me, pos - kWave), oid);
                 }
-                if (om && r == 0) vst(&w->overflowed, vld(&w->overflowed) + (uint32_t)popc(om));
+                if (om && r == 0) {
+                    const uint32_t base = vld(&w->nstage);
+                    if (base + (uint32_t)popc(om) <= 2u * kWave) vst(&w->nstage, base + (uint32_t)popc(om));
+                    vst(&w->overflowed, vld(&w->overflowed) + (uint32_t)popc(om));
+                }
                 vst(&w->mirror[qq][slot], id);
                 if constexpr (kDesc) {
                     if (qq < QL) {
@@ -940,6 +970,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         vst(&w->spawns, 0u);
         vst(&w->refills, 0u);
         vst(&w->overflowed, 0u);
+        vst(&w->nstage, 0u);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             vst(&w->tail[q], 0u);
@@ -969,8 +1000,6 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     bool idle_registered = false;
     uint32_t idle_iter = 0;
     uint32_t busy_iter = 0;
-    bool hungry_reg = false;    // this wave is counted in CTL_HUNGRY
-    uint32_t hungry_seen = 0;   // CTL_HUNGRY as last read (every 4th round)
     LoopStats st = {0, 0, 0, 0, 0, 0, 0, 0};
     // timeline profiling (lane 0, when dev.prof_max > 0)
     uint32_t ph_idx = 0, pw_idx = 0;
@@ -1003,14 +1032,6 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                     n = try_steal<P>(dev, me, q, rng, my_tid, idle_registered, st);
                     if (n) batch_q = (int)q;
                 }
-            }
-            if (n == 0 && !hungry_reg) {
-                hungry_reg = true;
-                if (lane == 0) atomic_add_noret(dev.ctl + CTL_HUNGRY * 32, 1);
-            }
-            if (n != 0 && hungry_reg) {
-                hungry_reg = false;
-                if (lane == 0) atomic_add_noret(dev.ctl + CTL_HUNGRY * 32, -1);
             }
             if (n == 0) {
                 if (had_tasks) {
@@ -1115,9 +1136,12 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
 
         // ---- epilogue: suspended records, joins, continuations, id release, next batch, publish
         {
+            // records staged during the round (entries pushed out of the window), then the
+            // tasks that suspended (up to 64 more), all written together
             const uint64_t sm = ballot(sus);
+            const uint32_t base = uniform(vld(&w->nstage));
             if (sm) {
-                const uint32_t r = (uint32_t)rank_in(sm);
+                const uint32_t r = base + (uint32_t)rank_in(sm);
                 if (sus) {
                     const u32x4 h = L.suspended_header();
                     uint32_t* dst = &w->stage[r][0];
@@ -1130,9 +1154,13 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                     w->stage_id[r] = L.tid | (L.lds ? 0u : kHeaderOnly);  // frame already in HBM
                 }
                 st.suspended += (uint32_t)popc(sm);
+            }
+            const uint32_t total = base + (uint32_t)popc(sm);
+            if (total) {
                 wave_sync();
-                write_staged<P>(dev, L.rs, w, (uint32_t)popc(sm));
+                write_staged<P>(dev, L.rs, w, total);
                 wave_sync();
+                if (lane == 0) vst(&w->nstage, 0u);
             }
         }
         complete_stores();  // results, suspended records and child records of this round
@@ -1165,11 +1193,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             }
         }
         const uint32_t keep_cap = ((int)kmax == WQ) ? 1u : (uint32_t)kWave;
-        uint32_t keep = cmax < keep_cap ? cmax : keep_cap;
-        // Share when many waves are hungry (start-up, drain): keep half of a batch that
-        // would otherwise stay private and publish the rest for the thieves.
-        if ((busy_iter & 3u) == 0) hungry_seen = uniform((uint32_t)ld32(dev.ctl + CTL_HUNGRY * 32));
-        if (keep == cmax && cmax >= 2 && hungry_seen > (dev.nworkers >> 4)) keep = (cmax + 1) / 2;
+        const uint32_t keep = cmax < keep_cap ? cmax : keep_cap;
         uint32_t next_tid = 0, next_slot = 0;
         bool next_lds = false;
         if (keep) {
