@@ -89,7 +89,7 @@ struct WaveCtx {
     // of one record leave in one store instruction and merge into one memory transaction
     uint32_t stage[2 * kWave][S];
     uint32_t stage_id[2 * kWave];
-    uint32_t nstage;        // records staged so far this round (entries pushed out of the window)
+    uint32_t nstage;        // records staged so far this round (entries pushed out of the window, <= 64)
 };
 
 template <class P>
@@ -370,7 +370,7 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w
                         const uint32_t base = uniform(vld(&w->nstage));
                         const uint32_t at = base + (uint32_t)rank_in(om);
                         const uint32_t* d = &w->desc[qq < QL ? qq : 0][slot][0];
-                        if (base + (uint32_t)popc(om) <= 2u * kWave) {
+                        if (base + (uint32_t)popc(om) <= (uint32_t)kWave) {  // + <= 64 suspended after the round
                             stage_from_desc<P>(w, at, oid, d);
                         } else {
                             record_from_desc<P>(dev, rs, oid, d);
@@ -380,7 +380,7 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w
                 }
                 if (om && r == 0) {
                     const uint32_t base = vld(&w->nstage);
-                    if (base + (uint32_t)popc(om) <= 2u * kWave) vst(&w->nstage, base + (uint32_t)popc(om));
+                    if (base + (uint32_t)popc(om) <= (uint32_t)kWave) vst(&w->nstage, base + (uint32_t)popc(om));
                     vst(&w->overflowed, vld(&w->overflowed) + (uint32_t)popc(om));
                 }
                 vst(&w->mirror[qq][slot], id);
