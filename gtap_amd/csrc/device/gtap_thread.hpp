@@ -1213,6 +1213,16 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 vst(&w->cur_queue, kmax);
             }
         }
+        // records pushed out of the window by the continuations appended above
+        {
+            const uint32_t ns = uniform(vld(&w->nstage));
+            if (ns) {
+                wave_sync();
+                write_staged<P>(dev, L.rs, w, ns);
+                wave_sync();
+                if (lane == 0) vst(&w->nstage, 0u);
+            }
+        }
         // write out what is published: entries still in the 64-slot window (lane i takes
         // slot i) get their deque slot, and their record if it is held in LDS
         const __amdgpu_buffer_rsrc_t rs = L.rs;
