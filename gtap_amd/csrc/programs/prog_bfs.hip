@@ -79,7 +79,7 @@ __device__ __forceinline__ void bfs_task(L& lane) {
     const int32_t s = lane.template arg<int32_t>(4);
     const int32_t e = lane.template arg<int32_t>(8);
     const int32_t d = lane.template arg<int32_t>(12);
-    if (ld32(g.depth + v) < d) {  // stale: v was reached by a shorter path meanwhile
+    if (ld32(reinterpret_cast<const int32_t*>(g.depth) + v) < d) {  // stale: v was reached by a shorter path meanwhile
         __gtap_finish_task(lane);
         return;
     }
@@ -91,18 +91,52 @@ __device__ __forceinline__ void bfs_task(L& lane) {
         __gtap_finish_task(lane);
         return;
     }
+    // Edges in chunks of kChunkEdges: the column loads, the depth probes and the atomicMins of
+    // a chunk are issued back to back, so their latencies overlap instead of adding up edge
+    // by edge. A plain load of depth[u] filters the atomic: depths only decrease, so a stale
+    // cached value is never below the true one and can only cost an unneeded atomic.
+    constexpr int kChunkEdges = 8;
     const int32_t nd = d + 1;
-    for (int32_t i = s; i < e; ++i) {
-        const int32_t u = g.col[i];
-        if ((uint32_t)u >= (uint32_t)g.nv) {
-            GTAP_DEVICE_ERROR(*lane.dev, GTAP_ERROR_INVALID_ARGUMENT, lane.tid, -1, u, g.nv);
+    for (int32_t b = s; b < e; b += kChunkEdges) {
+        int32_t u[kChunkEdges];
+#pragma unroll
+        for (int j = 0; j < kChunkEdges; ++j) u[j] = (b + j < e) ? g.col[b + j] : -1;
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < kChunkEdges; ++j) {
+            if (u[j] >= g.nv || u[j] < -1) {
+                bad = true;
+                u[j] = -1;
+            }
+        }
+        if (bad) {
+            GTAP_DEVICE_ERROR(*lane.dev, GTAP_ERROR_INVALID_ARGUMENT, lane.tid, -1, -1, g.nv);
             break;
         }
-        const int32_t old = __hip_atomic_fetch_min(g.depth + u, nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old > nd) {
-            const int32_t us = g.row[u];
-            const int32_t ue = g.row[u + 1];
-            __gtap_spawn_task_detached(lane, F_BFS, route<NQ>(ue - us), BfsArg{u, us, ue, nd});
+        int32_t old[kChunkEdges];
+#pragma unroll
+        for (int j = 0; j < kChunkEdges; ++j) old[j] = (u[j] >= 0) ? g.depth[u[j]] : 0;
+#pragma unroll
+        for (int j = 0; j < kChunkEdges; ++j) {
+            if (u[j] >= 0 && old[j] > nd)
+                old[j] = __hip_atomic_fetch_min(g.depth + u[j], nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                old[j] = 0;
+        }
+        // one spawn site for the improved vertices of the chunk (keeps the register count of the
+        // spawn path single)
+        uint32_t hits = 0;
+#pragma unroll
+        for (int j = 0; j < kChunkEdges; ++j) hits |= (old[j] > nd ? 1u : 0u) << j;
+        while (hits) {
+            const int j = __builtin_ctz(hits);
+            hits &= hits - 1u;
+            int32_t uj = u[0];
+#pragma unroll
+            for (int k = 1; k < kChunkEdges; ++k) uj = (j == k) ? u[k] : uj;
+            const int32_t us = g.row[uj];
+            const int32_t ue = g.row[uj + 1];
+            __gtap_spawn_task_detached(lane, F_BFS, route<NQ>(ue - us), BfsArg{uj, us, ue, nd});
         }
     }
     __gtap_finish_task(lane);
