@@ -1225,13 +1225,19 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         }
         // write out what is published: entries still in the 64-slot window (lane i takes
         // slot i) get their deque slot, and their record if it is held in LDS
+        // Queues other than the kept one (several EPAQ queues) keep their 64-slot window
+        // private for later rounds; only the entries pushed out of it are published.
         const __amdgpu_buffer_rsrc_t rs = L.rs;
         bool any_publish = false;
+        uint32_t pub_to[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const uint32_t t = uniform(vld(&w->tail[q]));
             const uint32_t b = uniform(vld(&w->bottom[q]));
-            if (t != b) {
+            const bool retain = Q > 1 && q < QL && (uint32_t)q != kmax;
+            pub_to[q] = !retain ? t : ((int32_t)(t - b) > kWave ? t - kWave : b);
+            if (retain && pub_to[q] != b) any_publish = true;
+            if (!retain && t != b) {
                 any_publish = true;
                 const uint64_t vmask = uniform(vld(&w->valid[q]));
                 const uint64_t lmask = uniform(vld(&w->inlds[q]));
@@ -1275,8 +1281,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             complete_stores();  // records and slots of everything published below
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                const uint32_t t = uniform(vld(&w->tail[q]));
                 const uint32_t b = uniform(vld(&w->bottom[q]));
+                const uint32_t t = pub_to[q];
                 if (t != b) {
                     // top_cache stays a lower bound of the top; the capacity check re-reads it
                     if (lane == 0) atomic_add_noret(dq_word_ptr(dev, q, me), (uint64_t)(t - b));
