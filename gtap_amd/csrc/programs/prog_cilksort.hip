@@ -93,13 +93,24 @@ __device__ __forceinline__ int64_t binary_search_gm(const uint32_t* x, int64_t l
     return low;
 }
 
+// Per-thread batch of kPer keys of a kTile-key tile: element e = k * kT + t (k < kPer), so the
+// loads of a warp are contiguous, and all kPer loads are issued before any is used.
+constexpr int kPer = kTile / (int)kT;  // 16
+
 // Workgroup sort of arr[0..n), n <= 4096: bitonic network over 4096 LDS slots (padded with
 // 0xFFFFFFFF, which sorts to the end and is never written back).
 template <class B>
 __device__ void leaf_sort(B& blk, uint32_t* arr, int64_t n) {
     uint32_t* k = blk.sh->keys;
     const uint32_t t = threadIdx.x;
-    for (uint32_t i = t; i < (uint32_t)kTile; i += kT) k[i] = (int64_t)i < n ? ldk(arr + i) : 0xFFFFFFFFu;
+    uint32_t r[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const uint32_t e = (uint32_t)q * kT + t;
+        r[q] = (int64_t)e < n ? ldk(arr + e) : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) k[(uint32_t)q * kT + t] = r[q];
     __syncthreads();
     for (uint32_t size = 2; size <= (uint32_t)kTile; size <<= 1) {
         for (uint32_t j = size >> 1; j > 0; j >>= 1) {
@@ -116,17 +127,46 @@ __device__ void leaf_sort(B& blk, uint32_t* arr, int64_t n) {
             __syncthreads();
         }
     }
-    for (uint32_t i = t; (int64_t)i < n; i += kT) stk(arr + i, k[i]);
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const uint32_t e = (uint32_t)q * kT + t;
+        if ((int64_t)e < n) stk(arr + e, k[e]);
+    }
 }
 
-// Workgroup copy of n keys.
+// Workgroup copy of n keys, kTile keys per step with all loads of a step in flight.
 template <class B>
 __device__ void block_copy(B& blk, const uint32_t* src, uint32_t* dst, int64_t n) {
     (void)blk;
-    for (int64_t i = threadIdx.x; i < n; i += kT) stk(dst + i, ldk(src + i));
+    const uint32_t t = threadIdx.x;
+    for (int64_t base = 0; base < n; base += kTile) {
+        uint32_t r[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int64_t e = base + (int64_t)q * kT + t;
+            r[q] = e < n ? ldk(src + e) : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int64_t e = base + (int64_t)q * kT + t;
+            if (e < n) stk(dst + e, r[q]);
+        }
+    }
 }
 
-// Workgroup merge of a[0..na) and b[0..nb) into dst, tile by tile (merge path).
+// The inputs of one merge tile: a[i0 .. i0+la) then b[j0 .. j0+lb), loaded kPer per thread.
+__device__ __forceinline__ void load_tile(const uint32_t* a, int64_t i0, int64_t la, const uint32_t* b, int64_t j0,
+                                          int64_t lb, uint32_t (&r)[kPer]) {
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int64_t e = (int64_t)q * kT + t;
+        r[q] = e < la ? ldk(a + i0 + e) : (e < la + lb ? ldk(b + j0 + (e - la)) : 0u);
+    }
+}
+
+// Workgroup merge of a[0..na) and b[0..nb) into dst, tile by tile (merge path). The inputs of
+// the next tile are loaded into registers while the current tile is merged in LDS.
 template <class B>
 __device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t* b, int64_t nb, uint32_t* dst) {
     typename Program::Shared* sh = blk.sh;
@@ -137,26 +177,49 @@ __device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t
     for (int64_t base = 0; base < total; base += (int64_t)kTile * 16) {
         // splits of up to 16 tiles, found in parallel
         const int64_t ntiles_here = ((total - base) + kTile - 1) / kTile;
-        const int64_t nsplit = (ntiles_here < 16 ? ntiles_here : 16) + 1;
-        if ((int64_t)t < nsplit) {
+        const int64_t ntl = ntiles_here < 16 ? ntiles_here : 16;
+        if ((int64_t)t <= ntl) {
             int64_t d = base + (int64_t)t * kTile;
             if (d > total) d = total;
             sh->split[t] = merge_path(ga, na, gb, nb, d);
         }
         __syncthreads();
-        for (int64_t ti = 0; ti + 1 < nsplit; ++ti) {
+        // The splits of sorted inputs are monotone along both inputs; anything else (unsorted
+        // runs) would index outside the tile, so it is reported and the merge abandoned.
+        bool ok = true;
+        for (int64_t ti = 0; ti < ntl; ++ti) {
+            const int64_t d0 = base + ti * kTile, d1 = (d0 + kTile < total) ? d0 + kTile : total;
+            const int64_t i0 = sh->split[ti], i1 = sh->split[ti + 1];
+            const int64_t la = i1 - i0, lb = (d1 - i1) - (d0 - i0);
+            ok = ok && la >= 0 && lb >= 0 && i1 <= na && (d1 - i1) <= nb;
+        }
+        if (!ok) {
+            if (t == 0) GTAP_DEVICE_ERROR(*blk.dev, GTAP_ERROR_INVALID_ARGUMENT, blk.tid, -1, (int)na, (int)nb);
+            __syncthreads();
+            return;
+        }
+        uint32_t r[kPer];
+        {
+            const int64_t i0 = sh->split[0], i1 = sh->split[1];
+            const int64_t d0 = base, d1 = (d0 + kTile < total) ? d0 + kTile : total;
+            load_tile(a, i0, i1 - i0, b, d0 - i0, (d1 - i1) - (d0 - i0), r);
+        }
+        for (int64_t ti = 0; ti < ntl; ++ti) {
             const int64_t d0 = base + ti * kTile;
             const int64_t d1 = (d0 + kTile < total) ? d0 + kTile : total;
             const int64_t i0 = sh->split[ti], i1 = sh->split[ti + 1];
             const int64_t la = i1 - i0, lb = (d1 - i1) - (d0 - i0);
-            const int64_t j0 = d0 - i0;
-            for (int64_t i = t; i < la; i += kT) sh->keys[i] = ldk(a + i0 + i);
-            for (int64_t i = t; i < lb; i += kT) sh->keys[la + i] = ldk(b + j0 + i);
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) sh->keys[(uint32_t)q * kT + t] = r[q];
             __syncthreads();
+            if (ti + 1 < ntl) {  // next tile's inputs in flight during this merge
+                const int64_t e0 = d1, e1 = (e0 + kTile < total) ? e0 + kTile : total;
+                const int64_t n0 = sh->split[ti + 1], n1 = sh->split[ti + 2];
+                load_tile(a, n0, n1 - n0, b, e0 - n0, (e1 - n1) - (e0 - n0), r);
+            }
             const uint32_t* A = sh->keys;
             const uint32_t* Bv = sh->keys + la;
             const int64_t len = d1 - d0;
-            constexpr int kPer = kTile / (int)kT;  // 16 outputs per thread
             const int64_t od = (int64_t)t * kPer;
             if (od < len) {
                 auto la_ = [A](int64_t i) { return A[i]; };
@@ -172,9 +235,13 @@ __device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t
                 }
             }
             __syncthreads();
-            for (int64_t o = t; o < len; o += kT) stk(dst + d0 + o, sh->out[o]);
-            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                const int64_t o = (int64_t)q * kT + t;
+                if (o < len) stk(dst + d0 + o, sh->out[o]);
+            }
         }
+        __syncthreads();
     }
 }
 
