@@ -11,11 +11,14 @@
 //                           task cilkmerge(tmp,len12, tmp+len12,len34, arr); taskwait
 //   cilkmerge(a,na,b,nb,dst): make na >= nb; nb == 0 -> copy;
 //                           na + nb <= MERGE_LEAF -> the workgroup merges in LDS tiles;
-//                           else a_split = (na+1)/2,
+//                           reference: a_split = (na+1)/2,
 //                                b_split = binary_search(b, nb, a[a_split-1]) + 1   (low = -1, :40-50)
 //                           task cilkmerge(a,a_split, b,b_split, dst);
 //                           task cilkmerge(a+a_split,na-a_split, b+b_split,nb-b_split,
 //                                          dst+a_split+b_split); taskwait
+//                           here: up to 64 chunks of equal output length, their boundaries found by
+//                           merge-path searches run by the workgroup's threads in parallel, each
+//                           chunk a cilkmerge task; taskwait (the output is the same sorted run)
 //
 // Differences from the reference's thread-mode program: uint32 keys; the leaf is `n <= 4096`
 // keys sorted by a 256-thread workgroup (bitonic network in LDS) instead of `n < 64` keys by
@@ -33,6 +36,7 @@ namespace cilk {
 
 enum : uint32_t { F_SORT = 1, F_MERGE = 2 };
 constexpr int kTile = 4096;
+constexpr int kMaxFan = 64;  // merge chunks spawned by one large merge (<= kBlockMaxSpawn)
 constexpr uint32_t kT = kBlockThreads;
 
 struct SortArg {
@@ -60,7 +64,7 @@ struct Program {
     struct Shared {
         uint32_t keys[kTile];
         uint32_t out[kTile];
-        int64_t split[18];
+        int64_t split[kMaxFan + 2];
     };
     template <class B>
     __device__ __forceinline__ static void run(B& blk);
@@ -319,11 +323,37 @@ __device__ void merge_task(B& blk) {
         blk.finish();
         return;
     }
-    if (threadIdx.x == 0) {
-        const int64_t a_split = (na + 1) / 2;
-        const int64_t b_split = binary_search_gm(b, nb, ldk(a + a_split - 1)) + 1;
-        blk.spawn(F_MERGE, 0, MergeArg{a, b, dst, a_split, b_split});
-        blk.spawn(F_MERGE, 0, MergeArg{a + a_split, b + b_split, dst + a_split + b_split, na - a_split, nb - b_split});
+    // A large merge is cut into up to kMaxFan chunks of equal output length by merge-path
+    // searches run in parallel (one per chunk boundary), and the chunks are spawned as merge
+    // tasks: one search latency per level instead of one per halving.
+    {
+        typename Program::Shared* sh = blk.sh;
+        const int64_t total = na + nb;
+        int64_t fan = (total + g.merge_leaf - 1) / g.merge_leaf;
+        if (fan > kMaxFan) fan = kMaxFan;
+        if (fan < 2) fan = 2;
+        const int64_t chunk = (total + fan - 1) / fan;
+        auto ga = [a](int64_t i) { return ldk(a + i); };
+        auto gb = [b](int64_t i) { return ldk(b + i); };
+        const int64_t t = threadIdx.x;
+        if (t <= fan) {
+            int64_t d = t * chunk;
+            if (d > total) d = total;
+            sh->split[t] = merge_path(ga, na, gb, nb, d);
+        }
+        __syncthreads();
+        if (t < fan) {
+            int64_t d0 = t * chunk, d1 = d0 + chunk;
+            if (d0 > total) d0 = total;
+            if (d1 > total) d1 = total;
+            const int64_t i0 = sh->split[t], i1 = sh->split[t + 1];
+            const int64_t ca = i1 - i0, cb = (d1 - i1) - (d0 - i0);
+            if (ca < 0 || cb < 0 || i1 > na || d1 - i1 > nb) {
+                GTAP_DEVICE_ERROR(*blk.dev, GTAP_ERROR_INVALID_ARGUMENT, blk.tid, -1, (int)na, (int)nb);
+            } else if (ca + cb > 0) {
+                blk.spawn(F_MERGE, 0, MergeArg{a + i0, b + (d0 - i0), dst + d0, ca, cb});
+            }
+        }
     }
     if (blk.taskwait(1)) return;
     blk.finish();
