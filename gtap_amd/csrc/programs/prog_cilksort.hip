@@ -101,36 +101,80 @@ __device__ __forceinline__ int64_t binary_search_gm(const uint32_t* x, int64_t l
 // loads of a warp are contiguous, and all kPer loads are issued before any is used.
 constexpr int kPer = kTile / (int)kT;  // 16
 
-// Workgroup sort of arr[0..n), n <= 4096: bitonic network over 4096 LDS slots (padded with
-// 0xFFFFFFFF, which sorts to the end and is never written back).
+// Compare-exchange of a pair held by one thread: afterwards x <= y when `up`, else x >= y.
+__device__ __forceinline__ void cmpx(uint32_t& x, uint32_t& y, bool up) {
+    const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
+    x = up ? lo : hi;
+    y = up ? hi : lo;
+}
+
+// Workgroup sort of arr[0..n), n <= 4096: bitonic sort of 4096 slots (padded with 0xFFFFFFFF,
+// which sorts to the end and is never written back). Thread t owns slots 16t .. 16t+15 in
+// registers: every step whose partner distance j is below 16 runs in registers; the steps
+// with j >= 16 run over LDS, one barrier each (36 such steps instead of 78 LDS steps).
 template <class B>
 __device__ void leaf_sort(B& blk, uint32_t* arr, int64_t n) {
     uint32_t* k = blk.sh->keys;
     const uint32_t t = threadIdx.x;
-    uint32_t r[kPer];
+    {
+        uint32_t r[kPer];
 #pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-        const uint32_t e = (uint32_t)q * kT + t;
-        r[q] = (int64_t)e < n ? ldk(arr + e) : 0xFFFFFFFFu;
+        for (int q = 0; q < kPer; ++q) {
+            const uint32_t e = (uint32_t)q * kT + t;
+            r[q] = (int64_t)e < n ? ldk(arr + e) : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) k[(uint32_t)q * kT + t] = r[q];
     }
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) k[(uint32_t)q * kT + t] = r[q];
     __syncthreads();
-    for (uint32_t size = 2; size <= (uint32_t)kTile; size <<= 1) {
+    uint32_t x[16];
+    const uint32_t g0 = 16u * t;  // global slot of x[0]
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = k[g0 + (uint32_t)i];
+    // sizes 2..16: entirely inside each thread's 16 slots
+#pragma unroll
+    for (uint32_t size = 2; size <= 16; size <<= 1) {
+#pragma unroll
         for (uint32_t j = size >> 1; j > 0; j >>= 1) {
-            for (uint32_t p = t; p < (uint32_t)kTile / 2; p += kT) {
-                const uint32_t lo = 2 * j * (p / j) + (p % j);
+#pragma unroll
+            for (uint32_t i = 0; i < 16; ++i) {
+                if ((i & j) == 0) cmpx(x[i], x[i + j], ((g0 + i) & size) == 0);
+            }
+        }
+    }
+    for (uint32_t size = 32; size <= (uint32_t)kTile; size <<= 1) {
+        __syncthreads();  // every thread has read its slots of the previous size
+#pragma unroll
+        for (int i = 0; i < 16; ++i) k[g0 + (uint32_t)i] = x[i];
+        __syncthreads();
+        for (uint32_t j = size >> 1; j >= 16; j >>= 1) {
+#pragma unroll
+            for (uint32_t q = 0; q < (uint32_t)kTile / 2 / kT; ++q) {
+                const uint32_t p = q * kT + t;
+                const uint32_t lo = ((p & ~(j - 1)) << 1) | (p & (j - 1));
                 const uint32_t hi = lo + j;
-                const uint32_t x = k[lo], y = k[hi];
-                const bool up = (lo & size) == 0;
-                if ((x > y) == up) {
-                    k[lo] = y;
-                    k[hi] = x;
-                }
+                uint32_t a = k[lo], b = k[hi];
+                cmpx(a, b, (lo & size) == 0);
+                k[lo] = a;
+                k[hi] = b;
             }
             __syncthreads();
         }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = k[g0 + (uint32_t)i];
+        const bool up = (g0 & size) == 0;
+#pragma unroll
+        for (uint32_t j = 8; j > 0; j >>= 1) {
+#pragma unroll
+            for (uint32_t i = 0; i < 16; ++i) {
+                if ((i & j) == 0) cmpx(x[i], x[i + j], up);
+            }
+        }
     }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) k[g0 + (uint32_t)i] = x[i];
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
         const uint32_t e = (uint32_t)q * kT + t;
