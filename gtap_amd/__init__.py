@@ -59,7 +59,6 @@ class RunStats(ctypes.Structure):
         ("published", ctypes.c_uint64),
         ("overflowed", ctypes.c_uint64),
         ("suspended", ctypes.c_uint64),
-        ("cached", ctypes.c_uint64),
         ("workers", ctypes.c_uint32),
         ("blocks", ctypes.c_uint32),
         ("error_code", ctypes.c_int32),

@@ -49,7 +49,7 @@ enum HostCtlIndex : int {
 // Per-wave statistics slots (uint64 each, 16 per wave).
 enum StatIndex : int {
     ST_TASKS = 0, ST_SPAWNS, ST_ROUNDS, ST_POPS, ST_STEALS, ST_STEAL_ATTEMPTS, ST_JOBS, ST_REFILLS,
-    ST_PUBLISHED, ST_OVERFLOWED, ST_SUSPENDED, ST_CACHED,
+    ST_PUBLISHED, ST_OVERFLOWED, ST_SUSPENDED,
     ST_COUNT = 16
 };
 

@@ -68,34 +68,7 @@ struct HasWaveQueue<P, decltype((void)P::kWaveQueue)> {
 // The wave queue (run on a whole wave) keeps its records in HBM: QL = queues with
 // descriptors. A descriptor is { word0, parent, result offset, frame[0..FW) } padded to an
 // odd word count (LDS bank spread).
-// Join cache (single-queue programs). A task that suspends in a round whose children all stay
-// in the wave's kept batch keeps its frame and join counter here instead of in HBM: in the
-// next round those children run on this wave, write their results into the cached frame and
-// count down the cached counter with LDS atomics. When they all finish, the parent resumes from
-// LDS in the round after; if any of them suspends, the parent is written out to HBM (its
-// remaining children may finish on other waves) and continues as an ordinary record.
-constexpr int kJoinCache = 32;                 // entries, direct-mapped by task id
-constexpr uint32_t kCacheFlag = 0x80000000u;   // in a descriptor's control word: frame in the join cache
-constexpr uint32_t kNoId = 0xFFFFFFFFu;
-enum : uint32_t { JC_FREE = 0, JC_WAIT = 1, JC_READY = 2, JC_SPILL = 3 };
-
-template <int FB>  // frame words (a multiple of 4), 0 = no join cache
-struct JoinCache {
-    uint32_t id[kJoinCache];
-    uint32_t state[kJoinCache];
-    uint32_t cnt[kJoinCache];
-    uint32_t parent[kJoinCache];
-    uint32_t res_off[kJoinCache];
-    uint32_t ctl[kJoinCache];
-    uint32_t frame[kJoinCache][FB + 1];
-    uint32_t list[kJoinCache];
-};
-template <>
-struct JoinCache<0> {
-    uint32_t unused;
-};
-
-template <int Q, int QL, int D, int S, int FB>
+template <int Q, int QL, int D, int S>
 struct WaveCtx {
     uint32_t free_top;
     uint32_t priv_top;
@@ -117,7 +90,6 @@ struct WaveCtx {
     uint32_t stage[2 * kWave][S];
     uint32_t stage_id[2 * kWave];
     uint32_t nstage;        // records staged so far this round (entries pushed out of the window, <= 64)
-    JoinCache<FB> jc;
 };
 
 template <class P>
@@ -132,10 +104,7 @@ struct CtxShape {
     static_assert(kFWr <= 12, "at most 12 frame words are loaded with a task (one 64-byte line)");
     static_assert(P::kFrameBytes >= 4 * kFWr, "the frame must cover the loaded frame words");
     static constexpr int kD = (3 + kFW) | 1;
-    static constexpr bool kLocalJoin = kQ == 1 && kWQ < 0;
-    static_assert(P::kFrameBytes % 4 == 0, "frames are whole words");
-    static constexpr int kFB = kLocalJoin ? ((int)(P::kFrameBytes / 4) + 3) & ~3 : 0;  // cached frame words
-    using type = WaveCtx<kQ, (kQL > 0 ? kQL : 1), kD, 4 * kPieces, kFB>;
+    using type = WaveCtx<kQ, (kQL > 0 ? kQL : 1), kD, 4 * kPieces>;
 };
 template <class P>
 using WaveCtxOf = typename CtxShape<P>::type;
@@ -336,8 +305,6 @@ __device__ __forceinline__ void record_from_desc(const GtapDev& dev, __amdgpu_bu
                                                  const uint32_t* d) {
     constexpr int FW = CtxShape<P>::kFW;
     const uint32_t off = id * dev.stride;
-    if (vld(&d[0]) & kCacheFlag)  // a parent resumed from the join cache never leaves its wave
-        GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, id, -1, 0, 2);
     uint32_t f[FW];
 #pragma unroll
     for (int i = 0; i < FW; ++i) f[i] = vld(&d[3 + i]);
@@ -504,47 +471,6 @@ __device__ __forceinline__ void write_staged(const GtapDev& dev, __amdgpu_buffer
     }
 }
 
-// Writes out the join-cache entries marked JC_SPILL as ordinary records (16-byte header with
-// the remaining join count, then the whole frame) and frees them. Piece k of the e-th entry
-// goes out from lane e * pieces + k. All lanes of the wave call it.
-template <class P, class W>
-__device__ __forceinline__ void spill_cache(const GtapDev& dev, __amdgpu_buffer_rsrc_t rs, W* w) {
-    if constexpr (CtxShape<P>::kLocalJoin) {
-        constexpr uint32_t FB = CtxShape<P>::kFB;
-        constexpr uint32_t PCS = 1 + FB / 4;  // pieces per record
-        const uint32_t lane = (uint32_t)lane_id();
-        const bool sp = lane < (uint32_t)kJoinCache && vld(&w->jc.state[lane < (uint32_t)kJoinCache ? lane : 0]) == JC_SPILL;
-        const uint64_t m = ballot(sp);
-        if (m == 0) return;
-        if (sp) vst(&w->jc.list[rank_in(m)], lane);
-        wave_sync();
-        const uint32_t total = (uint32_t)popc(m) * PCS;
-        for (uint32_t base = 0; base < total; base += kWave) {
-            const uint32_t item = base + lane;
-            if (item < total) {
-                const uint32_t sl = vld(&w->jc.list[item / PCS]) & (uint32_t)(kJoinCache - 1);
-                const uint32_t k = item % PCS;
-                const uint32_t id = vld(&w->jc.id[sl]);
-                u32x4 v;
-                if (k == 0) {
-                    v = u32x4{vld(&w->jc.parent[sl]), vld(&w->jc.res_off[sl]), vld(&w->jc.cnt[sl]), vld(&w->jc.ctl[sl])};
-                } else {
-                    const uint32_t* f = &w->jc.frame[sl][4 * (k - 1)];
-                    v = u32x4{vld(f), vld(f + 1), vld(f + 2), vld(f + 3)};
-                }
-                if (id < dev.nrecs) st128(rs, id * dev.stride + 16u * k, v);
-            }
-        }
-        wave_sync();
-        if (sp) {
-            vst(&w->jc.id[lane], kNoId);
-            vst(&w->jc.state[lane], (uint32_t)JC_FREE);
-        }
-    } else {
-        (void)dev; (void)rs; (void)w;
-    }
-}
-
 // ------------------------------------------------------------------ per-lane task context
 
 template <class P>
@@ -567,27 +493,7 @@ struct Lane {
     uint32_t join_count;    // children to wait for, after a taskwait that suspends
     uint32_t outcome;
     bool lds;               // the record's header and first FW frame words are not in HBM yet
-    int32_t cslot;          // join-cache slot holding this task's whole frame (-1: none)
     uint32_t frame[FW];     // the first frame words, loaded when the round starts
-    static constexpr bool kLJ = CtxShape<P>::kLocalJoin;
-    static constexpr int kFB = CtxShape<P>::kFB;
-
-    // Join-cache slot of a parent whose frame is cached (a child of this round), or -1.
-    __device__ __forceinline__ int cached_parent(uint32_t par) const {
-        if constexpr (kLJ) {
-            const uint32_t sl = par & (uint32_t)(kJoinCache - 1);
-            if (par != kNoParent && vld(&w->jc.id[sl]) == par && vld(&w->jc.state[sl]) == JC_WAIT) return (int)sl;
-        }
-        return -1;
-    }
-    // Word `i` of this task's cached frame (the caller checked cslot >= 0).
-    __device__ __forceinline__ uint32_t* cframe(uint32_t i) const {
-        if constexpr (kLJ) {
-            return &w->jc.frame[cslot & (kJoinCache - 1)][i < (uint32_t)kFB ? i : 0];
-        } else {
-            return nullptr;
-        }
-    }
 
     // -- reads of the loaded frame (arguments written by the spawner)
     template <typename T>
@@ -613,16 +519,6 @@ struct Lane {
     // -- coherent access to any part of this task's frame (offsets relative to the frame)
     template <typename T>
     __device__ __forceinline__ T frame_ld(int byte_off) {
-        if constexpr (kLJ) {
-            if (cslot >= 0) {
-                const uint32_t i = (uint32_t)byte_off / 4;
-                if constexpr (sizeof(T) == 4) {
-                    return __builtin_bit_cast(T, vld(cframe(i)));
-                } else {
-                    return __builtin_bit_cast(T, (uint64_t)vld(cframe(i)) | ((uint64_t)vld(cframe(i + 1)) << 32));
-                }
-            }
-        }
         if (lds) {
             materialize_frame();
             complete_stores();
@@ -643,47 +539,16 @@ struct Lane {
                 frame[byte_off / 4] = (uint32_t)u;
                 frame[byte_off / 4 + 1] = (uint32_t)(u >> 32);
             }
-            if constexpr (kLJ) {
-                if (cslot >= 0) {
-                    cframe_st(byte_off, v);
-                    return;
-                }
-            }
             if (!lds) stT<T>(rec + kHdrBytes + byte_off, v);
             return;
         }
-        if constexpr (kLJ) {
-            if (cslot >= 0) {
-                cframe_st(byte_off, v);
-                return;
-            }
-        }
         if (lds) materialize_frame();
         stT<T>(rec + kHdrBytes + byte_off, v);
-    }
-    template <typename T>
-    __device__ __forceinline__ void cframe_st(int byte_off, T v) {
-        const uint32_t i = (uint32_t)byte_off / 4;
-        if constexpr (sizeof(T) == 4) {
-            vst(cframe(i), __builtin_bit_cast(uint32_t, v));
-        } else {
-            const uint64_t u = __builtin_bit_cast(uint64_t, v);
-            vst(cframe(i), (uint32_t)u);
-            vst(cframe(i + 1), (uint32_t)(u >> 32));
-        }
     }
     // Two adjacent 8-byte fields at a 16-byte aligned frame offset, with one 16-byte load.
     template <typename T>
     __device__ __forceinline__ void frame_ld2(int byte_off, T& a, T& b) {
         static_assert(sizeof(T) == 8, "8-byte fields");
-        if constexpr (kLJ) {
-            if (cslot >= 0) {
-                const uint32_t i = (uint32_t)byte_off / 4;
-                a = __builtin_bit_cast(T, (uint64_t)vld(cframe(i)) | ((uint64_t)vld(cframe(i + 1)) << 32));
-                b = __builtin_bit_cast(T, (uint64_t)vld(cframe(i + 2)) | ((uint64_t)vld(cframe(i + 3)) << 32));
-                return;
-            }
-        }
         if (lds) {
             materialize_frame();
             complete_stores();
@@ -739,27 +604,7 @@ struct Lane {
     // return value: written into the parent's result slot
     template <typename T>
     __device__ __forceinline__ void ret(T v) {
-        if (parent != kNoParent) {
-            const int ps = cached_parent(parent);
-            if (ps >= 0) {
-                // the parent's frame is in this wave's join cache
-                const uint32_t i = (res_off - kHdrBytes) / 4;
-                if (res_off < kHdrBytes || i + sizeof(T) / 4 > (uint32_t)kFB) {
-                    GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_ARGUMENT, tid, -1, (int)res_off, kFB * 4);
-                } else if constexpr (kLJ) {
-                    uint32_t* f = &w->jc.frame[ps][0];
-                    if constexpr (sizeof(T) == 4) {
-                        vst(f + i, __builtin_bit_cast(uint32_t, v));
-                    } else {
-                        const uint64_t u = __builtin_bit_cast(uint64_t, v);
-                        vst(f + i, (uint32_t)u);
-                        vst(f + i + 1, (uint32_t)(u >> 32));
-                    }
-                }
-            } else {
-                stT<T>(rec_ptr(*dev, parent) + res_off, v);
-            }
-        }
+        if (parent != kNoParent) stT<T>(rec_ptr(*dev, parent) + res_off, v);
         finish();
     }
     __device__ __forceinline__ void finish() {
@@ -890,7 +735,7 @@ __device__ __forceinline__ uint32_t run_wave_task(const GtapDev& dev, const type
 // ------------------------------------------------------------------ scheduler
 
 struct LoopStats {
-    uint64_t tasks, rounds, pops, steals, attempts, jobs, published, suspended, cached;
+    uint64_t tasks, rounds, pops, steals, attempts, jobs, published, suspended;
 };
 
 template <class P>
@@ -908,7 +753,6 @@ __device__ __forceinline__ void load_task(Lane<P>& L, uint32_t id) {
     L.child_count = 0;
     L.outcome = OUT_RUNNING;
     L.lds = false;
-    L.cslot = -1;
     frame_words_ld<CtxShape<P>::kFWr>(L.rs, off, L.frame);
 }
 
@@ -926,18 +770,6 @@ __device__ __forceinline__ void load_task_lds(Lane<P>& L, uint32_t id, const uin
     L.child_count = 0;
     L.outcome = OUT_RUNNING;
     L.lds = true;
-    L.cslot = -1;
-    if constexpr (CtxShape<P>::kLocalJoin) {
-        if (w0 & kCacheFlag) {  // a parent resumed from the join cache: its frame is there
-            const uint32_t sl = id & (uint32_t)(kJoinCache - 1);
-            if (vld(&L.w->jc.id[sl]) == id && vld(&L.w->jc.state[sl]) == JC_READY) {
-                L.cslot = (int32_t)sl;
-            } else {
-                GTAP_DEVICE_ERROR(*L.dev, GTAP_ERROR_INVALID_TASKWAIT, id, -1, (int)vld(&L.w->jc.state[sl]), JC_READY);
-                L.func = 0;
-            }
-        }
-    }
 #pragma unroll
     for (int i = 0; i < P::kFrameWords; ++i) L.frame[i] = vld(&d[3 + i]);
 #pragma unroll
@@ -1148,12 +980,6 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             vst(&w->inlds[q], (uint64_t)0);
         }
     }
-    if constexpr (CtxShape<P>::kLocalJoin) {
-        if ((uint32_t)lane < (uint32_t)kJoinCache) {
-            vst(&w->jc.id[lane], kNoId);
-            vst(&w->jc.state[lane], (uint32_t)JC_FREE);
-        }
-    }
     __builtin_amdgcn_wave_barrier();
 
     Lane<P> L;
@@ -1162,10 +988,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     L.args = &args;
     L.me = me;
     L.rs = pool_rsrc(dev.recs);
-    L.cslot = -1;
     constexpr int QL = CtxShape<P>::kQL;
-    constexpr bool LJ = CtxShape<P>::kLocalJoin;
-    uint32_t ovf0 = 0;  // pushed-out count at the start of the round
 
     uint32_t rng = 0x9E3779B9u ^ (me * 0x85EBCA6Bu) ^ ((uint32_t)lane * 0xC2B2AE35u) ^ (uint32_t)t_begin;
     if (rng == 0) rng = 1;
@@ -1177,7 +1000,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     bool idle_registered = false;
     uint32_t idle_iter = 0;
     uint32_t busy_iter = 0;
-    LoopStats st = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    LoopStats st = {0, 0, 0, 0, 0, 0, 0, 0};
     // timeline profiling (lane 0, when dev.prof_max > 0)
     uint32_t ph_idx = 0, pw_idx = 0;
     bool had_tasks = false;
@@ -1261,7 +1084,6 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         st.rounds += 1;
         st.tasks += n;
         const uint32_t round_n = n;
-        if constexpr (LJ) ovf0 = uniform(vld(&w->overflowed));
         if (dev.prof_max && lane == 0) {
             if (!had_tasks) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, ph_idx, 0);
             prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, (int32_t)round_n);
@@ -1312,68 +1134,53 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             }
         }
 
-        // ---- epilogue: joins, continuations, id release, next batch, suspended records, publish
-        complete_stores();  // results written to HBM parents in this round
+        // ---- epilogue: suspended records, joins, continuations, id release, next batch, publish
+        {
+            // records staged during the round (entries pushed out of the window), then the
+            // tasks that suspended (up to 64 more), all written together
+            const uint64_t sm = ballot(sus);
+            const uint32_t base = uniform(vld(&w->nstage));
+            if (sm) {
+                const uint32_t r = base + (uint32_t)rank_in(sm);
+                if (sus) {
+                    const u32x4 h = L.suspended_header();
+                    uint32_t* dst = &w->stage[r][0];
+                    dst[0] = h.x;
+                    dst[1] = h.y;
+                    dst[2] = h.z;
+                    dst[3] = h.w;
+#pragma unroll
+                    for (int i = 0; i < CtxShape<P>::kFWr; ++i) dst[4 + i] = L.frame[i];
+                    w->stage_id[r] = L.tid | (L.lds ? 0u : kHeaderOnly);  // frame already in HBM
+                }
+                st.suspended += (uint32_t)popc(sm);
+            }
+            const uint32_t total = base + (uint32_t)popc(sm);
+            if (total) {
+                wave_sync();
+                write_staged<P>(dev, L.rs, w, total);
+                wave_sync();
+                if (lane == 0) vst(&w->nstage, 0u);
+            }
+        }
+        complete_stores();  // results, suspended records and child records of this round
         uint32_t ready = kNoParent, ready_ctl = 0;
-        int ready_cached = -1;  // join-cache slot of a parent made ready here
         if (fin && fin_parent != kNoParent) {
-            const int ps = L.cached_parent(fin_parent);
-            if (ps >= 0) {
-                if constexpr (LJ) {
-                    const uint32_t old = atomicSub(&w->jc.cnt[ps], 1u);
-                    if (old == 1u) {
-                        ready = fin_parent;
-                        ready_cached = ps;
-                    } else if (old == 0u) {
-                        GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, fin_tid, -1, 0, 1);
-                    }
-                }
-            } else {
-                // {join, control} of the parent: the last child also reads the resume queue
-                const uint64_t old = atomic_sub(reinterpret_cast<uint64_t*>(rec_ptr(dev, fin_parent) + kHdrJoin), (uint64_t)1);
-                if ((uint32_t)old == 1u) {
-                    ready = fin_parent;
-                    ready_ctl = (uint32_t)(old >> 32);
-                } else if ((uint32_t)old == 0u) {
-                    GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, fin_tid, -1, 0, 0);
-                }
+            // {join, control} of the parent: the last child also reads the resume queue
+            const uint64_t old = atomic_sub(reinterpret_cast<uint64_t*>(rec_ptr(dev, fin_parent) + kHdrJoin), (uint64_t)1);
+            if ((uint32_t)old == 1u) {
+                ready = fin_parent;
+                ready_ctl = (uint32_t)(old >> 32);
+            } else if ((uint32_t)old == 0u) {
+                GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, fin_tid, -1, 0, 0);
             }
         }
         if (ballot(ready != kNoParent)) {
             if (ready != kNoParent && ready >= dev.nrecs) complete_job<P>(dev, ready);
-            if (ready != kNoParent && ready < dev.nrecs && ready_cached < 0)
+            if (ready != kNoParent && ready < dev.nrecs)
                 queue_append<P, false>(dev, w, me, hdr_queue(ready_ctl) < (uint32_t)Q ? hdr_queue(ready_ctl) : 0u, ready, nullptr);
-            if constexpr (LJ) {
-                if (ballot(ready_cached >= 0)) {
-                    // resumes from the join cache: its descriptor points there (kCacheFlag)
-                    uint32_t d[CtxShape<P>::kD];
-                    if (ready_cached >= 0) {
-                        const int sl = ready_cached;
-                        d[0] = vld(&w->jc.ctl[sl]) | kCacheFlag;
-                        d[1] = vld(&w->jc.parent[sl]);
-                        d[2] = vld(&w->jc.res_off[sl]);
-#pragma unroll
-                        for (int i = 0; i < P::kFrameWords; ++i) d[3 + i] = vld(&w->jc.frame[sl][i]);
-#pragma unroll
-                        for (int i = 3 + P::kFrameWords; i < CtxShape<P>::kD; ++i) d[i] = 0u;
-                        vst(&w->jc.state[sl], (uint32_t)JC_READY);
-                        queue_append<P, true>(dev, w, me, 0u, ready, d);
-                    }
-                }
-            }
         }
         if (fin && fin_tid != 0u) free_id(dev, w, me, fin_tid);  // id 0 is the sink record
-        if constexpr (LJ) {
-            // a task resumed from the join cache and finished: its slot is free again
-            if (fin && L.cslot >= 0 && (uint32_t)lane < round_n) {
-                vst(&w->jc.id[L.cslot], kNoId);
-                vst(&w->jc.state[L.cslot], (uint32_t)JC_FREE);
-            }
-            // parents cached last round whose children did not all finish: written out below
-            if ((uint32_t)lane < (uint32_t)kJoinCache) {
-                if (vld(&w->jc.state[lane]) == JC_WAIT) vst(&w->jc.state[lane], (uint32_t)JC_SPILL);
-            }
-        }
 
         // choose the busiest queue, keep its newest <= 64 tasks, publish the rest
         uint32_t kmax = 0, cmax = 0;
@@ -1406,66 +1213,15 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 vst(&w->cur_queue, kmax);
             }
         }
-        // Tasks that suspended this round. When every child of this round stays in the kept
-        // batch (none was pushed out of the window), a suspended task goes to the join cache;
-        // otherwise its record is staged with the records pushed out of the window, and all
-        // are written together.
+        // records pushed out of the window by the continuations appended above
         {
-            bool cached = false;
-            if constexpr (LJ) {
-                const bool local_ok = uniform(vld(&w->overflowed)) == ovf0;
-                if (sus && local_ok) {
-                    if (L.cslot >= 0) {
-                        cached = true;  // resumed from the cache: keeps its slot and frame
-                    } else if (L.lds) {
-                        const uint32_t sl = L.tid & (uint32_t)(kJoinCache - 1);
-                        if (atomicCAS(&w->jc.id[sl], kNoId, L.tid) == kNoId) {
-                            L.cslot = (int32_t)sl;
-                            cached = true;
-#pragma unroll
-                            for (int i = 0; i < CtxShape<P>::kFWr; ++i) vst(&w->jc.frame[sl][i], L.frame[i]);
-                        }
-                    }
-                }
-                if (sus && L.cslot >= 0) {
-                    const int sl = L.cslot;
-                    vst(&w->jc.cnt[sl], L.join_count);
-                    vst(&w->jc.parent[sl], L.parent);
-                    vst(&w->jc.res_off[sl], L.res_off);
-                    vst(&w->jc.ctl[sl], hdr_word0(L.func, L.state, L.queue));
-                    // cached: waits for its children; else its whole frame is written out below
-                    vst(&w->jc.state[sl], cached ? (uint32_t)JC_WAIT : (uint32_t)JC_SPILL);
-                }
-                const uint64_t cm = ballot(sus && cached);
-                if (cm) st.cached += (uint32_t)popc(cm);
-            }
-            const bool stg = sus && !cached && !(LJ && L.cslot >= 0);
-            const uint64_t sm = ballot(stg);
-            const uint32_t base = uniform(vld(&w->nstage));
-            if (sm) {
-                const uint32_t r = base + (uint32_t)rank_in(sm);
-                if (stg) {
-                    const u32x4 h = L.suspended_header();
-                    uint32_t* dst = &w->stage[r][0];
-                    dst[0] = h.x;
-                    dst[1] = h.y;
-                    dst[2] = h.z;
-                    dst[3] = h.w;
-#pragma unroll
-                    for (int i = 0; i < CtxShape<P>::kFWr; ++i) dst[4 + i] = L.frame[i];
-                    w->stage_id[r] = L.tid | (L.lds ? 0u : kHeaderOnly);  // frame already in HBM
-                }
-            }
-            const uint64_t all_sus = ballot(sus);
-            if (all_sus) st.suspended += (uint32_t)popc(all_sus);
-            const uint32_t total = base + (uint32_t)popc(sm);
-            if (total) {
+            const uint32_t ns = uniform(vld(&w->nstage));
+            if (ns) {
                 wave_sync();
-                write_staged<P>(dev, L.rs, w, total);
+                write_staged<P>(dev, L.rs, w, ns);
                 wave_sync();
                 if (lane == 0) vst(&w->nstage, 0u);
             }
-            if constexpr (LJ) spill_cache<P>(dev, L.rs, w);
         }
         // write out what is published: entries still in the 64-slot window (lane i takes
         // slot i) get their deque slot, and their record if it is held in LDS
@@ -1521,8 +1277,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 }
             }
         }
-        complete_stores();  // records written above (suspended, pushed out, spilled, published)
         if (any_publish) {
+            complete_stores();  // records and slots of everything published below
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const uint32_t b = uniform(vld(&w->bottom[q]));
@@ -1570,7 +1326,6 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         s[ST_PUBLISHED] = st.published;
         s[ST_OVERFLOWED] = vld(&w->overflowed);
         s[ST_SUSPENDED] = st.suspended;
-        s[ST_CACHED] = st.cached;
     }
 }
 

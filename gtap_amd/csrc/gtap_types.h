@@ -86,7 +86,6 @@ typedef struct gtap_run_stats {
     uint64_t published;       /* task records written out when published (thread mode) */
     uint64_t overflowed;      /* task records written out when pushed out of the LDS window */
     uint64_t suspended;       /* taskwaits that suspended a task */
-    uint64_t cached;          /* of those, parents kept in a wave's join cache (frame in LDS) */
     uint32_t workers;         /* wavefront workers in the launch */
     uint32_t blocks;          /* workgroups in the launch */
     int32_t error_code;

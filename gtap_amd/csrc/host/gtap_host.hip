@@ -181,7 +181,6 @@ int finish_launch(const GtapDev& dev, uint32_t blocks, gtap_run_stats* stats) {
         s.published += p[ST_PUBLISHED];
         s.overflowed += p[ST_OVERFLOWED];
         s.suspended += p[ST_SUSPENDED];
-        s.cached += p[ST_CACHED];
     }
     s.workers = dev.nworkers;
     s.blocks = blocks;

@@ -50,4 +50,4 @@ def test_error_strings_without_gpu():
 def test_struct_sizes_match_c_layout():
     assert ctypes.sizeof(gtap_amd.Job) == 128
     assert ctypes.sizeof(gtap_amd.ErrorReport) == 36
-    assert ctypes.sizeof(gtap_amd.RunStats) == 8 * 13 + 16
+    assert ctypes.sizeof(gtap_amd.RunStats) == 8 * 12 + 16
