@@ -81,7 +81,8 @@ class Config(ctypes.Structure):
         ("max_jobs", ctypes.c_uint32),
         ("timeout_ms", ctypes.c_uint32),
         ("block_workers", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 6),
+        ("idle_sleep", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 5),
     ]
 
 
@@ -160,6 +161,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_stream_sorted_keys": ([_vp, ctypes.c_int32, _vp], _i),
     "gtap_stream_run_batched": ([_vp, _vp, _vp, ctypes.POINTER(RunStats)], _i),
     "gtap_stream_run_per_job": ([_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)], _i),
+    "gtap_stream_exec_ms": ([_vp, _vp], _i),
     "gtap_stream_serve": ([_vp, ctypes.c_double, _u64, ctypes.c_int32, ctypes.c_int32, _vp, _vp,
                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(RunStats)], _i),
     "gtap_rmat_edges": ([_i, _i64, _u64, _u32, _u32, _u32, _vp, _vp], _i),
@@ -441,6 +443,13 @@ class Stream:
         _check(lib().gtap_stream_run_per_job(self.h, res.ctypes.data, lat.ctypes.data, ctypes.byref(tot)),
                "gtap_stream_run_per_job")
         return res, lat, tot.value
+
+    def exec_ms(self):
+        """Device time from admission to completion of each job of the last run, in ms."""
+        import numpy as np
+        out = np.empty(self.njobs, dtype=np.float64)
+        _check(lib().gtap_stream_exec_ms(self.h, out.ctypes.data), "gtap_stream_exec_ms")
+        return out
 
     def serve(self, rate: float, arrival_seed: int = 1, per_job_launch: bool = False, max_live: int = 0):
         import numpy as np

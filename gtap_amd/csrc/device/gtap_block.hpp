@@ -282,10 +282,7 @@ __device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me,
 template <class P>
 __device__ __forceinline__ uint32_t block_admit(const GtapDev& dev, BlockCtx* c, bool& idle_registered) {
     uint32_t avail = dev.njobs;
-    if (dev.term_mode == GTAP_TERM_SERVE) {
-        host_poll(dev.ctl, dev.host_ctl);
-        avail = (uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32);
-    }
+    if (dev.term_mode == GTAP_TERM_SERVE) avail = (uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32);
     int32_t* nextp = dev.ctl + CTL_JOB_NEXT * 32;
     const uint32_t next = (uint32_t)ld32(nextp);
     if (next >= avail) return 0;
@@ -357,7 +354,11 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
     uint32_t ph_idx = 0, pw_idx = 0;  // timeline profiling (thread 0)
     bool had_task = false;
 
+    uint64_t host_poll_last = 0;
+    const bool host_poller = dev.term_mode == GTAP_TERM_SERVE && me == 0 && t == 0;
+
     while (true) {
+        if (host_poller) host_poll(dev.ctl, dev.host_ctl, host_poll_last);
         // keep enough ids in LDS for one invocation's spawns (collective)
         while (c->free_top < kBlockMaxSpawn) block_refill_chunk(dev, c, me);
         if (t == 0 && c->cur == 0) {
@@ -387,7 +388,10 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
                     flags |= 2;
                 }
                 if (flags) c->exit = 1;
-                else backoff_sleep(idle_iter < 8 ? 0 : (idle_iter < 64 ? 2 : (idle_iter < 256 ? 4 : 6)));
+                else {
+                    const uint32_t lvl = idle_iter < 8 ? 0u : (idle_iter < 64 ? 2u : (idle_iter < 256 ? 4u : 6u));
+                    backoff_sleep(lvl < dev.idle_level ? lvl : dev.idle_level);
+                }
             } else {
                 idle_iter = 0;
                 c->cur = id;

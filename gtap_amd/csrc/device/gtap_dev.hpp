@@ -35,7 +35,6 @@ enum CtlIndex : int {
     CTL_LIVE_JOBS = 6,  // admitted and not yet completed
     CTL_HOST_AVAIL = 7,   // serve mode: device mirror of HCTL_JOB_AVAIL
     CTL_HOST_STOP = 8,    // serve mode: device mirror of HCTL_STOP
-    CTL_HOST_POLL_T = 9,  // serve mode: wall-clock tick of the last host poll (64-bit)
     CTL_COUNT = 12
 };
 
@@ -87,6 +86,7 @@ struct GtapDev {
     uint32_t admit_reserve;    // free ids required per admitted job
     uint32_t max_live_jobs;    // admission cap on concurrently running jobs
     uint32_t prof_max;         // samples per worker; 0 = profiling off
+    uint32_t idle_level;       // deepest idle backoff level (1..6: s_sleep 1 .. 32)
 };
 
 // Timeline sample (profiling on): ns since an arbitrary epoch, from the 100 MHz wall clock.
@@ -324,23 +324,26 @@ __device__ __forceinline__ void vst(T* p, T v) { *reinterpret_cast<volatile T*>(
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
 
 // Serve mode. The host's control words sit in pinned host memory; a system-scope read of
-// them crosses the host bus and takes microseconds. Thousands of idle waves reading them on
-// every idle iteration would queue on that bus and stall the idle waves that should be
-// stealing, so at most one caller per kHostPollTicks (claimed by a CAS on the last poll
-// time) reads them and mirrors them into device control words, which every wave reads.
-// Call from one lane.
+// them crosses the host bus and takes microseconds. One worker (worker 0, one lane) reads
+// them, at most once per kHostPollTicks, at the top of each of its scheduler iterations,
+// busy or idle, and mirrors them into device control words that the other waves read.
+// A shared "last poll" word claimed by CAS would be read (and CAS'd) by every idle wave on
+// every iteration: thousands of waves on one line slow every access mapped to its memory
+// channel, including the running jobs' own.
 constexpr uint64_t kHostPollTicks = 100;  // 1 us of the 100 MHz wall clock
-__device__ __forceinline__ void host_poll(int32_t* ctl, const int32_t* host_ctl) {
-    uint64_t* lastp = reinterpret_cast<uint64_t*>(ctl + CTL_HOST_POLL_T * 32);
+__device__ __forceinline__ void host_poll(int32_t* ctl, const int32_t* host_ctl, uint64_t& last) {
     const uint64_t now = now_ticks();
-    uint64_t last = ld64(lastp);
     if (now - last < kHostPollTicks) return;
-    if (!atomic_cas(lastp, last, now)) return;
+    last = now;
     const int32_t stop = ld32_sys(host_ctl + HCTL_STOP * 32);
     complete_stores();  // the stop flag is read before the job count (the host writes them in the other order)
     const int32_t avail = ld32_sys(host_ctl + HCTL_JOB_AVAIL * 32);
-    __hip_atomic_fetch_max(ctl + CTL_HOST_AVAIL * 32, avail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (stop) st32(ctl + CTL_HOST_STOP * 32, 1);
+    // one writer and a non-decreasing host count: a plain store keeps the mirror monotonic
+    st32(ctl + CTL_HOST_AVAIL * 32, avail);
+    if (stop) {
+        complete_stores();  // a wave that sees the stop flag sees the final job count
+        st32(ctl + CTL_HOST_STOP * 32, 1);
+    }
 }
 
 __device__ __forceinline__ uint32_t xorshift32(uint32_t& s) {

@@ -783,10 +783,7 @@ __device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtxOf<P>*
                                               uint32_t& tid_out, bool& idle_registered) {
     const int lane = lane_id();
     uint32_t avail = dev.njobs;
-    if (dev.term_mode == GTAP_TERM_SERVE) {
-        if (lane == 0) host_poll(dev.ctl, dev.host_ctl);
-        avail = uniform((uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32));
-    }
+    if (dev.term_mode == GTAP_TERM_SERVE) avail = uniform((uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32));
     int32_t* nextp = dev.ctl + CTL_JOB_NEXT * 32;
     uint32_t next = uniform((uint32_t)ld32(nextp));
     if (next >= avail) return 0;
@@ -1005,7 +1002,11 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     uint32_t ph_idx = 0, pw_idx = 0;
     bool had_tasks = false;
 
+    uint64_t host_poll_last = 0;
+    const bool host_poller = dev.term_mode == GTAP_TERM_SERVE && me == 0;
+
     while (true) {
+        if (host_poller && lane == 0) host_poll(dev.ctl, dev.host_ctl, host_poll_last);
         if (n == 0) {
             // ---- acquire work: own deque, then the job table, then steal (records in HBM)
             my_lds = false;
@@ -1054,7 +1055,6 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 if (lane == 0) {
                     flags = ld32(dev.ctl + CTL_DONE * 32) | (ld32(dev.ctl + CTL_ABORT * 32) << 1);
                     if (dev.term_mode == GTAP_TERM_SERVE && (idle_iter & 15u) == 0) {
-                        host_poll(dev.ctl, dev.host_ctl);
                         const int32_t stop = ld32(dev.ctl + CTL_HOST_STOP * 32);
                         if (stop) {
                             const uint32_t avail = (uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32);
@@ -1074,7 +1074,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 }
                 flags = shfl(flags, 0);
                 if (flags) break;
-                backoff_sleep(idle_iter < 8 ? 0 : (idle_iter < 64 ? 2 : (idle_iter < 256 ? 4 : 6)));
+                const uint32_t lvl = idle_iter < 8 ? 0u : (idle_iter < 64 ? 2u : (idle_iter < 256 ? 4u : 6u));
+                backoff_sleep(lvl < dev.idle_level ? lvl : dev.idle_level);
                 continue;
             }
             idle_iter = 0;

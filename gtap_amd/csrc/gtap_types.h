@@ -105,7 +105,9 @@ typedef struct gtap_config {
     uint32_t max_jobs;        /* job table / anchor capacity (default 65536) */
     uint32_t timeout_ms;      /* device watchdog (default 20000) */
     uint32_t block_workers;   /* workgroups of the block-mode scheduler (default: co-resident) */
-    uint32_t reserved[6];
+    uint32_t idle_sleep;      /* longest backoff of an idle wave between polls, in units of 64
+                                 cycles: 1, 2, 4, 8, 16 or 32 (default 32) */
+    uint32_t reserved[5];
 } gtap_config;
 
 #ifdef __cplusplus

@@ -104,6 +104,8 @@ GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job
     d.admit_reserve = 0;
     d.max_live_jobs = R.nanchors;
     d.prof_max = R.prof_max;
+    d.idle_level = 1;
+    for (uint32_t v = R.cfg.idle_sleep; v > 1 && d.idle_level < 6; v >>= 1) d.idle_level += 1;
     d.prof_having = R.prof_having;
     d.prof_working = R.prof_working;
     d.prof_counts = R.prof_counts;
@@ -250,6 +252,7 @@ static void apply_defaults(gtap_config* c) {
     if (!c->probe_lanes || c->probe_lanes > 64) c->probe_lanes = 8;
     if (!c->max_jobs) c->max_jobs = 65536;
     if (!c->timeout_ms) c->timeout_ms = 20000;
+    if (!c->idle_sleep || c->idle_sleep > 32) c->idle_sleep = 32;
 }
 
 extern "C" {

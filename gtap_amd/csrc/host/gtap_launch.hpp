@@ -202,7 +202,7 @@ inline int replay_thread(const ThreadGraph& g, gtap_run_stats* stats) {
 // One job in slot 0 of the device job table; the root's return value goes to *result.
 template <class P>
 int run_single_job(const typename P::Args& args, const gtap_job& job, int64_t* result, gtap_run_stats* stats,
-                   uint32_t term_mode) {
+                   uint32_t term_mode, gtap_job* job_out = nullptr) {
     Runtime& R = rt();
     if (!R.initialized) {
         int st = gtap_initialize();
@@ -215,6 +215,7 @@ int run_single_job(const typename P::Args& args, const gtap_job& job, int64_t* r
     GTAP_HIP_TRY(hipMemcpy(&out, R.jobs, sizeof(gtap_job), hipMemcpyDeviceToHost));
     if (term_mode == GTAP_TERM_JOBS && out.status != GTAP_JOB_DONE) return fail(GTAP_ERROR_INVALID_TASKWAIT, "job did not complete");
     if (result) *result = out.result;
+    if (job_out) *job_out = out;
     return 0;
 }
 

@@ -452,6 +452,7 @@ struct gtap_stream {
     std::vector<gtap_job> table;    // job table template
     gtap_job* d_table = nullptr;    // the same, device-resident (source of the batched graph's copy)
     host::ThreadGraph graph;        // batched launch, captured on first use
+    std::vector<double> exec_ms;    // last run: device time from admission to completion per job
 };
 
 extern "C" int gtap_stream_destroy(gtap_stream* S);
@@ -470,6 +471,7 @@ extern "C" int gtap_stream_create(uint64_t seed, int32_t njobs, gtap_stream** ou
     S->param.resize(njobs);
     S->kseed.resize(njobs);
     S->key_off.assign(njobs, -1);
+    S->exec_ms.assign(njobs, 0.0);
     gtap_jobs_generate(seed, njobs, S->kind.data(), S->param.data(), S->kseed.data());
     for (int32_t i = 0; i < njobs; ++i) {
         if (S->kind[i] == 2) {
@@ -583,6 +585,7 @@ extern "C" int gtap_stream_run_batched(gtap_stream* S, int64_t* results, double*
         if (out[i].status != GTAP_JOB_DONE) return host::fail(GTAP_ERROR_INVALID_TASKWAIT, "stream job did not complete");
         if (results) results[i] = out[i].result;
         if (latency_ms) latency_ms[i] = (double)(out[i].t_end - t0) * 1e-5;  // 100 MHz ticks
+        S->exec_ms[i] = (double)(out[i].t_end - out[i].t_start) * 1e-5;
     }
     return 0;
 }
@@ -597,13 +600,23 @@ extern "C" int gtap_stream_run_per_job(gtap_stream* S, int64_t* results, double*
     const uint64_t t0 = host_now_ns();
     for (int32_t i = 0; i < S->njobs; ++i) {
         int64_t r = 0;
-        st = host::run_single_job<StreamProgram>(StreamProgram::Args{0}, S->table[i], &r, nullptr, GTAP_TERM_JOBS);
+        gtap_job out;
+        st = host::run_single_job<StreamProgram>(StreamProgram::Args{0}, S->table[i], &r, nullptr, GTAP_TERM_JOBS, &out);
         if (st) return st;
+        S->exec_ms[i] = (double)(out.t_end - out.t_start) * 1e-5;
         if (results) results[i] = r;
         if (latency_ms) latency_ms[i] = (double)(host_now_ns() - t0) * 1e-6;
     }
     if (total_ms) *total_ms = (double)(host_now_ns() - t0) * 1e-6;
     (void)R;
+    return 0;
+}
+
+// Device time from admission to completion of each job in the last run (batched, per job or
+// served), in ms.
+extern "C" int gtap_stream_exec_ms(gtap_stream* S, double* exec_ms) {
+    if (!S || !exec_ms) return host::invalid_argument("gtap_stream_exec_ms");
+    memcpy(exec_ms, S->exec_ms.data(), S->exec_ms.size() * sizeof(double));
     return 0;
 }
 
@@ -638,8 +651,10 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
         for (int32_t i = 0; i < n; ++i) {
             while (host_now_ns() - t0 < arrival[i]) std::this_thread::yield();
             int64_t r = 0;
-            st = host::run_single_job<StreamProgram>(StreamProgram::Args{0}, S->table[i], &r, nullptr, GTAP_TERM_JOBS);
+            gtap_job out;
+            st = host::run_single_job<StreamProgram>(StreamProgram::Args{0}, S->table[i], &r, nullptr, GTAP_TERM_JOBS, &out);
             if (st) return st;
+            S->exec_ms[i] = (double)(out.t_end - out.t_start) * 1e-5;
             res[i] = r;
             lat[i] = (double)(host_now_ns() - t0 - arrival[i]) * 1e-6;
         }
@@ -723,6 +738,7 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
                 res[i] = tab[i].result;
             }
         }
+        for (int32_t i = 0; i < n; ++i) S->exec_ms[i] = (double)(tab[i].t_end - tab[i].t_start) * 1e-5;
     }
     if (results) memcpy(results, res.data(), n * sizeof(int64_t));
     if (latency_ms) memcpy(latency_ms, lat.data(), n * sizeof(double));

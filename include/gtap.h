@@ -148,6 +148,9 @@ int gtap_stream_run_per_job(gtap_stream* stream, int64_t* results, double* laten
 int gtap_stream_serve(gtap_stream* stream, double rate, uint64_t arrival_seed, int32_t per_job_launch,
                       int32_t max_live, int64_t* results, double* latency_ms, double* total_ms,
                       gtap_run_stats* stats);
+/* Device time from admission to completion of each job of the last run (any of the three
+ * forms above), in ms: the execution part of latency_ms. */
+int gtap_stream_exec_ms(gtap_stream* stream, double* exec_ms);
 
 /* ------------------------------------------------------------ input generators (host) */
 /* R-MAT edges, CSR by stable counting sort, and the gen_vector key stream

@@ -50,6 +50,10 @@ def test_stream_small_all_forms():
         res_b, lat_b, stats = st.run_batched()
         check_stream(st, res_b)
         assert stats["jobs_taken"] == 48
+        ex_b = st.exec_ms()
+        # device time from admission to completion: positive, and within the launch-relative
+        # completion time (every job is admitted at or after the launch start)
+        assert np.all(ex_b > 0) and np.all(ex_b <= lat_b + 1e-6)
         print("case stream small per-job", flush=True)
         res_p, lat_p, tot = st.run_per_job()
         check_stream(st, res_p)
@@ -58,6 +62,8 @@ def test_stream_small_all_forms():
         res_s, lat_s, tot_s, stats_s = st.serve(rate=2000.0, arrival_seed=3)
         check_stream(st, res_s)
         assert np.all(lat_s > 0)
+        ex_s = st.exec_ms()
+        assert np.all(ex_s > 0) and np.all(ex_s < tot_s)
         print("case stream small served per-job", flush=True)
         res_q, lat_q, tot_q, _ = st.serve(rate=2000.0, arrival_seed=3, per_job_launch=True)
         check_stream(st, res_q)
