@@ -381,7 +381,7 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
                         st32(dev.ctl + CTL_DONE * 32, 1);
                 }
                 idle_iter += 1;
-                int32_t flags = ld32(dev.ctl + CTL_DONE * 32) | (ld32(dev.ctl + CTL_ABORT * 32) << 1);
+                int32_t flags = ld32(dev.ctl + CTL_DONE * 32);  // an abort sets it too
                 if ((idle_iter & 63u) == 0 && now_ticks() - t_begin > dev.timeout_ticks) {
                     GTAP_DEVICE_ERROR(dev, GTAP_ERROR_WATCHDOG, -1, -1, (int)((now_ticks() - t_begin) / 100000u),
                                       (int)(dev.timeout_ticks / 100000u));

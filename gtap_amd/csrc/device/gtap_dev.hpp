@@ -26,7 +26,7 @@ constexpr int kWave = 64;
 
 // Control words, each on its own 128-byte line (index * 32 ints).
 enum CtlIndex : int {
-    CTL_DONE = 0,       // set once: all waves leave the scheduler loop
+    CTL_DONE = 0,       // nonzero: all waves leave the scheduler loop (1 finished, bit 1 aborted)
     CTL_ABORT = 1,      // set on a device error or watchdog expiry
     CTL_IDLE = 2,       // idle-worker count (quiescence termination)
     CTL_ERROR = 3,      // first error code (CAS from 0)
@@ -384,6 +384,8 @@ __device__ __noinline__ void record_error(int32_t* ctl, gtap_error_report* repor
         }
     }
     st32(ctl + CTL_ABORT * 32, 1);
+    // idle waves watch only the done word
+    __hip_atomic_fetch_or(ctl + CTL_DONE * 32, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace gtap

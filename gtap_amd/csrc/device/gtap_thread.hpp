@@ -780,12 +780,16 @@ __device__ __forceinline__ void load_task_lds(Lane<P>& L, uint32_t id, const uin
 // i < count receives the root id in `tid_out`.
 template <class P>
 __device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me,
-                                              uint32_t& tid_out, bool& idle_registered) {
+                                              uint32_t& tid_out, bool& idle_registered, uint32_t& seen_next) {
     const int lane = lane_id();
     uint32_t avail = dev.njobs;
     if (dev.term_mode == GTAP_TERM_SERVE) avail = uniform((uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32));
+    // the admission counter only grows: once it was seen at `avail`, nothing is left to admit
+    // and the shared counter's line is not read again
+    if (seen_next >= avail) return 0;
     int32_t* nextp = dev.ctl + CTL_JOB_NEXT * 32;
     uint32_t next = uniform((uint32_t)ld32(nextp));
+    seen_next = next;
     if (next >= avail) return 0;
     uint32_t k = avail - next;
     if (k > kWave) k = kWave;
@@ -913,7 +917,10 @@ __device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtxOf<P>* w,
                                             uint32_t& tid_out) {
     const int lane = lane_id();
     uint64_t* wp = dq_word_ptr(dev, q, me);
-    for (int tries = 0; tries < 8; ++tries) {
+    // A failed CAS means a thief took tasks, so this ends; the owner leaves with a task or an
+    // empty deque. Only the owner pushes, so the deque stays empty while the owner is idle
+    // (idle waves do not pop again, and quiescence may count them idle).
+    while (true) {
         uint64_t wv = 0;
         if (lane == 0) wv = ld64(wp);
         wv = shfl(wv, 0);
@@ -936,7 +943,6 @@ __device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtxOf<P>* w,
             return k;
         }
     }
-    return 0;
 }
 
 __device__ __forceinline__ void backoff_sleep(uint32_t level) {
@@ -997,6 +1003,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     bool idle_registered = false;
     uint32_t idle_iter = 0;
     uint32_t busy_iter = 0;
+    uint32_t seen_next = 0;  // last value of the shared admission counter this wave read
     LoopStats st = {0, 0, 0, 0, 0, 0, 0, 0};
     // timeline profiling (lane 0, when dev.prof_max > 0)
     uint32_t ph_idx = 0, pw_idx = 0;
@@ -1012,7 +1019,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             my_lds = false;
             const uint32_t cq = uniform(vld(&w->cur_queue));
 #pragma unroll
-            for (int i = 0; i < Q && n == 0; ++i) {
+            for (int i = 0; i < Q && n == 0 && idle_iter == 0; ++i) {
                 const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
                 n = try_pop<P>(dev, w, me, q, my_tid);
                 if (n) {
@@ -1022,7 +1029,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 }
             }
             if (n == 0 && dev.jobs != nullptr) {
-                n = admit_jobs<P>(dev, w, me, my_tid, idle_registered);
+                n = admit_jobs<P>(dev, w, me, my_tid, idle_registered, seen_next);
                 st.jobs += n;
                 batch_q = -1;
             }
@@ -1053,7 +1060,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 idle_iter += 1;
                 int32_t flags = 0;
                 if (lane == 0) {
-                    flags = ld32(dev.ctl + CTL_DONE * 32) | (ld32(dev.ctl + CTL_ABORT * 32) << 1);
+                    flags = ld32(dev.ctl + CTL_DONE * 32);  // an abort sets it too
                     if (dev.term_mode == GTAP_TERM_SERVE && (idle_iter & 15u) == 0) {
                         const int32_t stop = ld32(dev.ctl + CTL_HOST_STOP * 32);
                         if (stop) {
