@@ -204,10 +204,13 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
                          "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas}}
 
 
-def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024, serve_rate: float = 16000.0) -> dict:
+def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024,
+                 serve_rates=(500.0, 2000.0, 8000.0, 32000.0), serve_jobs: int = 1024) -> dict:
     """The 4096-job mixed stream (fib / n-queens / sort jobs, fixed seed): one resident batched
-    launch against one launch per job, and served arrivals (Poisson at `serve_rate` jobs/s)
-    into the running kernel. Results of the three forms must agree job for job."""
+    launch against one launch per job; then Poisson arrivals swept over `serve_rates` (the first
+    `serve_jobs` jobs of the same stream), served by one resident launch and by one launch per
+    job, with p50/p99 latency from arrival to the result seen by the host. Results of every
+    form must agree job for job."""
     import numpy as np
     st = gtap_amd.Stream(seed=seed, njobs=njobs)
     try:
@@ -220,24 +223,41 @@ def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024, serve_rate: floa
         res_p, lat_p, tot_p = st.run_per_job()
         if not np.array_equal(res_b, res_p):
             raise RuntimeError("stream: batched and per-job results differ")
-        res_s, lat_s, tot_s, _ = st.serve(rate=serve_rate, arrival_seed=5)
-        if not np.array_equal(res_b, res_s):
-            raise RuntimeError("stream: served and batched results differ")
-        out = {
-            "jobs": njobs,
-            "batched": {"kernel_ms_median": kms, "jobs_per_s": njobs / (kms / 1e3),
-                        "p50_ms": float(np.percentile(lat_b, 50)), "p99_ms": float(np.percentile(lat_b, 99))},
-            "per_job_launch": {"total_ms": tot_p, "jobs_per_s": njobs / (tot_p / 1e3),
-                               "p50_ms": float(np.percentile(lat_p, 50)), "p99_ms": float(np.percentile(lat_p, 99))},
-            "served": {"rate_jobs_per_s": serve_rate, "total_ms": tot_s,
-                       "p50_ms": float(np.percentile(lat_s, 50)), "p99_ms": float(np.percentile(lat_s, 99))},
-        }
-        out["batched_over_per_job"] = out["batched"]["jobs_per_s"] / out["per_job_launch"]["jobs_per_s"]
-        log(f"stream: batched {out['batched']['jobs_per_s']:.0f} jobs/s, per-job {out['per_job_launch']['jobs_per_s']:.0f}"
-            f" jobs/s, served p99 {out['served']['p99_ms']:.3f} ms")
-        return out
     finally:
         st.close()
+    out = {
+        "jobs": njobs,
+        "batched": {"kernel_ms_median": kms, "jobs_per_s": njobs / (kms / 1e3),
+                    "p50_ms": float(np.percentile(lat_b, 50)), "p99_ms": float(np.percentile(lat_b, 99))},
+        "per_job_launch": {"total_ms": tot_p, "jobs_per_s": njobs / (tot_p / 1e3),
+                           "p50_ms": float(np.percentile(np.diff(lat_p, prepend=0.0), 50)),
+                           "p99_ms": float(np.percentile(np.diff(lat_p, prepend=0.0), 99))},
+    }
+    out["batched_over_per_job"] = out["batched"]["jobs_per_s"] / out["per_job_launch"]["jobs_per_s"]
+    sweep = []
+    sv = gtap_amd.Stream(seed=seed, njobs=serve_jobs)
+    try:
+        for rate in serve_rates:
+            row = {"rate_jobs_per_s": rate}
+            for form, per_job in (("resident", False), ("per_job_launch", True)):
+                r, lat, tot, _ = sv.serve(rate=rate, arrival_seed=5, per_job_launch=per_job)
+                if not np.array_equal(r, res_b[:serve_jobs]):
+                    raise RuntimeError(f"stream: served ({form}) and batched results differ")
+                ex = sv.exec_ms()
+                row[form] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                             "exec_p50_ms": float(np.percentile(ex, 50)), "jobs_per_s": serve_jobs / (tot / 1e3)}
+            sweep.append(row)
+            log(f"stream served at {rate:.0f} jobs/s: resident p50 {row['resident']['p50_ms']:.3f} p99 "
+                f"{row['resident']['p99_ms']:.3f} ms; per-job launch p50 {row['per_job_launch']['p50_ms']:.3f} "
+                f"p99 {row['per_job_launch']['p99_ms']:.3f} ms")
+    finally:
+        sv.close()
+    out["served_sweep"] = {"jobs": serve_jobs, "arrival_seed": 5, "points": sweep,
+                           "resident_p99_le_per_job_at_every_rate":
+                               all(p["resident"]["p99_ms"] <= p["per_job_launch"]["p99_ms"] for p in sweep)}
+    log(f"stream: batched {out['batched']['jobs_per_s']:.0f} jobs/s, per-job {out['per_job_launch']['jobs_per_s']:.0f}"
+        f" jobs/s ({out['batched_over_per_job']:.1f}x)")
+    return out
 
 
 def load_traffic(name: str):

@@ -202,6 +202,22 @@ def gtap_set_config(**kw) -> int:
     return lib().gtap_set_config(ctypes.byref(c))
 
 
+WORKER_STAT_FIELDS = ("tasks_executed", "tasks_spawned", "rounds", "pops", "steals", "steal_attempts",
+                      "jobs_taken", "pool_refills", "published", "overflowed", "suspended")
+
+
+def worker_stats():
+    """Per-wave counters of the last launch: a (workers, 16) uint64 array whose first columns
+    follow WORKER_STAT_FIELDS."""
+    import numpy as np
+    nw = ctypes.c_uint32(0)
+    _check(lib().gtap_get_worker_stats(None, 0, ctypes.byref(nw)), "gtap_get_worker_stats")
+    out = np.zeros((nw.value, 16), dtype=np.uint64)
+    if nw.value:
+        _check(lib().gtap_get_worker_stats(out.ctypes.data, nw.value, ctypes.byref(nw)), "gtap_get_worker_stats")
+    return out
+
+
 def gtap_get_config() -> dict:
     c = Config()
     _check(lib().gtap_get_config(ctypes.byref(c)), "gtap_get_config")

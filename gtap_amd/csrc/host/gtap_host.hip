@@ -113,9 +113,11 @@ GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job
 }
 
 // Clears the per-launch state: control words, deque words, id-pool ring, statistics.
-__global__ void reset_kernel(GtapDev dev) {
+__global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0) {
     const uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t gs = (uint64_t)gridDim.x * blockDim.x;
+    if (write_job0 && gid < sizeof(gtap_job) / 4)
+        reinterpret_cast<uint32_t*>(dev.jobs)[gid] = reinterpret_cast<const uint32_t*>(&job0)[gid];
     for (uint64_t i = gid; i < (uint64_t)CTL_COUNT * 32; i += gs) dev.ctl[i] = 0;
     for (uint64_t i = gid; i < (uint64_t)dev.nqueues * dev.nworkers; i += gs) dev.dq_word[i * 16] = 0;
     for (uint64_t i = gid; i < (uint64_t)dev.nshards * dev.ncells; i += gs)
@@ -134,16 +136,70 @@ __global__ void reset_kernel(GtapDev dev) {
         dev.pool_ctl[i] = (i == (uint64_t)dev.nshards * 32) ? 64u : 0u;
 }
 
-int enqueue_reset(const GtapDev& dev, hipStream_t stream) {
-    hipLaunchKernelGGL(reset_kernel, dim3(1024), dim3(256), 0, stream, dev);
+int enqueue_reset(const GtapDev& dev, hipStream_t stream, const gtap_job* job0) {
+    gtap_job j;
+    if (job0) j = *job0;
+    else memset(&j, 0, sizeof(j));
+    hipLaunchKernelGGL(reset_kernel, dim3(1024), dim3(256), 0, stream, dev, j, job0 ? 1 : 0);
     GTAP_HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int reset_launch_state(const GtapDev& dev, hipStream_t stream) {
-    if (int st = enqueue_reset(dev, stream)) return st;
+int reset_launch_state(const GtapDev& dev, hipStream_t stream, const gtap_job* job0) {
+    if (int st = enqueue_reset(dev, stream, job0)) return st;
     Runtime& R = rt();
     if (R.report) memset(R.report, 0, sizeof(gtap_error_report));
+    R.stats_host_valid = false;
+    return 0;
+}
+
+// One workgroup: sums the per-worker statistics of the launch that just ended and writes
+// them, the first error code and (on request) job-table slot 0 to the pinned summary.
+constexpr int kSummaryThreads = 256;
+__global__ void __launch_bounds__(kSummaryThreads) summary_kernel(GtapDev dev, LaunchSummary* out, int32_t copy_job0) {
+    __shared__ uint64_t part[ST_COUNT][kSummaryThreads];
+    const uint32_t t = threadIdx.x;
+    uint64_t acc[ST_COUNT];
+#pragma unroll
+    for (int i = 0; i < ST_COUNT; ++i) acc[i] = 0;
+    for (uint32_t w = t; w < dev.nworkers; w += kSummaryThreads) {
+#pragma unroll
+        for (int i = 0; i < ST_COUNT; ++i) acc[i] += dev.stats[(size_t)w * ST_COUNT + i];
+    }
+#pragma unroll
+    for (int i = 0; i < ST_COUNT; ++i) part[i][t] = acc[i];
+    __syncthreads();
+    for (uint32_t h = kSummaryThreads / 2; h > 0; h >>= 1) {
+        if (t < h) {
+#pragma unroll
+            for (int i = 0; i < ST_COUNT; ++i) part[i][t] += part[i][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < ST_COUNT) out->sums[t] = part[t][0];
+    if (t == 0) {
+        out->error_code = dev.ctl[CTL_ERROR * 32];
+        out->has_job0 = copy_job0;
+    }
+    if (copy_job0 && t < sizeof(gtap_job) / 4)
+        reinterpret_cast<uint32_t*>(&out->job0)[t] = reinterpret_cast<const uint32_t*>(dev.jobs)[t];
+}
+
+int enqueue_summary(const GtapDev& dev, hipStream_t stream, bool copy_job0) {
+    Runtime& R = rt();
+    hipLaunchKernelGGL(summary_kernel, dim3(1), dim3(kSummaryThreads), 0, stream, dev, R.summary_dev,
+                       copy_job0 ? 1 : 0);
+    GTAP_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int fetch_worker_stats() {
+    Runtime& R = rt();
+    if (R.stats_host_valid) return 0;
+    const size_t nst = (size_t)R.last_nworkers * ST_COUNT;
+    R.stats_host.resize(nst);
+    if (nst) GTAP_HIP_TRY(hipMemcpy(R.stats_host.data(), R.stats, nst * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    R.stats_host_valid = true;
     return 0;
 }
 
@@ -162,28 +218,22 @@ int finish_launch(const GtapDev& dev, uint32_t blocks, gtap_run_stats* stats) {
     Runtime& R = rt();
     float ms = 0.f;
     GTAP_HIP_TRY(hipEventElapsedTime(&ms, R.ev0, R.ev1));
-    int32_t err = 0;
-    GTAP_HIP_TRY(hipMemcpy(&err, R.ctl + CTL_ERROR * 32, sizeof(err), hipMemcpyDeviceToHost));
-    const size_t nst = (size_t)dev.nworkers * ST_COUNT;
-    R.stats_host.resize(nst);
-    GTAP_HIP_TRY(hipMemcpy(R.stats_host.data(), R.stats, nst * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    const volatile LaunchSummary* sm = R.summary;
+    const int32_t err = sm->error_code;
     gtap_run_stats s;
     memset(&s, 0, sizeof(s));
     s.kernel_ms = ms;
-    for (uint32_t w = 0; w < dev.nworkers; ++w) {
-        const uint64_t* p = &R.stats_host[(size_t)w * ST_COUNT];
-        s.tasks_executed += p[ST_TASKS];
-        s.tasks_spawned += p[ST_SPAWNS];
-        s.rounds += p[ST_ROUNDS];
-        s.pops += p[ST_POPS];
-        s.steals += p[ST_STEALS];
-        s.steal_attempts += p[ST_STEAL_ATTEMPTS];
-        s.jobs_taken += p[ST_JOBS];
-        s.pool_refills += p[ST_REFILLS];
-        s.published += p[ST_PUBLISHED];
-        s.overflowed += p[ST_OVERFLOWED];
-        s.suspended += p[ST_SUSPENDED];
-    }
+    s.tasks_executed = sm->sums[ST_TASKS];
+    s.tasks_spawned = sm->sums[ST_SPAWNS];
+    s.rounds = sm->sums[ST_ROUNDS];
+    s.pops = sm->sums[ST_POPS];
+    s.steals = sm->sums[ST_STEALS];
+    s.steal_attempts = sm->sums[ST_STEAL_ATTEMPTS];
+    s.jobs_taken = sm->sums[ST_JOBS];
+    s.pool_refills = sm->sums[ST_REFILLS];
+    s.published = sm->sums[ST_PUBLISHED];
+    s.overflowed = sm->sums[ST_OVERFLOWED];
+    s.suspended = sm->sums[ST_SUSPENDED];
     s.workers = dev.nworkers;
     s.blocks = blocks;
     R.last_nworkers = dev.nworkers;
@@ -224,6 +274,7 @@ static void free_all() {
     dfree(R.prof_counts);
     if (R.host_ctl) (void)hipHostFree(R.host_ctl);
     if (R.report) (void)hipHostFree(R.report);
+    if (R.summary) (void)hipHostFree(R.summary);
     if (R.jobs_host) (void)hipHostFree(R.jobs_host);
     if (R.ev0) (void)hipEventDestroy(R.ev0);
     if (R.ev1) (void)hipEventDestroy(R.ev1);
@@ -332,6 +383,10 @@ int gtap_initialize(void) {
                                hipHostMallocMapped | hipHostMallocCoherent));
     memset(R.report, 0, sizeof(gtap_error_report));
     GTAP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&R.report_dev), R.report, 0));
+    GTAP_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&R.summary), sizeof(LaunchSummary),
+                               hipHostMallocMapped | hipHostMallocCoherent));
+    memset(R.summary, 0, sizeof(LaunchSummary));
+    GTAP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&R.summary_dev), R.summary, 0));
     GTAP_HIP_TRY(hipDeviceSynchronize());
     R.initialized = true;
     return 0;
@@ -445,6 +500,9 @@ int gtap_get_last_stats(gtap_run_stats* out) {
 
 int gtap_get_worker_stats(uint64_t* out, uint32_t max_workers, uint32_t* nworkers_out) {
     Runtime& R = rt();
+    if (R.initialized) {
+        if (int st = fetch_worker_stats()) return st;
+    }
     const uint32_t nw = (uint32_t)(R.stats_host.size() / ST_COUNT);
     if (nworkers_out) *nworkers_out = nw;
     if (out) {

@@ -26,6 +26,16 @@ namespace host {
 // (n-queens: 16 args + 8 + 16 int64 result slots = 152 bytes of frame).
 constexpr uint32_t kBuiltinStride = 192;
 
+// Written by the summary kernel after each launch into pinned host memory: the per-worker
+// statistics summed, the first error code, and job-table slot 0 (the root of a single-job
+// launch). The host reads it after the stream synchronises, with no copy calls.
+struct LaunchSummary {
+    uint64_t sums[ST_COUNT];
+    int32_t error_code;
+    int32_t has_job0;
+    gtap_job job0;
+};
+
 struct Runtime {
     bool initialized = false;
     int device = 0;
@@ -62,7 +72,10 @@ struct Runtime {
     gtap_job* jobs_host = nullptr;     // pinned job table (served mode)
     gtap_job* jobs_host_dev = nullptr;
     uint64_t* stats = nullptr;
-    std::vector<uint64_t> stats_host;
+    std::vector<uint64_t> stats_host;  // per-worker statistics of the last launch, fetched on demand
+    bool stats_host_valid = false;
+    LaunchSummary* summary = nullptr;  // pinned, device-mapped
+    LaunchSummary* summary_dev = nullptr;
     // timeline profiling (gtap_set_profiling)
     uint32_t prof_max = 0;
     uint64_t* prof_having = nullptr;
@@ -79,10 +92,17 @@ int invalid_argument(const char* msg);
 int hip_status(hipError_t e, const char* what);
 gtap_job make_job(uint32_t func, uint32_t queue, const void* args, size_t nbytes);
 GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job* jobs, uint32_t job_ring);
-int reset_launch_state(const GtapDev& dev, hipStream_t stream);
+// Resets the per-launch device state; a non-null job0 is written into job-table slot 0 by
+// the reset kernel (a single-job launch needs no host-to-device copy).
+int reset_launch_state(const GtapDev& dev, hipStream_t stream, const gtap_job* job0 = nullptr);
 // Enqueues the per-launch state reset kernel only (usable under stream capture).
-int enqueue_reset(const GtapDev& dev, hipStream_t stream);
+int enqueue_reset(const GtapDev& dev, hipStream_t stream, const gtap_job* job0 = nullptr);
+// Enqueues the summary kernel (after the launch's end event): it fills R.summary.
+int enqueue_summary(const GtapDev& dev, hipStream_t stream, bool copy_job0);
+// After the stream has synchronised: kernel time, summed statistics and the error check.
 int finish_launch(const GtapDev& dev, uint32_t blocks, gtap_run_stats* stats);
+// The per-worker statistics of the last launch (copied from the device on first use).
+int fetch_worker_stats();
 int occupancy_blocks(const void* kernel, uint32_t requested, uint32_t* blocks_out);
 
 #define GTAP_HIP_TRY(call)                                              \
@@ -119,18 +139,20 @@ int thread_launch_shape(uint32_t job_ring, uint32_t* blocks_out) {
 
 template <class P>
 int launch_thread(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs,
-                  uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream = nullptr) {
+                  uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream = nullptr,
+                  const gtap_job* job0 = nullptr) {
     Runtime& R = rt();
     uint32_t blocks = 0;
     if (int st = thread_launch_shape<P>(job_ring, &blocks)) return st;
     if (!stream) stream = R.stream;
     GtapDev dev = make_dev(blocks * 4, term_mode, njobs, jobs, job_ring);
-    int st = reset_launch_state(dev, stream);
+    int st = reset_launch_state(dev, stream, job0);
     if (st) return st;
     GTAP_HIP_TRY(hipEventRecord(R.ev0, stream));
     hipLaunchKernelGGL(thread_kernel<P>, dim3(blocks), dim3(256), 0, stream, dev, args);
     GTAP_HIP_TRY(hipGetLastError());
     GTAP_HIP_TRY(hipEventRecord(R.ev1, stream));
+    if ((st = enqueue_summary(dev, stream, job0 != nullptr))) return st;
     GTAP_HIP_TRY(hipStreamSynchronize(stream));
     return finish_launch(dev, blocks, stats);
 }
@@ -195,6 +217,7 @@ inline int replay_thread(const ThreadGraph& g, gtap_run_stats* stats) {
     GTAP_HIP_TRY(hipEventRecord(R.ev0, R.stream));
     GTAP_HIP_TRY(hipGraphLaunch(g.exec, R.stream));
     GTAP_HIP_TRY(hipEventRecord(R.ev1, R.stream));
+    if (int st = enqueue_summary(g.dev, R.stream, false)) return st;
     GTAP_HIP_TRY(hipStreamSynchronize(R.stream));
     return finish_launch(g.dev, g.blocks, stats);
 }
@@ -208,11 +231,9 @@ int run_single_job(const typename P::Args& args, const gtap_job& job, int64_t* r
         int st = gtap_initialize();
         if (st) return st;
     }
-    GTAP_HIP_TRY(hipMemcpyAsync(R.jobs, &job, sizeof(gtap_job), hipMemcpyHostToDevice, R.stream));
-    int st = launch_thread<P>(args, term_mode, 1, R.jobs, 1, stats);
+    int st = launch_thread<P>(args, term_mode, 1, R.jobs, 1, stats, nullptr, &job);
     if (st) return st;
-    gtap_job out;
-    GTAP_HIP_TRY(hipMemcpy(&out, R.jobs, sizeof(gtap_job), hipMemcpyDeviceToHost));
+    const gtap_job out = R.summary->job0;
     if (term_mode == GTAP_TERM_JOBS && out.status != GTAP_JOB_DONE) return fail(GTAP_ERROR_INVALID_TASKWAIT, "job did not complete");
     if (result) *result = out.result;
     if (job_out) *job_out = out;
@@ -222,7 +243,7 @@ int run_single_job(const typename P::Args& args, const gtap_job& job, int64_t* r
 // Persistent launch of a block-mode program: one worker (and one deque) per workgroup.
 template <class P>
 int launch_block(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs, uint32_t job_ring,
-                 gtap_run_stats* stats, hipStream_t stream = nullptr) {
+                 gtap_run_stats* stats, hipStream_t stream = nullptr, const gtap_job* job0 = nullptr) {
     Runtime& R = rt();
     if (!R.initialized) {
         int st = gtap_initialize();
@@ -241,12 +262,13 @@ int launch_block(const typename P::Args& args, uint32_t term_mode, uint32_t njob
     if (R.cfg.block_workers && R.cfg.block_workers < blocks) blocks = R.cfg.block_workers;
     if (blocks > R.max_workers) blocks = R.max_workers;
     GtapDev dev = make_dev(blocks, term_mode, njobs, jobs, job_ring);
-    int st = reset_launch_state(dev, stream);
+    int st = reset_launch_state(dev, stream, job0);
     if (st) return st;
     GTAP_HIP_TRY(hipEventRecord(R.ev0, stream));
     hipLaunchKernelGGL(block_kernel<P>, dim3(blocks), dim3(kBlockThreads), 0, stream, dev, args);
     GTAP_HIP_TRY(hipGetLastError());
     GTAP_HIP_TRY(hipEventRecord(R.ev1, stream));
+    if ((st = enqueue_summary(dev, stream, job0 != nullptr))) return st;
     GTAP_HIP_TRY(hipStreamSynchronize(stream));
     return finish_launch(dev, blocks, stats);
 }
@@ -258,11 +280,9 @@ int run_single_block_job(const typename P::Args& args, const gtap_job& job, int6
         int st = gtap_initialize();
         if (st) return st;
     }
-    GTAP_HIP_TRY(hipMemcpyAsync(R.jobs, &job, sizeof(gtap_job), hipMemcpyHostToDevice, R.stream));
-    int st = launch_block<P>(args, GTAP_TERM_JOBS, 1, R.jobs, 1, stats);
+    int st = launch_block<P>(args, GTAP_TERM_JOBS, 1, R.jobs, 1, stats, nullptr, &job);
     if (st) return st;
-    gtap_job out;
-    GTAP_HIP_TRY(hipMemcpy(&out, R.jobs, sizeof(gtap_job), hipMemcpyDeviceToHost));
+    const gtap_job out = R.summary->job0;
     if (out.status != GTAP_JOB_DONE) return fail(GTAP_ERROR_INVALID_TASKWAIT, "job did not complete");
     if (result) *result = out.result;
     return 0;

@@ -47,6 +47,7 @@ extern "C" void gtap_visualize_profile(const char* app_name) {
     host::Runtime& R = host::rt();
     const char* app = (app_name && app_name[0]) ? app_name : "gtap";
     if (!ensure_dir()) return;
+    if (R.initialized && host::fetch_worker_stats() != 0) return;
 
     if (FILE* f = open_csv(app, "warp_counters")) {
         fprintf(f, "warp_id,tasks_executed,rounds,tasks_spawned,pops,steals,steal_attempts,jobs,pool_refills\n");

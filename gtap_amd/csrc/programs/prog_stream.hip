@@ -698,6 +698,8 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
                            StreamProgram::Args{0});
         GTAP_HIP_TRY(hipGetLastError());
         GTAP_HIP_TRY(hipEventRecord(R.ev1, R.stream));
+        st = host::enqueue_summary(dev, R.stream, false);
+        if (st) return st;
         const uint64_t t0 = host_now_ns();
         int32_t submitted = 0, done = 0, first_pending = 0;
         std::vector<char> seen(n, 0);

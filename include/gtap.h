@@ -68,7 +68,9 @@ int gtap_set_profiling(int enable, uint32_t max_samples);
 /* Statistics of the last persistent launch (no reference counterpart; the reference
  * prints "Execution time" around its launch, examples/fib/fib.cu:36-51). */
 int gtap_get_last_stats(gtap_run_stats* out);
-/* Per-wave counters of the last launch: 16 uint64 per worker. */
+/* Per-wave counters of the last launch: 16 uint64 per worker, in the order tasks executed,
+ * tasks spawned, rounds, pops, steals, steal attempts, jobs admitted, pool refills, records
+ * published, records pushed out of the LDS window, suspended taskwaits, then 5 unused. */
 int gtap_get_worker_stats(uint64_t* out, uint32_t max_workers, uint32_t* nworkers_out);
 int gtap_device_info(char* name, int name_len, int* num_cus, uint64_t* hbm_bytes);
 

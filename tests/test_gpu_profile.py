@@ -4,6 +4,7 @@ task totals agree with the scheduler's own counters."""
 import csv
 import os
 
+import numpy as np
 import pytest
 
 import gtap_amd as G
@@ -73,3 +74,22 @@ def test_profile_off_writes_counters_only(tmp_path, monkeypatch):
     G.lib().gtap_visualize_profile(b"fib_off")
     assert os.path.exists(tmp_path / "profile" / "fib_off_warp_counters.csv")
     assert not os.path.exists(tmp_path / "profile" / "fib_off_warp_timeline_working.csv")
+
+
+@pytest.mark.parametrize("prog,arg,expect", [("nqueens", 10, 724), ("fib", 20, 6765), ("cilksort", 1 << 16, None)])
+def test_worker_stats_sum_to_run_stats(prog, arg, expect):
+    """The per-launch totals are summed on the device; the per-wave counters, copied on
+    demand, must add up to them."""
+    print(f"case worker stats {prog}({arg})", flush=True)
+    if prog == "cilksort":
+        keys = O.gen_keys_u32(arg)
+        out, s = G.cilksort(keys)
+        assert (out == np.sort(keys)).all()
+    else:
+        v, s = getattr(G, prog)(arg)
+        assert v == expect
+    ws = G.worker_stats()
+    assert ws.shape == (s["workers"], 16)
+    for i, f in enumerate(G.WORKER_STAT_FIELDS):
+        assert int(ws[:, i].sum()) == s[f], f
+    assert int(ws[:, 0].sum()) > 0
