@@ -398,7 +398,7 @@ class DeviceBuffer:
 
 # ---------------------------------------------------------------- sorting
 
-def cilksort(keys, leaf: int = 4096, merge_leaf: int = 65536):
+def cilksort(keys, leaf: int = 4096, merge_leaf: int = 32768):
     """Sorted copy of uint32 keys (block-cooperative cilksort on the GPU); returns (keys, stats)."""
     import numpy as np
     a = np.ascontiguousarray(keys, dtype=np.uint32).copy()

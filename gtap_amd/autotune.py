@@ -117,13 +117,13 @@ def workload(name: str, reps: int = 3) -> Tuple[Measure, Dict[str, Sequence[int]
     if name == "cilksort":
         import numpy as np
         keys = gtap_amd.gen_keys(1 << 26, 123456789)
-        space = {"block_workers": (0, 256, 512, 1024), "merge_leaf": (65536, 16384, 262144)}
+        space = {"block_workers": (0, 256, 512, 768), "merge_leaf": (32768, 16384, 65536)}
 
         def measure(p):
             apply(p)
 
             def one():
-                out, st = gtap_amd.cilksort(keys, merge_leaf=p.get("merge_leaf", 65536))
+                out, st = gtap_amd.cilksort(keys, merge_leaf=p.get("merge_leaf", 32768))
                 return hash(out.tobytes()), st["kernel_ms"]
             return _median_ms(one, reps)
         return measure, space

@@ -57,6 +57,7 @@ struct CilkArgs {
 };
 
 struct Program {
+    static constexpr int kWavesPerSimd = 4;  // four 256-thread workgroups per CU
     static constexpr int kFrameWords = 10;
     static constexpr uint32_t kMaxFunc = F_MERGE;
     static constexpr uint32_t kFrameBytes = 40;
@@ -108,10 +109,17 @@ __device__ __forceinline__ void cmpx(uint32_t& x, uint32_t& y, bool up) {
     y = up ? hi : lo;
 }
 
+// Partner exchange of a bitonic step whose partner slot is in lane (lane ^ m) of the same
+// wave: every register of the thread meets the same register of that lane.
+__device__ __forceinline__ uint32_t xlane(uint32_t v, int m) { return (uint32_t)__shfl_xor((int)v, m, kWave); }
+
 // Workgroup sort of arr[0..n), n <= 4096: bitonic sort of 4096 slots (padded with 0xFFFFFFFF,
 // which sorts to the end and is never written back). Thread t owns slots 16t .. 16t+15 in
-// registers: every step whose partner distance j is below 16 runs in registers; the steps
-// with j >= 16 run over LDS, one barrier each (36 such steps instead of 78 LDS steps).
+// registers. A step with partner distance j pairs slot s with s ^ j:
+//   j < 16          the partner is in the same thread: registers;
+//   16 <= j < 1024  the partner is register (s & 15) of lane (lane ^ j/16) of the same wave:
+//                   a cross-lane exchange, no barrier;
+//   j >= 1024       the partner is in another wave: through LDS with barriers (3 steps).
 template <class B>
 __device__ void leaf_sort(B& blk, uint32_t* arr, int64_t n) {
     uint32_t* k = blk.sh->keys;
@@ -143,26 +151,30 @@ __device__ void leaf_sort(B& blk, uint32_t* arr, int64_t n) {
         }
     }
     for (uint32_t size = 32; size <= (uint32_t)kTile; size <<= 1) {
-        __syncthreads();  // every thread has read its slots of the previous size
-#pragma unroll
-        for (int i = 0; i < 16; ++i) k[g0 + (uint32_t)i] = x[i];
-        __syncthreads();
+        const bool up = (g0 & size) == 0;  // direction of this thread's slots at this size
         for (uint32_t j = size >> 1; j >= 16; j >>= 1) {
+            const uint32_t m = j / 16;  // partner thread = t ^ m
+            const bool low = (t & m) == 0;
+            const bool keep_min = (low == up);
+            if (j >= 1024) {
+                __syncthreads();  // every thread has read the previous contents
 #pragma unroll
-            for (uint32_t q = 0; q < (uint32_t)kTile / 2 / kT; ++q) {
-                const uint32_t p = q * kT + t;
-                const uint32_t lo = ((p & ~(j - 1)) << 1) | (p & (j - 1));
-                const uint32_t hi = lo + j;
-                uint32_t a = k[lo], b = k[hi];
-                cmpx(a, b, (lo & size) == 0);
-                k[lo] = a;
-                k[hi] = b;
+                for (int i = 0; i < 16; ++i) k[g0 + (uint32_t)i] = x[i];
+                __syncthreads();
+                const uint32_t p0 = 16u * (t ^ m);
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t y = k[p0 + (uint32_t)i];
+                    x[i] = keep_min ? (x[i] < y ? x[i] : y) : (x[i] < y ? y : x[i]);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t y = xlane(x[i], (int)m);
+                    x[i] = keep_min ? (x[i] < y ? x[i] : y) : (x[i] < y ? y : x[i]);
+                }
             }
-            __syncthreads();
         }
-#pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = k[g0 + (uint32_t)i];
-        const bool up = (g0 & size) == 0;
 #pragma unroll
         for (uint32_t j = 8; j > 0; j >>= 1) {
 #pragma unroll
@@ -418,14 +430,14 @@ static_assert(CilkProgram::kFrameWords <= 16, "block frame cache holds 16 words"
 
 // Sorts n uint32 keys in place (device buffers keys and tmp, n elements each).
 // leaf <= 4096 keys are sorted by one workgroup in LDS (0 -> 4096); merges of at most
-// merge_leaf keys are done by one workgroup (0 -> 65536).
+// merge_leaf keys are done by one workgroup (0 -> 32768).
 // Replaces the program of evaluation/2-comparison/cilksort/gtap_cilksort.cu:169-281.
 extern "C" int gtap_cilksort(uint32_t* keys, uint32_t* tmp, int64_t n, int64_t leaf, int64_t merge_leaf,
                              gtap_run_stats* stats) {
     using namespace gtap;
     if (!keys || !tmp || n < 0) return host::invalid_argument("gtap_cilksort: bad arguments");
     if (leaf <= 0) leaf = 4096;
-    if (merge_leaf <= 0) merge_leaf = 65536;
+    if (merge_leaf <= 0) merge_leaf = 32768;
     if (leaf > cilk::kTile) return host::invalid_argument("gtap_cilksort: leaf must be <= 4096");
     if (merge_leaf < 3) merge_leaf = 3;
     if (n == 0) {

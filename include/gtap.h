@@ -119,7 +119,7 @@ int gtap_bfs_host(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne
 
 /* Cilksort of n uint32 keys in place, block-cooperative (one task per 256-thread
  * workgroup): keys and tmp are device buffers of n keys; leaves of <= leaf keys (0 -> 4096,
- * max 4096) are sorted in LDS by one workgroup; merges of <= merge_leaf keys (0 -> 65536)
+ * max 4096) are sorted in LDS by one workgroup; merges of <= merge_leaf keys (0 -> 32768)
  * are merged by one workgroup. Replaces evaluation/2-comparison/cilksort/gtap_cilksort.cu:89-281. */
 int gtap_cilksort(uint32_t* keys, uint32_t* tmp, int64_t n, int64_t leaf, int64_t merge_leaf,
                   gtap_run_stats* stats);
