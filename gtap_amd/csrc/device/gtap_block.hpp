@@ -22,7 +22,21 @@ namespace gtap {
 constexpr uint32_t kBlockThreads = 256;
 constexpr uint32_t kBlockMaxSpawn = 128;   // spawns per task invocation (ids kept ready in LDS)
 
+// Scheduler loop state of thread 0, kept in LDS rather than in registers: it stays live
+// across every task body, where registers are scarce.
+struct BlockLoop {
+    uint64_t t_begin;
+    uint64_t host_poll_last;
+    uint32_t rng;
+    uint32_t idle_iter;
+    uint32_t ph_idx, pw_idx;  // timeline profiling
+    bool idle_registered;
+    bool had_task;
+    uint32_t tasks, pops, steals, attempts, jobs;
+};
+
 struct BlockCtx {
+    BlockLoop lp;
     // scheduler state (thread 0)
     uint32_t cur;          // task to run next (0 = none)
     uint32_t exit;
@@ -253,24 +267,25 @@ __device__ __forceinline__ uint32_t block_pop(const GtapDev& dev, BlockCtx* c, u
 }
 
 // Thread 0 only: steal the oldest task of a random victim.
-__device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me, uint32_t& rng, bool& idle_registered,
-                                                LoopStats& st) {
+__device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me, BlockLoop& lp) {
     if (dev.nworkers < 2) return 0;
     for (int probe = 0; probe < 4; ++probe) {
+        uint32_t rng = lp.rng;
         uint32_t victim = xorshift32(rng) % (dev.nworkers - 1);
+        lp.rng = rng;
         if (victim >= me) victim += 1;
         uint64_t* wp = dq_word_ptr(dev, 0, victim);
         const uint64_t wv = ld64(wp);
         const uint32_t t = (uint32_t)(wv >> 32), b = (uint32_t)wv;
         if ((int32_t)(b - t) <= 0) continue;
-        if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
+        if (dev.term_mode == GTAP_TERM_QUIESCE && lp.idle_registered) {
             atomic_add(dev.ctl + CTL_IDLE * 32, -1);
-            idle_registered = false;
+            lp.idle_registered = false;
         }
         uint64_t expected = wv;
-        st.attempts += 1;
+        lp.attempts += 1;
         if (atomic_cas(wp, expected, wv + ((uint64_t)1 << 32))) {
-            st.steals += 1;
+            lp.steals += 1;
             return ld32(dq_slot_ptr(dev, 0, victim, t));
         }
         return 0;
@@ -325,8 +340,17 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
                                 typename P::Shared* sh) {
     const uint32_t t = blk_thread();
     const uint32_t me = blockIdx.x;
-    const uint64_t t_begin = now_ticks();
+    BlockLoop& lp = c->lp;
     if (t == 0) {
+        lp.t_begin = now_ticks();
+        lp.host_poll_last = 0;
+        lp.rng = 0x9E3779B9u ^ (me * 0x85EBCA6Bu) ^ (uint32_t)lp.t_begin;
+        if (lp.rng == 0) lp.rng = 1;
+        lp.idle_iter = 0;
+        lp.ph_idx = lp.pw_idx = 0;
+        lp.idle_registered = false;
+        lp.had_task = false;
+        lp.tasks = lp.pops = lp.steals = lp.attempts = lp.jobs = 0;
         c->cur = 0;
         c->exit = 0;
         c->tail = c->bottom = c->top_cache = 0;
@@ -346,44 +370,35 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
     T.sh = sh;
     T.me = me;
 
-    uint32_t rng = 0x9E3779B9u ^ (me * 0x85EBCA6Bu) ^ (uint32_t)t_begin;
-    if (rng == 0) rng = 1;
-    bool idle_registered = false;
-    uint32_t idle_iter = 0;
-    LoopStats st = {0, 0, 0, 0, 0, 0};
-    uint32_t ph_idx = 0, pw_idx = 0;  // timeline profiling (thread 0)
-    bool had_task = false;
-
-    uint64_t host_poll_last = 0;
     const bool host_poller = dev.term_mode == GTAP_TERM_SERVE && me == 0 && t == 0;
 
     while (true) {
-        if (host_poller) host_poll(dev.ctl, dev.host_ctl, host_poll_last);
+        if (host_poller) host_poll(dev.ctl, dev.host_ctl, lp.host_poll_last);
         // keep enough ids in LDS for one invocation's spawns (collective)
         while (c->free_top < kBlockMaxSpawn) block_refill_chunk(dev, c, me);
         if (t == 0 && c->cur == 0) {
             uint32_t id = block_pop(dev, c, me);
-            if (id) st.pops += 1;
+            if (id) lp.pops += 1;
             if (!id && dev.jobs != nullptr) {
-                id = block_admit<P>(dev, c, idle_registered);
-                if (id) st.jobs += 1;
+                id = block_admit<P>(dev, c, lp.idle_registered);
+                if (id) lp.jobs += 1;
             }
-            if (!id) id = block_steal(dev, me, rng, idle_registered, st);
+            if (!id) id = block_steal(dev, me, lp);
             if (!id) {
-                if (had_task) {
-                    had_task = false;
-                    if (dev.prof_max) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, ph_idx, 0);
+                if (lp.had_task) {
+                    lp.had_task = false;
+                    if (dev.prof_max) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, lp.ph_idx, 0);
                 }
-                if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered) {
-                    idle_registered = true;
+                if (dev.term_mode == GTAP_TERM_QUIESCE && !lp.idle_registered) {
+                    lp.idle_registered = true;
                     const int32_t cnt = atomic_add(dev.ctl + CTL_IDLE * 32, 1) + 1;
                     if ((uint32_t)cnt == dev.nworkers && (uint32_t)ld32(dev.ctl + CTL_JOB_NEXT * 32) >= dev.njobs)
                         st32(dev.ctl + CTL_DONE * 32, 1);
                 }
-                idle_iter += 1;
+                const uint32_t idle_iter = ++lp.idle_iter;
                 int32_t flags = ld32(dev.ctl + CTL_DONE * 32);  // an abort sets it too
-                if ((idle_iter & 63u) == 0 && now_ticks() - t_begin > dev.timeout_ticks) {
-                    GTAP_DEVICE_ERROR(dev, GTAP_ERROR_WATCHDOG, -1, -1, (int)((now_ticks() - t_begin) / 100000u),
+                if ((idle_iter & 63u) == 0 && now_ticks() - lp.t_begin > dev.timeout_ticks) {
+                    GTAP_DEVICE_ERROR(dev, GTAP_ERROR_WATCHDOG, -1, -1, (int)((now_ticks() - lp.t_begin) / 100000u),
                                       (int)(dev.timeout_ticks / 100000u));
                     flags |= 2;
                 }
@@ -393,7 +408,7 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
                     backoff_sleep(lvl < dev.idle_level ? lvl : dev.idle_level);
                 }
             } else {
-                idle_iter = 0;
+                lp.idle_iter = 0;
                 c->cur = id;
             }
             if (c->cur) {
@@ -435,12 +450,14 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
         T.parent = c->parent;
         T.res_off = c->res_off;
         T.rec = rec_ptr(dev, T.tid);
-        st.tasks += 1;
-        if (t == 0 && dev.prof_max) {
-            if (!had_task) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, ph_idx, 0);
-            prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, 1);
+        if (t == 0) {
+            lp.tasks += 1;
+            if (dev.prof_max) {
+                if (!lp.had_task) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, lp.ph_idx, 0);
+                prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, lp.pw_idx, 1);
+            }
+            lp.had_task = true;
         }
-        had_task = true;
         if (T.func == 0 || T.func > P::kMaxFunc) {
             if (t == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, T.tid, 0, T.func, P::kMaxFunc);
             T.finish();
@@ -484,8 +501,8 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
                 c->bottom = c->tail;
             }
             c->cur = next;
-            if (dev.prof_max) prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, 1);
-            if ((st.tasks & 63u) == 0 && ld32(dev.ctl + CTL_ABORT * 32)) c->exit = 1;
+            if (dev.prof_max) prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, lp.pw_idx, 1);
+            if ((lp.tasks & 63u) == 0 && ld32(dev.ctl + CTL_ABORT * 32)) c->exit = 1;
         }
         __syncthreads();
         if (c->exit) break;
@@ -494,13 +511,13 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
 
     if (t == 0) {
         uint64_t* s = dev.stats + (size_t)me * ST_COUNT;
-        s[ST_TASKS] = st.tasks;
+        s[ST_TASKS] = lp.tasks;
         s[ST_SPAWNS] = c->spawns;
-        s[ST_ROUNDS] = st.tasks;
-        s[ST_POPS] = st.pops;
-        s[ST_STEALS] = st.steals;
-        s[ST_STEAL_ATTEMPTS] = st.attempts;
-        s[ST_JOBS] = st.jobs;
+        s[ST_ROUNDS] = lp.tasks;
+        s[ST_POPS] = lp.pops;
+        s[ST_STEALS] = lp.steals;
+        s[ST_STEAL_ATTEMPTS] = lp.attempts;
+        s[ST_JOBS] = lp.jobs;
         s[ST_REFILLS] = c->refills;
     }
 }

@@ -74,19 +74,6 @@ struct Program {
 __device__ __forceinline__ uint32_t ldk(const uint32_t* p) { return ld32(p); }
 __device__ __forceinline__ void stk(uint32_t* p, uint32_t v) { st32(p, v); }
 
-// Number of `a` elements among the first d outputs of merge(a, b) when a goes first on ties.
-template <class LDA, class LDB>
-__device__ __forceinline__ int64_t merge_path(LDA lda, int64_t na, LDB ldb, int64_t nb, int64_t d) {
-    int64_t lo = d > nb ? d - nb : 0;
-    int64_t hi = d < na ? d : na;
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (lda(mid) <= ldb(d - 1 - mid)) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // gtap_cilksort.cu:40-50: largest index with x[index] <= val, or -1.
 __device__ __forceinline__ int64_t binary_search_gm(const uint32_t* x, int64_t len, uint32_t val) {
     int64_t low = -1, high = len;
@@ -214,6 +201,57 @@ __device__ void block_copy(B& blk, const uint32_t* src, uint32_t* dst, int64_t n
     }
 }
 
+// Merge-path splits of nd diagonals at once (nd <= kT), d_k = min(d0 + k * step, na + nb):
+// split[k] = number of a elements among the first d_k outputs of merge(a, b), a first on
+// ties. The result is that of merge_path; the search is k-ary instead of binary. The
+// workgroup's threads form groups of G lanes (G a power of two, G * nd <= kT, G <= 64, so a
+// group never straddles a wave); each round a group evaluates the predicate
+// a[i] <= b[d - 1 - i] at G points of its range together (one load pair per lane) and keeps
+// the sub-range between the last true and the first false probe, so a range of R candidates
+// takes about log_(G+1) R dependent global loads instead of log_2 R.
+template <class LDA, class LDB>
+__device__ void block_split_search(LDA lda, int64_t na, LDB ldb, int64_t nb, int64_t d0, int64_t step, int nd,
+                                   int64_t* split) {
+    const uint32_t t = threadIdx.x;
+    int G = kWave;
+    while (G > 1 && G * nd > (int)kT) G >>= 1;
+    const int grp = (int)t / G, k = (int)t % G;
+    const int64_t total = na + nb;
+    const bool mine = grp < nd;
+    int64_t d = 0, lo = 0, hi = 0;
+    if (mine) {
+        d = d0 + (int64_t)grp * step;
+        if (d > total) d = total;
+        lo = d > nb ? d - nb : 0;
+        hi = d < na ? d : na;
+    }
+    const uint32_t shift = (uint32_t)(lane_id() / G) * (uint32_t)G;  // this group's bits in a ballot
+    const uint64_t gmask = G == kWave ? ~0ull : ((1ull << G) - 1ull);
+    while (true) {
+        const bool active = mine && lo < hi;
+        if (ballot(active) == 0) break;
+        const int64_t span = hi - lo;
+        const bool fine = span <= G;  // every candidate gets a probe
+        const int64_t ik = fine ? lo + k : lo + ((int64_t)(k + 1) * span) / (G + 1);
+        bool p = false;
+        if (active && (!fine || k < span)) p = lda(ik) <= ldb(d - 1 - ik);
+        const int c = popc((ballot(p) >> shift) & gmask);  // the predicate holds on the first c probes
+        if (active) {
+            if (fine) {
+                lo += c;
+                hi = lo;
+            } else {
+                const int64_t nlo = c > 0 ? lo + ((int64_t)c * span) / (G + 1) + 1 : lo;
+                const int64_t nhi = c < G ? lo + ((int64_t)(c + 1) * span) / (G + 1) : hi;
+                lo = nlo;
+                hi = nhi;
+            }
+        }
+    }
+    if (mine && k == 0) split[grp] = lo;
+    __syncthreads();
+}
+
 // The inputs of one merge tile: a[i0 .. i0+la) then b[j0 .. j0+lb), loaded kPer per thread.
 __device__ __forceinline__ void load_tile(const uint32_t* a, int64_t i0, int64_t la, const uint32_t* b, int64_t j0,
                                           int64_t lb, uint32_t (&r)[kPer]) {
@@ -225,8 +263,23 @@ __device__ __forceinline__ void load_tile(const uint32_t* a, int64_t i0, int64_t
     }
 }
 
+// Number of A elements among the first d outputs of merge(A, B) over LDS, A first on ties.
+__device__ __forceinline__ int32_t merge_path_lds(const uint32_t* A, int32_t la, const uint32_t* Bv, int32_t lb,
+                                                  int32_t d) {
+    int32_t lo = d > lb ? d - lb : 0;
+    int32_t hi = d < la ? d : la;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (A[mid] <= Bv[d - 1 - mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Workgroup merge of a[0..na) and b[0..nb) into dst, tile by tile (merge path). The inputs of
-// the next tile are loaded into registers while the current tile is merged in LDS.
+// the next tile are loaded into registers while the current tile is merged in LDS. Each
+// thread merges 16 consecutive outputs of a tile, keeping the current head of each input in
+// a register, so each output costs one LDS load.
 template <class B>
 __device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t* b, int64_t nb, uint32_t* dst) {
     typename Program::Shared* sh = blk.sh;
@@ -234,21 +287,19 @@ __device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t
     const int64_t total = na + nb;
     auto ga = [a](int64_t i) { return ldk(a + i); };
     auto gb = [b](int64_t i) { return ldk(b + i); };
+    int64_t carry = 0;  // split at the first diagonal of the group
     for (int64_t base = 0; base < total; base += (int64_t)kTile * 16) {
-        // splits of up to 16 tiles, found in parallel
+        // splits at the ends of up to 16 tiles
         const int64_t ntiles_here = ((total - base) + kTile - 1) / kTile;
-        const int64_t ntl = ntiles_here < 16 ? ntiles_here : 16;
-        if ((int64_t)t <= ntl) {
-            int64_t d = base + (int64_t)t * kTile;
-            if (d > total) d = total;
-            sh->split[t] = merge_path(ga, na, gb, nb, d);
-        }
-        __syncthreads();
+        const int ntl = ntiles_here < 16 ? (int)ntiles_here : 16;
+        if (t == 0) sh->split[0] = carry;
+        block_split_search(ga, na, gb, nb, base + kTile, kTile, ntl, sh->split + 1);
+        carry = sh->split[ntl];
         // The splits of sorted inputs are monotone along both inputs; anything else (unsorted
         // runs) would index outside the tile, so it is reported and the merge abandoned.
         bool ok = true;
-        for (int64_t ti = 0; ti < ntl; ++ti) {
-            const int64_t d0 = base + ti * kTile, d1 = (d0 + kTile < total) ? d0 + kTile : total;
+        for (int ti = 0; ti < ntl; ++ti) {
+            const int64_t d0 = base + (int64_t)ti * kTile, d1 = (d0 + kTile < total) ? d0 + kTile : total;
             const int64_t i0 = sh->split[ti], i1 = sh->split[ti + 1];
             const int64_t la = i1 - i0, lb = (d1 - i1) - (d0 - i0);
             ok = ok && la >= 0 && lb >= 0 && i1 <= na && (d1 - i1) <= nb;
@@ -264,11 +315,11 @@ __device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t
             const int64_t d0 = base, d1 = (d0 + kTile < total) ? d0 + kTile : total;
             load_tile(a, i0, i1 - i0, b, d0 - i0, (d1 - i1) - (d0 - i0), r);
         }
-        for (int64_t ti = 0; ti < ntl; ++ti) {
-            const int64_t d0 = base + ti * kTile;
+        for (int ti = 0; ti < ntl; ++ti) {
+            const int64_t d0 = base + (int64_t)ti * kTile;
             const int64_t d1 = (d0 + kTile < total) ? d0 + kTile : total;
             const int64_t i0 = sh->split[ti], i1 = sh->split[ti + 1];
-            const int64_t la = i1 - i0, lb = (d1 - i1) - (d0 - i0);
+            const int32_t la = (int32_t)(i1 - i0), lb = (int32_t)((d1 - i1) - (d0 - i0));
 #pragma unroll
             for (int q = 0; q < kPer; ++q) sh->keys[(uint32_t)q * kT + t] = r[q];
             __syncthreads();
@@ -277,27 +328,43 @@ __device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t
                 const int64_t n0 = sh->split[ti + 1], n1 = sh->split[ti + 2];
                 load_tile(a, n0, n1 - n0, b, e0 - n0, (e1 - n1) - (e0 - n0), r);
             }
-            const uint32_t* A = sh->keys;
-            const uint32_t* Bv = sh->keys + la;
-            const int64_t len = d1 - d0;
-            const int64_t od = (int64_t)t * kPer;
+            const uint32_t* K = sh->keys;  // A = K[0..la), B = K[la..la+lb)
+            const int32_t len = (int32_t)(d1 - d0);
+            const int32_t od = (int32_t)t * kPer;
             if (od < len) {
-                auto la_ = [A](int64_t i) { return A[i]; };
-                auto lb_ = [Bv](int64_t i) { return Bv[i]; };
-                int64_t i = merge_path(la_, la, lb_, lb, od);
-                int64_t j = od - i;
-                const int64_t oe = (od + kPer < len) ? od + kPer : len;
-                for (int64_t o = od; o < oe; ++o) {
-                    const bool takeA = i < la && (j >= lb || A[i] <= Bv[j]);
-                    sh->out[o] = takeA ? A[i] : Bv[j];
-                    if (takeA) ++i;
-                    else ++j;
+                int32_t i = merge_path_lds(K, la, K + la, lb, od);
+                int32_t j = od - i;
+                uint32_t av = i < la ? K[i] : 0u;
+                uint32_t bv = j < lb ? K[la + j] : 0u;
+                const int32_t cnt = (od + kPer < len) ? kPer : len - od;
+                uint32_t outv[kPer];
+#pragma unroll
+                for (int o = 0; o < kPer; ++o) {
+                    const bool takeA = i < la && (j >= lb || av <= bv);
+                    outv[o] = takeA ? av : bv;
+                    i += takeA ? 1 : 0;
+                    j += takeA ? 0 : 1;
+                    // the next head of the input just consumed
+                    const int32_t at = takeA ? i : la + j;
+                    const bool in = takeA ? i < la : j < lb;
+                    const uint32_t v = (in && o + 1 < cnt) ? K[at] : 0u;
+                    av = takeA ? v : av;
+                    bv = takeA ? bv : v;
+                }
+                if (cnt == kPer) {
+#pragma unroll
+                    for (int o = 0; o < kPer; o += 4)
+                        *reinterpret_cast<uint4*>(&sh->out[od + o]) = make_uint4(outv[o], outv[o + 1], outv[o + 2], outv[o + 3]);
+                } else {
+#pragma unroll
+                    for (int o = 0; o < kPer; ++o)
+                        if (o < cnt) sh->out[od + o] = outv[o];
                 }
             }
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < kPer; ++q) {
-                const int64_t o = (int64_t)q * kT + t;
+                const int32_t o = q * (int32_t)kT + (int32_t)t;
                 if (o < len) stk(dst + d0 + o, sh->out[o]);
             }
         }
@@ -392,12 +459,11 @@ __device__ void merge_task(B& blk) {
         auto ga = [a](int64_t i) { return ldk(a + i); };
         auto gb = [b](int64_t i) { return ldk(b + i); };
         const int64_t t = threadIdx.x;
-        if (t <= fan) {
-            int64_t d = t * chunk;
-            if (d > total) d = total;
-            sh->split[t] = merge_path(ga, na, gb, nb, d);
+        if (t == 0) {
+            sh->split[0] = 0;
+            sh->split[fan] = na;
         }
-        __syncthreads();
+        block_split_search(ga, na, gb, nb, chunk, chunk, (int)fan - 1, sh->split + 1);
         if (t < fan) {
             int64_t d0 = t * chunk, d1 = d0 + chunk;
             if (d0 > total) d0 = total;
