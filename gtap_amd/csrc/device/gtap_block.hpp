@@ -266,8 +266,15 @@ __device__ __forceinline__ uint32_t block_pop(const GtapDev& dev, BlockCtx* c, u
     return 0;
 }
 
-// Thread 0 only: steal the oldest task of a random victim.
-__device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me, BlockLoop& lp) {
+// Thread 0 only: steal the oldest half (at most kBlockStealMax, and at most the configured
+// steal batch) of a random victim's published tasks. The oldest one runs here; the others
+// are pushed onto this workgroup's own deque and published at once, so a wide fan-out
+// reaches idle workgroups in a logarithmic number of steal rounds instead of one steal per
+// task from the spawner. Claimed slots are read after the CAS; the owner never writes
+// within kMargin of the top it last saw, so they cannot be overwritten first.
+constexpr uint32_t kBlockStealMax = 16;
+__device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me, BlockCtx* c) {
+    BlockLoop& lp = c->lp;
     if (dev.nworkers < 2) return 0;
     for (int probe = 0; probe < 4; ++probe) {
         uint32_t rng = lp.rng;
@@ -277,18 +284,45 @@ __device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me,
         uint64_t* wp = dq_word_ptr(dev, 0, victim);
         const uint64_t wv = ld64(wp);
         const uint32_t t = (uint32_t)(wv >> 32), b = (uint32_t)wv;
-        if ((int32_t)(b - t) <= 0) continue;
+        const int32_t sz = (int32_t)(b - t);
+        if (sz <= 0) continue;
+        uint32_t k = ((uint32_t)sz + 1u) / 2u;
+        const uint32_t cap = dev.steal_batch < kBlockStealMax ? dev.steal_batch : kBlockStealMax;
+        if (k > cap) k = cap;
+        if (k == 0) k = 1;
         if (dev.term_mode == GTAP_TERM_QUIESCE && lp.idle_registered) {
             atomic_add(dev.ctl + CTL_IDLE * 32, -1);
             lp.idle_registered = false;
         }
         uint64_t expected = wv;
         lp.attempts += 1;
-        if (atomic_cas(wp, expected, wv + ((uint64_t)1 << 32))) {
-            lp.steals += 1;
-            return ld32(dq_slot_ptr(dev, 0, victim, t));
+        if (!atomic_cas(wp, expected, wv + ((uint64_t)k << 32))) return 0;
+        lp.steals += 1;
+        const uint32_t first = ld32(dq_slot_ptr(dev, 0, victim, t));
+        if (k > 1) {
+            uint32_t ids[kBlockStealMax];
+#pragma unroll
+            for (uint32_t i = 1; i < kBlockStealMax; ++i)
+                if (i < k) ids[i] = ld32(dq_slot_ptr(dev, 0, victim, t + i));
+            uint32_t pos = c->tail;
+#pragma unroll
+            for (uint32_t i = 1; i < kBlockStealMax; ++i) {
+                if (i < k) {
+                    if (pos + 1u - c->top_cache > dev.cap - kMargin) {
+                        GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, ids[i], 0, pos + 1u - c->top_cache, dev.cap - kMargin);
+                    } else {
+                        st32(dq_slot_ptr(dev, 0, me, pos), ids[i]);
+                        pos += 1;
+                    }
+                }
+            }
+            c->tail = pos;
+            complete_stores();  // the slots are written before they are published
+            const uint64_t old = atomic_add(dq_word_ptr(dev, 0, me), (uint64_t)(c->tail - c->bottom));
+            c->top_cache = (uint32_t)(old >> 32);
+            c->bottom = c->tail;
         }
-        return 0;
+        return first;
     }
     return 0;
 }
@@ -383,7 +417,7 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
                 id = block_admit<P>(dev, c, lp.idle_registered);
                 if (id) lp.jobs += 1;
             }
-            if (!id) id = block_steal(dev, me, lp);
+            if (!id) id = block_steal(dev, me, c);
             if (!id) {
                 if (lp.had_task) {
                     lp.had_task = false;

@@ -185,6 +185,16 @@ __device__ __forceinline__ bool atomic_cas(T* p, T& expected, T desired) {
 // Wait until every memory operation this wave issued has completed. After it, the wave's
 // sc1 stores are visible device-wide, so the next atomic may publish them.
 __device__ __forceinline__ void complete_stores() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Workgroup barrier that orders LDS accesses only. __syncthreads() also waits for every global
+// load and store of the calling wave to complete (a workgroup-scope fence over all address
+// spaces); this one leaves them in flight, so a tile loop can keep its prefetches and its
+// output stores outstanding across the barriers that hand its LDS tiles between waves.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 __device__ __forceinline__ void compiler_barrier() { __asm__ volatile("" ::: "memory"); }
 
 // 16-byte device-coherent access to task records. A raw buffer access over the record pool

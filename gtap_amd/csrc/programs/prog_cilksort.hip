@@ -107,6 +107,8 @@ __device__ __forceinline__ uint32_t xlane(uint32_t v, int m) { return (uint32_t)
 //   16 <= j < 1024  the partner is register (s & 15) of lane (lane ^ j/16) of the same wave:
 //                   a cross-lane exchange, no barrier;
 //   j >= 1024       the partner is in another wave: through LDS with barriers (3 steps).
+// Barriers here order LDS only (lds_barrier): the global loads and stores of a task need no
+// ordering among its threads, and the scheduler's barrier after the task waits for them.
 template <class B>
 __device__ void leaf_sort(B& blk, uint32_t* arr, int64_t n) {
     uint32_t* k = blk.sh->keys;
@@ -121,7 +123,7 @@ __device__ void leaf_sort(B& blk, uint32_t* arr, int64_t n) {
 #pragma unroll
         for (int q = 0; q < kPer; ++q) k[(uint32_t)q * kT + t] = r[q];
     }
-    __syncthreads();
+    lds_barrier();
     uint32_t x[16];
     const uint32_t g0 = 16u * t;  // global slot of x[0]
 #pragma unroll
@@ -144,10 +146,10 @@ __device__ void leaf_sort(B& blk, uint32_t* arr, int64_t n) {
             const bool low = (t & m) == 0;
             const bool keep_min = (low == up);
             if (j >= 1024) {
-                __syncthreads();  // every thread has read the previous contents
+                lds_barrier();  // every thread has read the previous contents
 #pragma unroll
                 for (int i = 0; i < 16; ++i) k[g0 + (uint32_t)i] = x[i];
-                __syncthreads();
+                lds_barrier();
                 const uint32_t p0 = 16u * (t ^ m);
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
@@ -170,10 +172,10 @@ __device__ void leaf_sort(B& blk, uint32_t* arr, int64_t n) {
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int i = 0; i < 16; ++i) k[g0 + (uint32_t)i] = x[i];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
         const uint32_t e = (uint32_t)q * kT + t;
@@ -249,7 +251,7 @@ __device__ void block_split_search(LDA lda, int64_t na, LDB ldb, int64_t nb, int
         }
     }
     if (mine && k == 0) split[grp] = lo;
-    __syncthreads();
+    lds_barrier();
 }
 
 // The inputs of one merge tile: a[i0 .. i0+la) then b[j0 .. j0+lb), loaded kPer per thread.
@@ -306,7 +308,7 @@ __device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t
         }
         if (!ok) {
             if (t == 0) GTAP_DEVICE_ERROR(*blk.dev, GTAP_ERROR_INVALID_ARGUMENT, blk.tid, -1, (int)na, (int)nb);
-            __syncthreads();
+            lds_barrier();
             return;
         }
         uint32_t r[kPer];
@@ -322,7 +324,7 @@ __device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t
             const int32_t la = (int32_t)(i1 - i0), lb = (int32_t)((d1 - i1) - (d0 - i0));
 #pragma unroll
             for (int q = 0; q < kPer; ++q) sh->keys[(uint32_t)q * kT + t] = r[q];
-            __syncthreads();
+            lds_barrier();
             if (ti + 1 < ntl) {  // next tile's inputs in flight during this merge
                 const int64_t e0 = d1, e1 = (e0 + kTile < total) ? e0 + kTile : total;
                 const int64_t n0 = sh->split[ti + 1], n1 = sh->split[ti + 2];
@@ -361,14 +363,14 @@ __device__ void leaf_merge(B& blk, const uint32_t* a, int64_t na, const uint32_t
                         if (o < cnt) sh->out[od + o] = outv[o];
                 }
             }
-            __syncthreads();
+            lds_barrier();
 #pragma unroll
             for (int q = 0; q < kPer; ++q) {
                 const int32_t o = q * (int32_t)kT + (int32_t)t;
                 if (o < len) stk(dst + d0 + o, sh->out[o]);
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
