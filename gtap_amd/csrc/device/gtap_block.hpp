@@ -272,7 +272,7 @@ __device__ __forceinline__ uint32_t block_pop(const GtapDev& dev, BlockCtx* c, u
 // reaches idle workgroups in a logarithmic number of steal rounds instead of one steal per
 // task from the spawner. Claimed slots are read after the CAS; the owner never writes
 // within kMargin of the top it last saw, so they cannot be overwritten first.
-constexpr uint32_t kBlockStealMax = 16;
+constexpr uint32_t kBlockStealMax = 64;
 __device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me, BlockCtx* c) {
     BlockLoop& lp = c->lp;
     if (dev.nworkers < 2) return 0;
@@ -300,19 +300,22 @@ __device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me,
         lp.steals += 1;
         const uint32_t first = ld32(dq_slot_ptr(dev, 0, victim, t));
         if (k > 1) {
-            uint32_t ids[kBlockStealMax];
-#pragma unroll
-            for (uint32_t i = 1; i < kBlockStealMax; ++i)
-                if (i < k) ids[i] = ld32(dq_slot_ptr(dev, 0, victim, t + i));
             uint32_t pos = c->tail;
+            for (uint32_t i0 = 1; i0 < k; i0 += 16) {  // 16 slot loads in flight at a time
+                uint32_t ids[16];
 #pragma unroll
-            for (uint32_t i = 1; i < kBlockStealMax; ++i) {
-                if (i < k) {
-                    if (pos + 1u - c->top_cache > dev.cap - kMargin) {
-                        GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, ids[i], 0, pos + 1u - c->top_cache, dev.cap - kMargin);
-                    } else {
-                        st32(dq_slot_ptr(dev, 0, me, pos), ids[i]);
-                        pos += 1;
+                for (uint32_t i = 0; i < 16; ++i)
+                    if (i0 + i < k) ids[i] = ld32(dq_slot_ptr(dev, 0, victim, t + i0 + i));
+#pragma unroll
+                for (uint32_t i = 0; i < 16; ++i) {
+                    if (i0 + i < k) {
+                        if (pos + 1u - c->top_cache > dev.cap - kMargin) {
+                            GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, ids[i], 0, pos + 1u - c->top_cache,
+                                              dev.cap - kMargin);
+                        } else {
+                            st32(dq_slot_ptr(dev, 0, me, pos), ids[i]);
+                            pos += 1;
+                        }
                     }
                 }
             }
