@@ -559,19 +559,8 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
     }
 }
 
-// A program may ask for a minimum number of waves per SIMD (P::kWavesPerSimd): the compiler
-// then keeps the kernel's registers within that occupancy.
-template <class P, class = void>
-struct BlockWavesPerSimd {
-    static constexpr int value = 1;
-};
 template <class P>
-struct BlockWavesPerSimd<P, decltype((void)P::kWavesPerSimd)> {
-    static constexpr int value = P::kWavesPerSimd;
-};
-
-template <class P>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BlockWavesPerSimd<P>::value)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<P>::value)))
 void block_kernel(GtapDev dev, typename P::Args args) {
     __shared__ BlockCtx ctx;
     __shared__ typename P::Shared sh;

@@ -186,6 +186,17 @@ __device__ __forceinline__ bool atomic_cas(T* p, T& expected, T desired) {
 // sc1 stores are visible device-wide, so the next atomic may publish them.
 __device__ __forceinline__ void complete_stores() { __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// A program may ask for a minimum number of waves per SIMD (P::kWavesPerSimd): the compiler
+// then keeps the kernel's registers within that occupancy, spilling if it must.
+template <class P, class = void>
+struct WavesPerSimd {
+    static constexpr int value = 1;
+};
+template <class P>
+struct WavesPerSimd<P, decltype((void)P::kWavesPerSimd)> {
+    static constexpr int value = P::kWavesPerSimd;
+};
+
 // Workgroup barrier that orders LDS accesses only. __syncthreads() also waits for every global
 // load and store of the calling wave to complete (a workgroup-scope fence over all address
 // spaces); this one leaves them in flight, so a tile loop can keep its prefetches and its

@@ -68,7 +68,7 @@ struct HasWaveQueue<P, decltype((void)P::kWaveQueue)> {
 // The wave queue (run on a whole wave) keeps its records in HBM: QL = queues with
 // descriptors. A descriptor is { word0, parent, result offset, frame[0..FW) } padded to an
 // odd word count (LDS bank spread).
-template <int Q, int QL, int D, int S>
+template <int Q, int QL, int D, int S, int NS>
 struct WaveCtx {
     uint32_t free_top;
     uint32_t priv_top;
@@ -87,9 +87,20 @@ struct WaveCtx {
     // records written out together after a round (suspended tasks, published entries): the
     // 16-byte header and S/4 - 1 frame pieces each, spread over the lanes so that the pieces
     // of one record leave in one store instruction and merge into one memory transaction
-    uint32_t stage[2 * kWave][S];
-    uint32_t stage_id[2 * kWave];
+    uint32_t stage[NS][S];
+    uint32_t stage_id[NS];
     uint32_t nstage;        // records staged so far this round (entries pushed out of the window, <= 64)
+};
+
+// A program whose tasks never wait for children (P::kNoTaskwait) never suspends a task, so
+// a round stages at most the 64 records pushed out of the window: half the staging area.
+template <class P, class = void>
+struct NoTaskwait {
+    static constexpr bool value = false;
+};
+template <class P>
+struct NoTaskwait<P, decltype((void)P::kNoTaskwait)> {
+    static constexpr bool value = P::kNoTaskwait;
 };
 
 template <class P>
@@ -104,7 +115,8 @@ struct CtxShape {
     static_assert(kFWr <= 12, "at most 12 frame words are loaded with a task (one 64-byte line)");
     static_assert(P::kFrameBytes >= 4 * kFWr, "the frame must cover the loaded frame words");
     static constexpr int kD = (3 + kFW) | 1;
-    using type = WaveCtx<kQ, (kQL > 0 ? kQL : 1), kD, 4 * kPieces>;
+    static constexpr int kStage = NoTaskwait<P>::value ? kWave : 2 * kWave;  // evicted (+ suspended) records
+    using type = WaveCtx<kQ, (kQL > 0 ? kQL : 1), kD, 4 * kPieces, kStage>;
 };
 template <class P>
 using WaveCtxOf = typename CtxShape<P>::type;
@@ -585,6 +597,7 @@ struct Lane {
     // the other suspended tasks of the wave.
     // (reference: __gtap_set_state_for_join, gtap_runtime_thread.cuh:672-696)
     __device__ __forceinline__ bool taskwait(uint32_t next_state, uint32_t q_after) {
+        static_assert(!NoTaskwait<P>::value, "a program declared kNoTaskwait calls taskwait");
         if (q_after >= (uint32_t)Q) {
             GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_QUEUE_IDX_AFTER_JOIN, tid, q_after, q_after, Q);
             q_after = 0;
@@ -660,6 +673,7 @@ struct WaveTask {
         if (lane == 0) spawn_record<P>(*dev, w, me, tid, f, q, kNoParent, 0u, a);
     }
     __device__ __forceinline__ bool taskwait(uint32_t next_state, uint32_t q_after) {
+        static_assert(!NoTaskwait<P>::value, "a program declared kNoTaskwait calls taskwait");
         if (q_after >= (uint32_t)Q) q_after = 0;
         if (child_count == 0) {
             state = next_state;
@@ -1350,7 +1364,8 @@ struct WaveSharedOf<P, decltype((void)sizeof(typename P::WaveShared))> {
 };
 
 template <class P>
-__global__ __launch_bounds__(256) void thread_kernel(GtapDev dev, typename P::Args args) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<P>::value)))
+void thread_kernel(GtapDev dev, typename P::Args args) {
     __shared__ WaveCtxOf<P> ctxs[4];
     __shared__ typename WaveSharedOf<P>::type wsh[4];
     thread_scheduler<P>(dev, args, &ctxs[threadIdx.x / kWave], &wsh[threadIdx.x / kWave]);

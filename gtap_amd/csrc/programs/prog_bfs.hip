@@ -55,6 +55,8 @@ __host__ __device__ __forceinline__ uint32_t degree_bucket(int32_t deg) {
 template <int NQ>
 struct Program {
     static constexpr int kQueues = NQ;
+    static constexpr bool kNoTaskwait = true;  // detached spawns only
+    static constexpr int kWavesPerSimd = 3;    // 3072 wavefront workers with 4 queues too
     static constexpr int kFrameWords = 4;
     static constexpr uint32_t kMaxFunc = F_BFS;
     static constexpr uint32_t kFrameBytes = 16;
