@@ -56,7 +56,7 @@ template <int NQ>
 struct Program {
     static constexpr int kQueues = NQ;
     static constexpr bool kNoTaskwait = true;  // detached spawns only
-    static constexpr int kWavesPerSimd = 3;    // 3072 wavefront workers with 4 queues too
+    static constexpr int kWavesPerSimd = 3;    // 3072 wavefront workers with 4 queues too (4 per SIMD measured slower)
     static constexpr int kFrameWords = 4;
     static constexpr uint32_t kMaxFunc = F_BFS;
     static constexpr uint32_t kFrameBytes = 16;
