@@ -82,7 +82,8 @@ class Config(ctypes.Structure):
         ("timeout_ms", ctypes.c_uint32),
         ("block_workers", ctypes.c_uint32),
         ("idle_sleep", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 5),
+        ("scheduler", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 4),
     ]
 
 

@@ -87,6 +87,9 @@ struct GtapDev {
     uint32_t max_live_jobs;    // admission cap on concurrently running jobs
     uint32_t prof_max;         // samples per worker; 0 = profiling off
     uint32_t idle_level;       // deepest idle backoff level (1..6: s_sleep 1 .. 32)
+    uint32_t gq_cap;           // global-queue scheduler: ring slots per EPAQ queue (power of two)
+    uint32_t* gq_slots;        // global-queue scheduler: [queue][gq_cap] task ids
+    uint64_t* gq_ctl;          // global-queue scheduler: [queue][head, reserved, committed] * 16
 };
 
 // Timeline sample (profiling on): ns since an arbitrary epoch, from the 100 MHz wall clock.

@@ -882,8 +882,10 @@ __device__ __forceinline__ void complete_job(const GtapDev& dev, uint32_t anchor
 // Try to steal from random victims on queue q. Returns the number of ids stolen into `tid_out`.
 template <class P>
 __device__ __forceinline__ uint32_t try_steal(const GtapDev& dev, uint32_t me, uint32_t q, uint32_t& rng,
-                                              uint32_t& tid_out, bool& idle_registered, LoopStats& st) {
-    const uint32_t batch_cap = ((int)q == HasWaveQueue<P>::value) ? 1u : dev.steal_batch;
+                                              uint32_t& tid_out, bool& idle_registered, LoopStats& st,
+                                              uint32_t max_items = kWave) {
+    uint32_t batch_cap = ((int)q == HasWaveQueue<P>::value) ? 1u : dev.steal_batch;
+    if (batch_cap > max_items) batch_cap = max_items;
     const int lane = lane_id();
     if (dev.nworkers < 2) return 0;
     uint32_t victim = 0;
@@ -928,7 +930,7 @@ __device__ __forceinline__ uint32_t try_steal(const GtapDev& dev, uint32_t me, u
 // Pop up to 64 of this wave's newest published tasks from queue q.
 template <class P>
 __device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t q,
-                                            uint32_t& tid_out) {
+                                            uint32_t& tid_out, uint32_t max_items = kWave) {
     const int lane = lane_id();
     uint64_t* wp = dq_word_ptr(dev, q, me);
     // A failed CAS means a thief took tasks, so this ends; the owner leaves with a task or an
@@ -941,7 +943,8 @@ __device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtxOf<P>* w,
         const uint32_t t = (uint32_t)(wv >> 32), b = (uint32_t)wv;
         const int32_t sz = (int32_t)(b - t);
         if (sz <= 0) return 0;
-        const uint32_t cap = ((int)q == HasWaveQueue<P>::value) ? 1u : (uint32_t)kWave;
+        uint32_t cap = ((int)q == HasWaveQueue<P>::value) ? 1u : (uint32_t)kWave;
+        if (cap > max_items) cap = max_items;
         const uint32_t k = (uint32_t)sz < cap ? (uint32_t)sz : cap;
         int ok = 0;
         if (lane == 0) {
@@ -959,6 +962,129 @@ __device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtxOf<P>* w,
     }
 }
 
+// ------------------------------------------------------------------ other schedulers
+// The alternatives the reference compares its batched work stealing with
+// (runtime/detail/experimental/thread/gtap_runtime_thread_gq.cuh:484-621,
+// gtap_runtime_thread_chaselev.cuh:516-712), kept for the worker-scaling comparison:
+//   kSchedGQ  one global FIFO of task ids per EPAQ queue. A wave claims up to 64 ids from
+//             its head with one CAS. A publishing wave reserves a range with a fetch-add,
+//             writes the ids, then waits until every earlier range is committed and
+//             advances the committed tail (in-order commit).
+//   kSchedCL  the owner pops and thieves steal one task per CAS; a wave fills its batch
+//             with up to 64 such single-item operations in a row.
+enum : int { kSchedWS = 0, kSchedGQ = 1, kSchedCL = 2 };
+constexpr uint32_t kGqMargin = 1024;  // ring slots kept free beyond the head
+
+// Control words of global queue q: head, reservation counter and committed tail, each on its
+// own 128-byte line.
+__device__ __forceinline__ uint64_t* gq_word(const GtapDev& dev, uint32_t q, uint32_t which) {
+    return dev.gq_ctl + ((size_t)q * 3 + which) * 16;
+}
+__device__ __forceinline__ uint32_t* gq_slot(const GtapDev& dev, uint32_t q, uint64_t pos) {
+    return dev.gq_slots + (size_t)q * dev.gq_cap + (pos & (uint64_t)(dev.gq_cap - 1));
+}
+
+// Claim up to 64 (1 for the wave queue) committed ids from the head of global queue q.
+template <class P>
+__device__ __forceinline__ uint32_t gq_pop(const GtapDev& dev, uint32_t q, uint32_t& tid_out, bool& idle_registered,
+                                           LoopStats& st) {
+    const int lane = lane_id();
+    const uint32_t cap = ((int)q == HasWaveQueue<P>::value) ? 1u : (uint32_t)kWave;
+    uint64_t* headp = gq_word(dev, q, 0);
+    uint64_t* tailp = gq_word(dev, q, 2);
+    for (int tries = 0; tries < 8; ++tries) {
+        uint64_t h = 0, tl = 0;
+        if (lane == 0) {
+            h = ld64(headp);
+            tl = ld64(tailp);
+        }
+        h = shfl(h, 0);
+        tl = shfl(tl, 0);
+        if ((int64_t)(tl - h) <= 0) return 0;
+        const uint32_t k = (tl - h) < (uint64_t)cap ? (uint32_t)(tl - h) : cap;
+        if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
+            if (lane == 0) atomic_add(dev.ctl + CTL_IDLE * 32, -1);
+            idle_registered = false;
+        }
+        int ok = 0;
+        if (lane == 0) {
+            uint64_t expected = h;
+            ok = atomic_cas(headp, expected, h + k) ? 1 : 0;
+        }
+        ok = shfl((int32_t)ok, 0);
+        st.attempts += 1;
+        if (ok) {
+            if ((uint32_t)lane < k) tid_out = ld32(gq_slot(dev, q, h + (uint32_t)lane));
+            st.steals += 1;
+            return k;
+        }
+    }
+    return 0;
+}
+
+// Append the ids at positions [b, t) of this wave's own slot array for queue q (all written
+// and completed) to global queue q: reserve, copy, wait for the earlier reservations to
+// commit, commit. The own array then holds nothing published: its top moves to t.
+template <class P>
+__device__ __forceinline__ void gq_push(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t q, uint32_t b,
+                                        uint32_t t) {
+    const int lane = lane_id();
+    const uint32_t n = t - b;
+    uint64_t base = 0;
+    int overflow = 0;
+    if (lane == 0) {
+        base = atomic_add(gq_word(dev, q, 1), (uint64_t)n);
+        const uint64_t head = ld64(gq_word(dev, q, 0));
+        overflow = (base + n - head > (uint64_t)(dev.gq_cap - kGqMargin)) ? 1 : 0;
+        if (overflow) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, -1, q, (int)(base + n - head), dev.gq_cap - kGqMargin);
+    }
+    base = shfl(base, 0);
+    overflow = shfl((int32_t)overflow, 0);
+    if (!overflow) {
+        for (uint32_t j = (uint32_t)lane; j < n; j += kWave) st32(gq_slot(dev, q, base + j), ld32(dq_slot_ptr(dev, q, me, b + j)));
+        complete_stores();  // the ids are in the ring before the range is committed
+        if (lane == 0) {
+            // in-order commit: every earlier reservation commits in bounded time (its owner
+            // only copies ids between reserving and committing); an abort ends the wait
+            uint64_t* tailp = gq_word(dev, q, 2);
+            for (uint32_t spin = 0;; ++spin) {
+                if (ld64(tailp) == base) {
+                    atomic_add_noret(tailp, (uint64_t)n);
+                    break;
+                }
+                if ((spin & 255u) == 255u && ld32(dev.ctl + CTL_ABORT * 32)) break;
+                if (spin > (1u << 24)) {
+                    GTAP_DEVICE_ERROR(dev, GTAP_ERROR_WATCHDOG, -1, q, (int)base, (int)n);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+    }
+    if (lane == 0) {
+        // nothing of [b, t) stays in the own array: its top and bottom move to t
+        st64(dq_word_ptr(dev, q, me), ((uint64_t)t << 32) | (uint64_t)t);
+        vst(&w->top_cache[q], t);
+    }
+}
+
+// Single-item Chase-Lev pops: lane k receives the k-th task popped, one CAS per task.
+template <class P>
+__device__ __forceinline__ uint32_t cl_pop_batch(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t q,
+                                                 uint32_t& tid_out) {
+    const int lane = lane_id();
+    const uint32_t cap = ((int)q == HasWaveQueue<P>::value) ? 1u : (uint32_t)kWave;
+    uint32_t k = 0;
+    while (k < cap) {
+        uint32_t one = 0;
+        if (!try_pop<P>(dev, w, me, q, one, 1u)) break;
+        one = shfl(one, 0);
+        if ((uint32_t)lane == k) tid_out = one;
+        ++k;
+    }
+    return k;
+}
+
 __device__ __forceinline__ void backoff_sleep(uint32_t level) {
     switch (level) {
         case 0: break;
@@ -971,7 +1097,7 @@ __device__ __forceinline__ void backoff_sleep(uint32_t level) {
     }
 }
 
-template <class P>
+template <class P, int Sched>
 __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& args, WaveCtxOf<P>* w,
                                  void* wave_shared) {
     constexpr int Q = P::kQueues;
@@ -1033,9 +1159,10 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             my_lds = false;
             const uint32_t cq = uniform(vld(&w->cur_queue));
 #pragma unroll
-            for (int i = 0; i < Q && n == 0 && idle_iter == 0; ++i) {
+            for (int i = 0; i < Q && n == 0 && idle_iter == 0 && Sched != kSchedGQ; ++i) {
                 const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
-                n = try_pop<P>(dev, w, me, q, my_tid);
+                if constexpr (Sched == kSchedCL) n = cl_pop_batch<P>(dev, w, me, q, my_tid);
+                else n = try_pop<P>(dev, w, me, q, my_tid);
                 if (n) {
                     vst(&w->cur_queue, q);
                     st.pops += 1;
@@ -1051,7 +1178,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
 #pragma unroll
                 for (int i = 0; i < Q && n == 0; ++i) {
                     const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
-                    n = try_steal<P>(dev, me, q, rng, my_tid, idle_registered, st);
+                    if constexpr (Sched == kSchedGQ) n = gq_pop<P>(dev, q, my_tid, idle_registered, st);
+                    else n = try_steal<P>(dev, me, q, rng, my_tid, idle_registered, st, Sched == kSchedCL ? 1u : (uint32_t)kWave);
                     if (n) batch_q = (int)q;
                 }
             }
@@ -1256,7 +1384,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         for (int q = 0; q < Q; ++q) {
             const uint32_t t = uniform(vld(&w->tail[q]));
             const uint32_t b = uniform(vld(&w->bottom[q]));
-            const bool retain = Q > 1 && q < QL && (uint32_t)q != kmax;
+            const bool retain = Sched != kSchedGQ && Q > 1 && q < QL && (uint32_t)q != kmax;
             pub_to[q] = !retain ? t : ((int32_t)(t - b) > kWave ? t - kWave : b);
             if (retain && pub_to[q] != b) any_publish = true;
             if (!retain && t != b) {
@@ -1306,8 +1434,12 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 const uint32_t b = uniform(vld(&w->bottom[q]));
                 const uint32_t t = pub_to[q];
                 if (t != b) {
-                    // top_cache stays a lower bound of the top; the capacity check re-reads it
-                    if (lane == 0) atomic_add_noret(dq_word_ptr(dev, q, me), (uint64_t)(t - b));
+                    if constexpr (Sched == kSchedGQ) {
+                        gq_push<P>(dev, w, me, (uint32_t)q, b, t);
+                    } else {
+                        // top_cache stays a lower bound of the top; the capacity check re-reads it
+                        if (lane == 0) atomic_add_noret(dq_word_ptr(dev, q, me), (uint64_t)(t - b));
+                    }
                     vst(&w->bottom[q], t);
                 }
             }
@@ -1363,12 +1495,12 @@ struct WaveSharedOf<P, decltype((void)sizeof(typename P::WaveShared))> {
     using type = typename P::WaveShared;
 };
 
-template <class P>
+template <class P, int Sched = kSchedWS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<P>::value)))
 void thread_kernel(GtapDev dev, typename P::Args args) {
     __shared__ WaveCtxOf<P> ctxs[4];
     __shared__ typename WaveSharedOf<P>::type wsh[4];
-    thread_scheduler<P>(dev, args, &ctxs[threadIdx.x / kWave], &wsh[threadIdx.x / kWave]);
+    thread_scheduler<P, Sched>(dev, args, &ctxs[threadIdx.x / kWave], &wsh[threadIdx.x / kWave]);
 }
 
 }  // namespace gtap

@@ -92,6 +92,14 @@ typedef struct gtap_run_stats {
     int32_t reserved;
 } gtap_run_stats;
 
+/* Schedulers of gtap_config.scheduler. */
+enum {
+    GTAP_SCHED_DEFAULT = 0,
+    GTAP_SCHED_WORK_STEALING = 1,
+    GTAP_SCHED_GLOBAL_QUEUE = 2,
+    GTAP_SCHED_CHASE_LEV = 3
+};
+
 /* Tunables of the scheduler (worker count, deque capacity, steal batch width, ...).
  * Zero fields take the defaults. */
 typedef struct gtap_config {
@@ -107,7 +115,10 @@ typedef struct gtap_config {
     uint32_t block_workers;   /* workgroups of the block-mode scheduler (default: co-resident) */
     uint32_t idle_sleep;      /* longest backoff of an idle wave between polls, in units of 64
                                  cycles: 1, 2, 4, 8, 16 or 32 (default 32) */
-    uint32_t reserved[5];
+    uint32_t scheduler;       /* thread mode: 0 or 1 = batched work stealing (default), 2 = one
+                                 global queue per EPAQ queue, 3 = single-item Chase-Lev pops and
+                                 steals (programs built with the alternatives: fib, n-queens) */
+    uint32_t reserved[4];
 } gtap_config;
 
 #ifdef __cplusplus

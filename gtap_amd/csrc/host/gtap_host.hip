@@ -104,6 +104,9 @@ GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job
     d.admit_reserve = 0;
     d.max_live_jobs = R.nanchors;
     d.prof_max = R.prof_max;
+    d.gq_cap = R.gq_cap;
+    d.gq_slots = R.gq_slots;
+    d.gq_ctl = R.gq_ctl;
     d.idle_level = 1;
     for (uint32_t v = R.cfg.idle_sleep; v > 1 && d.idle_level < 6; v >>= 1) d.idle_level += 1;
     d.prof_having = R.prof_having;
@@ -123,6 +126,8 @@ __global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0) {
     for (uint64_t i = gid; i < (uint64_t)dev.nshards * dev.ncells; i += gs)
         dev.pool_seq[i] = (uint32_t)(i & (dev.ncells - 1));
     for (uint64_t i = gid; i < (uint64_t)dev.nworkers * ST_COUNT; i += gs) dev.stats[i] = 0;
+    if (dev.gq_ctl)
+        for (uint64_t i = gid; i < (uint64_t)dev.nqueues * 3 * 16; i += gs) dev.gq_ctl[i] = 0;
     if (dev.prof_max) {
         for (uint64_t i = gid; i < (uint64_t)dev.nworkers * dev.prof_max; i += gs) {
             dev.prof_having[i] = 0;
@@ -190,6 +195,18 @@ int enqueue_summary(const GtapDev& dev, hipStream_t stream, bool copy_job0) {
     hipLaunchKernelGGL(summary_kernel, dim3(1), dim3(kSummaryThreads), 0, stream, dev, R.summary_dev,
                        copy_job0 ? 1 : 0);
     GTAP_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+constexpr uint32_t kGlobalQueueSlots = 1u << 24;  // per EPAQ queue
+
+int ensure_global_queues() {
+    Runtime& R = rt();
+    if (R.gq_slots) return 0;
+    GTAP_HIP_TRY(hipMalloc(&R.gq_slots, (size_t)R.nqueues * kGlobalQueueSlots * sizeof(uint32_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.gq_ctl, (size_t)R.nqueues * 3 * 16 * sizeof(uint64_t)));
+    GTAP_HIP_TRY(hipMemset(R.gq_ctl, 0, (size_t)R.nqueues * 3 * 16 * sizeof(uint64_t)));
+    R.gq_cap = kGlobalQueueSlots;
     return 0;
 }
 
@@ -272,6 +289,8 @@ static void free_all() {
     dfree(R.prof_having);
     dfree(R.prof_working);
     dfree(R.prof_counts);
+    dfree(R.gq_slots);
+    dfree(R.gq_ctl);
     if (R.host_ctl) (void)hipHostFree(R.host_ctl);
     if (R.report) (void)hipHostFree(R.report);
     if (R.summary) (void)hipHostFree(R.summary);
@@ -304,6 +323,7 @@ static void apply_defaults(gtap_config* c) {
     if (!c->max_jobs) c->max_jobs = 65536;
     if (!c->timeout_ms) c->timeout_ms = 20000;
     if (!c->idle_sleep || c->idle_sleep > 32) c->idle_sleep = 32;
+    if (c->scheduler == GTAP_SCHED_DEFAULT) c->scheduler = GTAP_SCHED_WORK_STEALING;
 }
 
 extern "C" {
@@ -311,6 +331,7 @@ extern "C" {
 int gtap_set_config(const gtap_config* cfg) {
     Runtime& R = rt();
     if (R.initialized) return invalid_argument("gtap_set_config must precede gtap_initialize");
+    if (cfg && cfg->scheduler > GTAP_SCHED_CHASE_LEV) return invalid_argument("gtap_config.scheduler must be 0..3");
     if (cfg) R.cfg = *cfg;
     else memset(&R.cfg, 0, sizeof(R.cfg));
     return 0;

@@ -76,6 +76,10 @@ struct Runtime {
     bool stats_host_valid = false;
     LaunchSummary* summary = nullptr;  // pinned, device-mapped
     LaunchSummary* summary_dev = nullptr;
+    // global-queue scheduler (allocated on first use)
+    uint32_t* gq_slots = nullptr;
+    uint64_t* gq_ctl = nullptr;
+    uint32_t gq_cap = 0;
     // timeline profiling (gtap_set_profiling)
     uint32_t prof_max = 0;
     uint64_t* prof_having = nullptr;
@@ -114,7 +118,7 @@ int occupancy_blocks(const void* kernel, uint32_t requested, uint32_t* blocks_ou
 // Persistent launch of a thread-mode program over `njobs` jobs of the job table `jobs`.
 // Checks a thread-mode launch against the provisioned runtime and picks its workgroup count
 // (every co-resident workgroup unless gtap_config.blocks asks for fewer).
-template <class P>
+template <class P, int Sched = kSchedWS>
 int thread_launch_shape(uint32_t job_ring, uint32_t* blocks_out) {
     Runtime& R = rt();
     if (!R.initialized) {
@@ -125,7 +129,7 @@ int thread_launch_shape(uint32_t job_ring, uint32_t* blocks_out) {
     if (P::kFrameBytes + kHdrBytes > R.stride) return invalid_argument("task frame exceeds the record stride");
     if (job_ring == 0 || job_ring > R.nanchors) return invalid_argument("job ring exceeds the anchor capacity");
     static uint32_t max_blocks = 0;
-    const void* kfn = reinterpret_cast<const void*>(&thread_kernel<P>);
+    const void* kfn = reinterpret_cast<const void*>(&thread_kernel<P, Sched>);
     if (max_blocks == 0) {
         int st = occupancy_blocks(kfn, 0, &max_blocks);
         if (st) return st;
@@ -137,24 +141,58 @@ int thread_launch_shape(uint32_t job_ring, uint32_t* blocks_out) {
     return 0;
 }
 
+// The ring of the global-queue scheduler, allocated on first use.
+int ensure_global_queues();
+
+// A program that is also built with the alternative schedulers (P::kSchedVariants).
+template <class P, class = void>
+struct HasSchedVariants {
+    static constexpr bool value = false;
+};
 template <class P>
-int launch_thread(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs,
-                  uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream = nullptr,
-                  const gtap_job* job0 = nullptr) {
+struct HasSchedVariants<P, decltype((void)P::kSchedVariants)> {
+    static constexpr bool value = P::kSchedVariants;
+};
+
+template <class P, int Sched>
+int launch_thread_as(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs,
+                     uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream, const gtap_job* job0) {
     Runtime& R = rt();
     uint32_t blocks = 0;
-    if (int st = thread_launch_shape<P>(job_ring, &blocks)) return st;
+    if (int st = thread_launch_shape<P, Sched>(job_ring, &blocks)) return st;
+    if (Sched == kSchedGQ) {
+        if (int st = ensure_global_queues()) return st;
+    }
     if (!stream) stream = R.stream;
     GtapDev dev = make_dev(blocks * 4, term_mode, njobs, jobs, job_ring);
     int st = reset_launch_state(dev, stream, job0);
     if (st) return st;
     GTAP_HIP_TRY(hipEventRecord(R.ev0, stream));
-    hipLaunchKernelGGL(thread_kernel<P>, dim3(blocks), dim3(256), 0, stream, dev, args);
+    hipLaunchKernelGGL((thread_kernel<P, Sched>), dim3(blocks), dim3(256), 0, stream, dev, args);
     GTAP_HIP_TRY(hipGetLastError());
     GTAP_HIP_TRY(hipEventRecord(R.ev1, stream));
     if ((st = enqueue_summary(dev, stream, job0 != nullptr))) return st;
     GTAP_HIP_TRY(hipStreamSynchronize(stream));
     return finish_launch(dev, blocks, stats);
+}
+
+// Launches with the scheduler gtap_config.scheduler names (batched work stealing unless a
+// program built with the alternatives asks for one of them).
+template <class P>
+int launch_thread(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs,
+                  uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream = nullptr,
+                  const gtap_job* job0 = nullptr) {
+    const uint32_t sched = rt().cfg.scheduler;
+    if constexpr (HasSchedVariants<P>::value) {
+        if (sched == GTAP_SCHED_GLOBAL_QUEUE)
+            return launch_thread_as<P, kSchedGQ>(args, term_mode, njobs, jobs, job_ring, stats, stream, job0);
+        if (sched == GTAP_SCHED_CHASE_LEV)
+            return launch_thread_as<P, kSchedCL>(args, term_mode, njobs, jobs, job_ring, stats, stream, job0);
+    } else {
+        if (sched > GTAP_SCHED_WORK_STEALING)
+            return invalid_argument("this program is built for the work-stealing scheduler only");
+    }
+    return launch_thread_as<P, kSchedWS>(args, term_mode, njobs, jobs, job_ring, stats, stream, job0);
 }
 
 // A thread-mode launch held as an instantiated HIP graph: [copy of the job table from a

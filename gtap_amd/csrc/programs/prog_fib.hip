@@ -32,6 +32,7 @@ constexpr int kOffB = 16;
 
 struct Program {
     static constexpr int kQueues = 1;
+    static constexpr bool kSchedVariants = true;  // also built with the global-queue and Chase-Lev schedulers
     static constexpr int kFrameWords = 2;  // n (+pad)
     static constexpr uint32_t kMaxFunc = F_FIB;
     static constexpr uint32_t kFrameBytes = 24;

@@ -42,6 +42,7 @@ constexpr int kMaxN = 16;
 
 struct Program {
     static constexpr int kQueues = 1;
+    static constexpr bool kSchedVariants = true;  // also built with the global-queue and Chase-Lev schedulers
     static constexpr int kFrameWords = 6;  // row, cols, diag, anti, k, pad
     static constexpr uint32_t kMaxFunc = F_NQ;
     static constexpr uint32_t kFrameBytes = kOffCounts + 8 * kMaxN;

@@ -51,3 +51,20 @@ def test_struct_sizes_match_c_layout():
     assert ctypes.sizeof(gtap_amd.Job) == 128
     assert ctypes.sizeof(gtap_amd.ErrorReport) == 36
     assert ctypes.sizeof(gtap_amd.RunStats) == 8 * 12 + 16
+    assert ctypes.sizeof(gtap_amd.Config) == 8 + 4 * 16
+
+
+def test_config_fields_and_scheduler_check_without_gpu():
+    """gtap_set_config stores the configuration (no device call before gtap_initialize);
+    gtap_get_config fills the defaults; an unknown scheduler is refused."""
+    if not os.path.exists(gtap_amd.LIB_PATH):
+        pytest.skip("libgtap_amd.so not built")
+    names = [f for f, _ in gtap_amd.Config._fields_]
+    assert names[-3:] == ["idle_sleep", "scheduler", "reserved"]
+    assert gtap_amd.gtap_set_config(scheduler=9) != 0
+    assert gtap_amd.gtap_set_config(scheduler=2, idle_sleep=4) == 0
+    c = gtap_amd.gtap_get_config()
+    assert c["scheduler"] == 2 and c["idle_sleep"] == 4
+    assert gtap_amd.gtap_set_config() == 0
+    c = gtap_amd.gtap_get_config()
+    assert c["scheduler"] == 1 and c["idle_sleep"] == 32 and c["probe_lanes"] == 8
