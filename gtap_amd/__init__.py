@@ -169,6 +169,22 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_csr_from_edges": ([ctypes.c_int32, _i64, _vp, _vp, _vp, _vp], _i),
     "gtap_rmat_csr": ([_i, _i, _u64, _u32, _u32, _u32, _vp, _vp], _i),
     "gtap_gen_keys": ([_u64, _u64, _vp], _i),
+    "gtap_spmv": ([_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(RunStats)], _i),
+    "gtap_spmv_host": ([_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                        ctypes.POINTER(RunStats)], _i),
+    "gtap_spmv_random": ([ctypes.c_int32, ctypes.c_int32, _u64, _vp, _vp, _vp, ctypes.POINTER(_i64)], _i),
+    "gtap_tree": ([ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _i64, _vp, ctypes.POINTER(RunStats)], _i),
+    "gtap_tree_input": ([_u64, _i64, _vp], _i),
+    "gtap_csr_file_info": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_i64)], _i),
+    "gtap_csr_file_read": ([ctypes.c_char_p, _vp, _vp], _i),
+    "gtap_csr_file_write": ([ctypes.c_char_p, ctypes.c_int32, _i64, _vp, _vp], _i),
+    "gtap_keys_file_info": ([ctypes.c_char_p, ctypes.POINTER(_u64)], _i),
+    "gtap_keys_file_read": ([ctypes.c_char_p, _vp], _i),
+    "gtap_keys_file_write": ([ctypes.c_char_p, _u64, _vp], _i),
+    "gtap_spmv_file_info": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                             ctypes.POINTER(ctypes.c_int32)], _i),
+    "gtap_spmv_file_read": ([ctypes.c_char_p, _vp, _vp, _vp], _i),
+    "gtap_spmv_file_write": ([ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp], _i),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -395,6 +411,128 @@ class DeviceBuffer:
         if self.ptr:
             lib().gtap_device_free(self.ptr)
             self.ptr = None
+
+
+# ---------------------------------------------------------------- SpMV and the binary tree
+
+def spmv_random(rows: int, avg_nnz: int = 300, seed: int = 42):
+    """(row_ptr, col_idx, val) of the library's random CSR matrix (rows x rows)."""
+    import numpy as np
+    row_ptr = np.empty(rows + 1, dtype=np.int32)
+    nnz = ctypes.c_int64(0)
+    _check(lib().gtap_spmv_random(rows, avg_nnz, seed, row_ptr.ctypes.data, None, None, ctypes.byref(nnz)),
+           "gtap_spmv_random")
+    col = np.empty(nnz.value, dtype=np.int32)
+    val = np.empty(nnz.value, dtype=np.float64)
+    _check(lib().gtap_spmv_random(rows, avg_nnz, seed, row_ptr.ctypes.data, col.ctypes.data, val.ctypes.data,
+                                  ctypes.byref(nnz)), "gtap_spmv_random")
+    return row_ptr, col, val
+
+
+def spmv(row_ptr, col_idx, val, x, cutoff: int = 16):
+    """y = A x on the GPU (task per row, divide-and-conquer over row ranges); returns (y, stats)."""
+    import numpy as np
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    col_idx = np.ascontiguousarray(col_idx, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    rows = row_ptr.size - 1
+    y = np.empty(rows, dtype=np.float64)
+    s = RunStats()
+    _check(lib().gtap_spmv_host(row_ptr.ctypes.data, col_idx.ctypes.data if col_idx.size else None,
+                                val.ctypes.data if val.size else None, x.ctypes.data if x.size else None,
+                                y.ctypes.data if rows else None, rows, x.size, cutoff, ctypes.byref(s)),
+           "gtap_spmv_host")
+    return y, s.as_dict()
+
+
+def tree_input(n: int, seed: int = 0xC0FFEE):
+    """The binary-tree workload's input array (float64 in [-1, 1))."""
+    import numpy as np
+    out = np.empty(n, dtype=np.float64)
+    _check(lib().gtap_tree_input(seed, n, out.ctypes.data), "gtap_tree_input")
+    return out
+
+
+def tree(height: int, mem_ops: int, compute_iters: int, inp):
+    """The binary-tree workload on the GPU; returns (out: one float64 per node, stats)."""
+    import numpy as np
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    nodes = (1 << (height + 1)) - 1
+    din = DeviceBuffer.from_array(inp)
+    dout = DeviceBuffer(nodes * 8)
+    try:
+        s = RunStats()
+        _check(lib().gtap_tree(height, mem_ops, compute_iters, din.ptr, inp.size, dout.ptr, ctypes.byref(s)),
+               "gtap_tree")
+        out = dout.to_array(np.empty(nodes, dtype=np.float64))
+    finally:
+        din.free()
+        dout.free()
+    return out, s.as_dict()
+
+
+# ---------------------------------------------------------------- the reference's file formats
+
+def read_csr(path: str):
+    """(row, col) of a graph CSR file (int32 N, int32 M, int32 row[N+1], int32 col[M])."""
+    import numpy as np
+    nv, ne = ctypes.c_int32(0), ctypes.c_int64(0)
+    _check(lib().gtap_csr_file_info(path.encode(), ctypes.byref(nv), ctypes.byref(ne)), "gtap_csr_file_info")
+    row = np.empty(nv.value + 1, dtype=np.int32)
+    col = np.empty(ne.value, dtype=np.int32)
+    _check(lib().gtap_csr_file_read(path.encode(), row.ctypes.data, col.ctypes.data if ne.value else None),
+           "gtap_csr_file_read")
+    return row, col
+
+
+def write_csr(path: str, row, col) -> None:
+    import numpy as np
+    row = np.ascontiguousarray(row, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    _check(lib().gtap_csr_file_write(path.encode(), row.size - 1, col.size, row.ctypes.data,
+                                     col.ctypes.data if col.size else None), "gtap_csr_file_write")
+
+
+def read_keys(path: str):
+    """uint32 bit patterns of a key-vector file (size_t n, int32 data[n])."""
+    import numpy as np
+    n = ctypes.c_uint64(0)
+    _check(lib().gtap_keys_file_info(path.encode(), ctypes.byref(n)), "gtap_keys_file_info")
+    out = np.empty(n.value, dtype=np.uint32)
+    _check(lib().gtap_keys_file_read(path.encode(), out.ctypes.data if n.value else None), "gtap_keys_file_read")
+    return out
+
+
+def write_keys(path: str, keys) -> None:
+    import numpy as np
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    _check(lib().gtap_keys_file_write(path.encode(), keys.size, keys.ctypes.data if keys.size else None),
+           "gtap_keys_file_write")
+
+
+def read_spmv(path: str):
+    """(rows, cols, row_ptr, col_idx, val) of an SpMV CSR file."""
+    import numpy as np
+    r, c, z = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+    _check(lib().gtap_spmv_file_info(path.encode(), ctypes.byref(r), ctypes.byref(c), ctypes.byref(z)),
+           "gtap_spmv_file_info")
+    row_ptr = np.empty(r.value + 1, dtype=np.int32)
+    col = np.empty(z.value, dtype=np.int32)
+    val = np.empty(z.value, dtype=np.float64)
+    _check(lib().gtap_spmv_file_read(path.encode(), row_ptr.ctypes.data, col.ctypes.data if z.value else None,
+                                     val.ctypes.data if z.value else None), "gtap_spmv_file_read")
+    return r.value, c.value, row_ptr, col, val
+
+
+def write_spmv(path: str, cols: int, row_ptr, col_idx, val) -> None:
+    import numpy as np
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    col_idx = np.ascontiguousarray(col_idx, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    _check(lib().gtap_spmv_file_write(path.encode(), row_ptr.size - 1, cols, col_idx.size, row_ptr.ctypes.data,
+                                      col_idx.ctypes.data if col_idx.size else None,
+                                      val.ctypes.data if val.size else None), "gtap_spmv_file_write")
 
 
 # ---------------------------------------------------------------- sorting

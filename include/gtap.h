@@ -154,6 +154,47 @@ int gtap_stream_serve(gtap_stream* stream, double rate, uint64_t arrival_seed, i
  * forms above), in ms: the execution part of latency_ms. */
 int gtap_stream_exec_ms(gtap_stream* stream, double* exec_ms);
 
+/* ------------------------------------------------------------ more reference workloads */
+/* y = A x, CSR float64, divide-and-conquer over row ranges of at most `cutoff` rows (0 ->
+ * 16), one task per row, detached spawns (examples/spmv/spmv_thread.cu:34-66). Device
+ * buffers; the row sums are fma chains in column order. */
+int gtap_spmv(const int32_t* row_ptr, const int32_t* col_idx, const double* val, const double* x, double* y,
+              int32_t rows, int32_t cols, int32_t cutoff, gtap_run_stats* stats);
+/* The same on host buffers (copied in and out). */
+int gtap_spmv_host(const int32_t* row_ptr, const int32_t* col_idx, const double* val, const double* x, double* y,
+                   int32_t rows, int32_t cols, int32_t cutoff, gtap_run_stats* stats);
+/* A rows x rows random CSR matrix shaped like examples/spmv/spmv_thread.cu:89-107 (1 to
+ * 2*avg_nnz-1 column draws per row, sorted, deduplicated; values in [-1, 1)). Call once with
+ * col_idx = val = NULL for row_ptr and *nnz_out, then with buffers of that size. */
+int gtap_spmv_random(int32_t rows, int32_t avg_nnz, uint64_t seed, int32_t* row_ptr, int32_t* col_idx, double* val,
+                     int64_t* nnz_out);
+/* The binary-tree workload (examples/binary_tree/binary_tree_thread.cu:63-120): a node task
+ * spawns two children, waits, then reads mem_ops random input entries and runs
+ * compute_iters fma steps; out[node] receives its float64 result. Device buffers; input_n a
+ * power of two; 2^(height+1) - 1 nodes. */
+int gtap_tree(int32_t height, int32_t mem_ops, int32_t compute_iters, const double* input, int64_t input_n,
+              double* out, gtap_run_stats* stats);
+/* The tree's input: n float64 in [-1, 1) from `seed` (splitmix64). */
+int gtap_tree_input(uint64_t seed, int64_t n, double* out);
+
+/* ------------------------------------------------------------ the reference's file formats */
+/* Graph CSR (examples/asynchronous_bfs/asynchronous_bfs_thread.cu:69-84): int32 N, int32 M,
+ * int32 row[N+1], int32 col[M]. */
+int gtap_csr_file_info(const char* path, int32_t* nv, int64_t* ne);
+int gtap_csr_file_read(const char* path, int32_t* row, int32_t* col);
+int gtap_csr_file_write(const char* path, int32_t nv, int64_t ne, const int32_t* row, const int32_t* col);
+/* Key vector (evaluation/util/gen_vector.c:21-60): size_t n, int32 data[n] (read as uint32
+ * bit patterns). */
+int gtap_keys_file_info(const char* path, uint64_t* n);
+int gtap_keys_file_read(const char* path, uint32_t* keys);
+int gtap_keys_file_write(const char* path, uint64_t n, const uint32_t* keys);
+/* SpMV CSR (examples/spmv/spmv_thread.cu:70-86): int32 rows, cols, nnz, int32
+ * row_ptr[rows+1], int32 col_idx[nnz], float64 val[nnz]. */
+int gtap_spmv_file_info(const char* path, int32_t* rows, int32_t* cols, int32_t* nnz);
+int gtap_spmv_file_read(const char* path, int32_t* row_ptr, int32_t* col_idx, double* val);
+int gtap_spmv_file_write(const char* path, int32_t rows, int32_t cols, int32_t nnz, const int32_t* row_ptr,
+                         const int32_t* col_idx, const double* val);
+
 /* ------------------------------------------------------------ input generators (host) */
 /* R-MAT edges, CSR by stable counting sort, and the gen_vector key stream
  * (evaluation/util/gen_vector.c:8-18; read format examples/asynchronous_bfs/

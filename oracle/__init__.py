@@ -71,6 +71,10 @@ def lib() -> ctypes.CDLL:
         L.gtap_oracle_jobs.argtypes = [u64, i32, vp, vp, vp]
         L.gtap_oracle_hash_u32.argtypes = [vp, i64]
         L.gtap_oracle_hash_u32.restype = u64
+        L.gtap_oracle_spmv.argtypes = [i32, vp, vp, vp, vp, vp]
+        L.gtap_oracle_tree.argtypes = [i32, i32, i32, vp, i64, vp]
+        L.gtap_oracle_tree_node.argtypes = [i32, i32, i32, vp, i64]
+        L.gtap_oracle_tree_node.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -199,3 +203,26 @@ def job_result(kind: int, param: int, key_seed: int) -> int:
 def ref_binary(name: str) -> Optional[str]:
     p = os.path.join(REF_DIR, name)
     return p if os.path.exists(p) else None
+
+
+def spmv(row_ptr: np.ndarray, col_idx: np.ndarray, val: np.ndarray, x: np.ndarray) -> np.ndarray:
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    col_idx = np.ascontiguousarray(col_idx, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty(row_ptr.size - 1, dtype=np.float64)
+    lib().gtap_oracle_spmv(row_ptr.size - 1, _p(row_ptr), _p(col_idx) if col_idx.size else None,
+                           _p(val) if val.size else None, _p(x) if x.size else None, _p(y) if y.size else None)
+    return y
+
+
+def tree(height: int, mem_ops: int, compute_iters: int, inp: np.ndarray) -> np.ndarray:
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    out = np.empty((1 << (height + 1)) - 1, dtype=np.float64)
+    lib().gtap_oracle_tree(height, mem_ops, compute_iters, _p(inp), inp.size, _p(out))
+    return out
+
+
+def tree_node(node: int, mem_ops: int, compute_iters: int, inp: np.ndarray) -> float:
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    return lib().gtap_oracle_tree_node(node, mem_ops, compute_iters, _p(inp), inp.size)

@@ -385,3 +385,51 @@ uint64_t gtap_oracle_hash_u32(const uint32_t* a, int64_t n) {
     }
     return h;
 }
+
+/* ------------------------------------------------------------------ SpMV, binary tree */
+
+void gtap_oracle_spmv(int32_t rows, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+                      const double* x, double* y) {
+    for (int32_t i = 0; i < rows; ++i) {
+        double sum = 0.0;
+        for (int32_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) sum = fma(val[k], x[col_idx[k]], sum);
+        y[i] = sum;
+    }
+}
+
+static uint32_t oracle_xs32(uint32_t x) {
+    x ^= x << 13;
+    x ^= x >> 17;
+    x ^= x << 5;
+    return x;
+}
+
+static uint32_t oracle_sat_u32(double v) {
+    if (!(v > 0.0)) return 0u;
+    if (v >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)v;
+}
+
+double gtap_oracle_tree_node(int32_t node, int32_t mem_ops, int32_t compute_iters, const double* input,
+                             int64_t input_n) {
+    const uint32_t seed = oracle_xs32((uint32_t)node ^ 0x9e3779b9u);
+    const uint32_t mask = (uint32_t)input_n - 1u;
+    double acc = 0.0;
+    for (int32_t m = 0; m < mem_ops; ++m) acc += input[oracle_xs32(seed + (uint32_t)m * 747796405u) & mask];
+    uint32_t mix = (0x9e3779b9u ^ (uint32_t)node) + oracle_sat_u32(acc);
+    for (int32_t it = 0; it < compute_iters; ++it) {
+        const double xv = (double)oracle_xs32(seed + (uint32_t)it * 747796405u);
+        const double yv = fma(xv, 1.0000001192092896, 0.9999999403953552);
+        uint64_t bits;
+        memcpy(&bits, &yv, sizeof(bits));
+        mix ^= (uint32_t)bits;
+    }
+    return (double)mix;
+}
+
+void gtap_oracle_tree(int32_t height, int32_t mem_ops, int32_t compute_iters, const double* input, int64_t input_n,
+                      double* out) {
+    const int64_t nodes = ((int64_t)1 << (height + 1)) - 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < nodes; ++v) out[v] = gtap_oracle_tree_node((int32_t)v, mem_ops, compute_iters, input, input_n);
+}

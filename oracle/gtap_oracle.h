@@ -100,6 +100,23 @@ void gtap_oracle_jobs(uint64_t seed, int32_t njobs, int32_t* kind, int32_t* para
 /* 64-bit FNV-1a over the bytes of a uint32 array (used to compare sorted outputs). */
 uint64_t gtap_oracle_hash_u32(const uint32_t* a, int64_t n);
 
+/* SpMV: examples/spmv/spmv_thread.cu:34-46 (one row = sum over the row's entries in column
+ * order), with the product-add written as fma(val, x, sum) so the result is one bit pattern.
+ * Pinned: dense numpy products on integer-valued matrices (exact), scipy.sparse within
+ * rounding on random ones, identity and all-ones closed forms (tests). */
+void gtap_oracle_spmv(int32_t rows, const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+                      const double* x, double* y);
+/* Binary-tree workload: examples/binary_tree/binary_tree_thread.cu:44-120. out[node] for
+ * every node of the tree of the given height; the conversion of the (possibly negative)
+ * accumulator to uint32 saturates (negative and NaN -> 0, >= 2^32-1 -> 2^32-1).
+ * Pinned: a pure-Python restatement with exact rational fma on small trees, and closed forms
+ * for mem_ops = compute_iters = 0 (tests). */
+void gtap_oracle_tree(int32_t height, int32_t mem_ops, int32_t compute_iters, const double* input, int64_t input_n,
+                      double* out);
+/* One node's result of the above. */
+double gtap_oracle_tree_node(int32_t node, int32_t mem_ops, int32_t compute_iters, const double* input,
+                             int64_t input_n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@ low=0 binary search of omp_cilksort.cpp:56-66 fails at least one of them.
 import itertools
 import json
 import os
+import struct
 
 import numpy as np
 import pytest
@@ -288,3 +289,86 @@ def test_hash_is_fnv1a():
         h ^= byte
         h = (h * 0x100000001b3) % (1 << 64)
     assert oracle.hash_u32(a) == h
+
+
+# ---------------------------------------------------------------- SpMV and the binary tree
+
+def _dense(rows, cols, row_ptr, col_idx, val):
+    A = np.zeros((rows, cols))
+    for i in range(rows):
+        for k in range(row_ptr[i], row_ptr[i + 1]):
+            A[i, col_idx[k]] += val[k]
+    return A
+
+
+def test_oracle_spmv_exact_on_integer_matrices():
+    # small integer values: every fma and every dense product is exact, so they agree exactly
+    rng = np.random.default_rng(5)
+    rows, cols = 40, 30
+    counts = rng.integers(0, 6, rows)
+    row_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    col_idx = np.concatenate([np.sort(rng.choice(cols, c, replace=False)) for c in counts]).astype(np.int32)
+    val = rng.integers(-8, 9, row_ptr[-1]).astype(np.float64)
+    x = rng.integers(-8, 9, cols).astype(np.float64)
+    y = oracle.spmv(row_ptr, col_idx, val, x)
+    assert np.array_equal(y, _dense(rows, cols, row_ptr, col_idx, val) @ x)
+    assert np.all(y[counts == 0] == 0.0)
+
+
+def test_oracle_spmv_random_matrix_against_scipy():
+    import scipy.sparse as sp
+    import gtap_amd
+    if not os.path.exists(gtap_amd.LIB_PATH):
+        pytest.skip("libgtap_amd.so not built")
+    rp, ci, va = gtap_amd.spmv_random(2000, 12, 3)
+    x = np.random.default_rng(1).uniform(-1, 1, 2000)
+    y = oracle.spmv(rp, ci, va, x)
+    ref = sp.csr_matrix((va, ci, rp), shape=(2000, 2000)) @ x
+    assert np.allclose(y, ref, rtol=1e-12, atol=1e-12)
+    # identity matrix: y == x exactly; all-ones x: row sums
+    eye = np.arange(2001, dtype=np.int32)
+    assert np.array_equal(oracle.spmv(eye, np.arange(2000, dtype=np.int32), np.ones(2000), x), x)
+
+
+def _py_fma(a: float, b: float, c: float) -> float:
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))  # exact, then one rounding
+
+
+def _py_tree_node(node, mem_ops, iters, inp):
+    def xs(v):
+        v &= 0xFFFFFFFF
+        v ^= (v << 13) & 0xFFFFFFFF
+        v ^= v >> 17
+        v ^= (v << 5) & 0xFFFFFFFF
+        return v
+    seed = xs(node ^ 0x9E3779B9)
+    acc = 0.0
+    for m in range(mem_ops):
+        acc += float(inp[xs(seed + m * 747796405) & (len(inp) - 1)])
+    sat = 0 if not acc > 0 else (0xFFFFFFFF if acc >= 4294967295.0 else int(acc))
+    mix = ((0x9E3779B9 ^ node) + sat) & 0xFFFFFFFF
+    for it in range(iters):
+        y = _py_fma(float(xs(seed + it * 747796405)), 1.0000001192092896, 0.9999999403953552)
+        mix ^= struct.unpack("<Q", struct.pack("<d", y))[0] & 0xFFFFFFFF
+    return float(mix)
+
+
+def test_oracle_tree_against_exact_python():
+    inp = np.random.default_rng(2).uniform(-1, 1, 256)
+    out = oracle.tree(4, 7, 9, inp)
+    assert out.size == 31
+    for node in range(31):
+        assert out[node] == _py_tree_node(node, 7, 9, inp)
+
+
+def test_oracle_tree_closed_forms_and_saturation():
+    inp = np.full(16, -0.5)
+    # no work: out = (0x9e3779b9 ^ node) as a double
+    out = oracle.tree(3, 0, 0, inp)
+    assert out.tolist() == [float(0x9E3779B9 ^ v) for v in range(15)]
+    # a negative accumulator converts to 0 (saturation), so it adds nothing
+    assert oracle.tree_node(5, 10, 0, inp) == float(0x9E3779B9 ^ 5)
+    # a large positive one saturates at 2^32 - 1
+    big = np.full(16, 1e12)
+    assert oracle.tree_node(5, 3, 0, big) == float(((0x9E3779B9 ^ 5) + 0xFFFFFFFF) & 0xFFFFFFFF)
