@@ -204,6 +204,57 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
                          "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas}}
 
 
+def bench_spmv_tree(gtap_amd, reps: int = 3) -> dict:
+    """The reference's other example workloads at their default sizes: SpMV of a random
+    1,000,000 x 1,000,000 CSR matrix with about 300 entries per row (examples/spmv/
+    spmv_thread.cu:123-126, range cutoff 16) and the binary tree of height 15 with 64 random
+    reads and 512 fma steps per node (binary_tree/binary_tree_thread.cu:122-126). Matrix,
+    vectors and input resident on the GPU; kernel time per launch."""
+    import ctypes
+    import numpy as np
+    L = gtap_amd.lib()
+    out = {}
+    rp, ci, va = gtap_amd.spmv_random(1_000_000, 300, 42)
+    rows = rp.size - 1
+    x = np.ones(rows)
+    bufs = [gtap_amd.DeviceBuffer.from_array(a) for a in (rp, ci, va, x)]
+    dy = gtap_amd.DeviceBuffer(rows * 8)
+    ms = []
+    for _ in range(reps + 1):
+        s = gtap_amd.RunStats()
+        st = L.gtap_spmv(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, bufs[3].ptr, dy.ptr, rows, rows, 16, ctypes.byref(s))
+        if st != 0:
+            raise RuntimeError(f"gtap_spmv failed: {st}")
+        ms.append(s.kernel_ms)
+    y = dy.to_array(np.empty(rows))
+    for b in bufs + [dy]:
+        b.free()
+    # x = 1: every row sum is the fma chain of the row's values, checked on a sample of rows
+    sample = np.arange(0, rows, 9973)
+    for i in sample:
+        acc = 0.0
+        for v in va[rp[i]:rp[i + 1]]:
+            acc = acc + float(v)  # v * 1.0 + acc: fma with a unit factor rounds like the add
+        if y[i] != acc:
+            raise RuntimeError(f"spmv row {i}: {y[i]} != {acc}")
+    med = statistics.median(ms[1:])
+    nnz = int(rp[-1])
+    out["spmv_1m_x300"] = {"kernel_ms_median": med, "rows": rows, "nnz": nnz, "nnz_per_s": nnz / (med / 1e3),
+                           "tasks_executed": s.tasks_executed,
+                           "algorithmic_gbps": (12 * nnz + 8 * nnz + 4 * (rows + 1) + 8 * rows) / (med / 1e3) / 1e9}
+    log(f"spmv 1M x ~300: {med:.3f} ms, {nnz / (med / 1e3) / 1e9:.2f} G nnz/s")
+    inp = gtap_amd.tree_input(1 << 20)
+    ms = []
+    for _ in range(reps + 1):
+        o, ts = gtap_amd.tree(15, 64, 512, inp)
+        ms.append(ts["kernel_ms"])
+    med = statistics.median(ms[1:])
+    nodes = (1 << 16) - 1
+    out["tree_h15_m64_c512"] = {"kernel_ms_median": med, "nodes": nodes, "nodes_per_s": nodes / (med / 1e3)}
+    log(f"tree height 15 (64 reads, 512 fma per node): {med:.3f} ms")
+    return out
+
+
 def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024,
                  serve_rates=(500.0, 2000.0, 8000.0, 32000.0), serve_jobs: int = 1024) -> dict:
     """The 4096-job mixed stream (fib / n-queens / sort jobs, fixed seed): one resident batched
@@ -331,11 +382,12 @@ def main() -> int:
         fib_ms.append(fs["kernel_ms"])
     fib_med = statistics.median(fib_ms) if fib_ms else None
 
-    extra_sort = extra_bfs = extra_stream = fences = None
+    extra_sort = extra_bfs = extra_stream = fences = extra_more = None
     if not args.headline_only:
         extra_sort = bench_cilksort(gtap_amd, roofline, hbm_peak_meas)
         extra_bfs = bench_bfs(gtap_amd, roofline, hbm_peak_meas)
         extra_stream = bench_stream(gtap_amd)
+        extra_more = bench_spmv_tree(gtap_amd)
         fences = gtap_amd.bench_fences(max(4, workers // 4 * 4), 64)
 
     cpu = None if args.no_cpu else cpu_baseline(nq_tasks, args.headline_only)
@@ -386,6 +438,7 @@ def main() -> int:
             "cilksort_2e26": extra_sort,
             "bfs_rmat22": extra_bfs,
             "stream_4096": extra_stream,
+            "spmv_tree": extra_more,
             "publication_order_ns_per_wave": fences,
             "device": info,
         },

@@ -31,6 +31,7 @@
 // which a compiler may or may not contract).
 #include <algorithm>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "../device/gtap_abi.hpp"
@@ -201,32 +202,61 @@ uint64_t spmv_mix(uint64_t z) {
 // A random CSR matrix of the shape of examples/spmv/spmv_thread.cu:89-107 (rows x rows;
 // row i has 1 + (h mod (2*avg_nnz - 1)) column draws, sorted and deduplicated; values uniform
 // in [-1, 1)), from a splitmix64 stream keyed by (seed, row, draw), so every row can be
-// generated on its own. Two calls: with row_ptr only (col_idx = val = NULL) it fills row_ptr
-// and *nnz_out; with all buffers it fills the matrix.
+// generated on its own: rows are generated by several host threads, twice (counts, then
+// entries). Two calls: with row_ptr only (col_idx = val = NULL) it fills row_ptr and
+// *nnz_out; with all buffers it fills the matrix.
 extern "C" int gtap_spmv_random(int32_t rows, int32_t avg_nnz, uint64_t seed, int32_t* row_ptr, int32_t* col_idx,
                                 double* val, int64_t* nnz_out) {
-    if (rows < 0 || avg_nnz < 1 || !row_ptr) return gtap::host::invalid_argument("gtap_spmv_random");
+    if (rows < 0 || avg_nnz < 1 || avg_nnz > (1 << 24) || !row_ptr) return gtap::host::invalid_argument("gtap_spmv_random");
     const uint64_t span = 2ull * (uint64_t)avg_nnz - 1ull;
-    std::vector<int32_t> cols;
-    int64_t nnz = 0;
-    row_ptr[0] = 0;
-    for (int32_t i = 0; i < rows; ++i) {
+    // the sorted, deduplicated columns of row i in `cols`; returns the row's key
+    auto row_cols = [&](int32_t i, std::vector<int32_t>& cols) {
         const uint64_t key = spmv_mix(seed ^ (0x9E3779B97F4A7C15ULL * ((uint64_t)i + 1)));
         const uint32_t cnt = 1u + (uint32_t)(key % span);
         cols.resize(cnt);
         for (uint32_t k = 0; k < cnt; ++k) cols[k] = (int32_t)(spmv_mix(key + 2ull * k + 1ull) % (uint64_t)rows);
         std::sort(cols.begin(), cols.end());
         cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
-        if (col_idx && val) {
-            for (size_t k = 0; k < cols.size(); ++k) {
-                col_idx[nnz + (int64_t)k] = cols[k];
-                const uint64_t r = spmv_mix(key + 2ull * (uint64_t)cols[k] + 2ull);
-                val[nnz + (int64_t)k] = (double)(r >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-            }
+        return key;
+    };
+    unsigned nth = std::thread::hardware_concurrency();
+    nth = nth == 0 ? 1u : (nth > 32u ? 32u : nth);
+    if ((int64_t)rows < 4096) nth = 1;
+    auto parallel_rows = [&](auto&& body) {
+        std::vector<std::thread> pool;
+        const int64_t per = ((int64_t)rows + nth - 1) / nth;
+        for (unsigned t = 0; t < nth; ++t) {
+            const int32_t r0 = (int32_t)std::min<int64_t>((int64_t)t * per, rows);
+            const int32_t r1 = (int32_t)std::min<int64_t>((int64_t)(t + 1) * per, rows);
+            pool.emplace_back([&body, r0, r1] {
+                std::vector<int32_t> cols;
+                for (int32_t i = r0; i < r1; ++i) body(i, cols);
+            });
         }
-        nnz += (int64_t)cols.size();
+        for (auto& th : pool) th.join();
+    };
+    std::vector<int32_t> count((size_t)rows);
+    parallel_rows([&](int32_t i, std::vector<int32_t>& cols) {
+        row_cols(i, cols);
+        count[(size_t)i] = (int32_t)cols.size();
+    });
+    int64_t nnz = 0;
+    row_ptr[0] = 0;
+    for (int32_t i = 0; i < rows; ++i) {
+        nnz += count[(size_t)i];
         if (nnz > 0x7FFFFFFF) return gtap::host::invalid_argument("gtap_spmv_random: more than 2^31-1 entries");
         row_ptr[i + 1] = (int32_t)nnz;
+    }
+    if (col_idx && val) {
+        parallel_rows([&](int32_t i, std::vector<int32_t>& cols) {
+            const uint64_t key = row_cols(i, cols);
+            const int64_t at = row_ptr[i];
+            for (size_t k = 0; k < cols.size(); ++k) {
+                col_idx[at + (int64_t)k] = cols[k];
+                const uint64_t r = spmv_mix(key + 2ull * (uint64_t)cols[k] + 2ull);
+                val[at + (int64_t)k] = (double)(r >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+            }
+        });
     }
     if (nnz_out) *nnz_out = nnz;
     return 0;
