@@ -169,6 +169,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_csr_from_edges": ([ctypes.c_int32, _i64, _vp, _vp, _vp, _vp], _i),
     "gtap_rmat_csr": ([_i, _i, _u64, _u32, _u32, _u32, _vp, _vp], _i),
     "gtap_gen_keys": ([_u64, _u64, _vp], _i),
+    "gtap_bfs_block": ([_vp, _vp, ctypes.c_int32, _i64, ctypes.c_int32, _vp, ctypes.POINTER(RunStats)], _i),
     "gtap_spmv": ([_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(RunStats)], _i),
     "gtap_spmv_host": ([_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                         ctypes.POINTER(RunStats)], _i),
@@ -221,6 +222,7 @@ def gtap_set_config(**kw) -> int:
 
 WORKER_STAT_FIELDS = ("tasks_executed", "tasks_spawned", "rounds", "pops", "steals", "steal_attempts",
                       "jobs_taken", "pool_refills", "published", "overflowed", "suspended")
+WORKER_START_TICKS = 11  # column of worker_stats(): 100 MHz wall-clock tick at which the worker started
 
 
 def worker_stats():
@@ -411,6 +413,24 @@ class DeviceBuffer:
         if self.ptr:
             lib().gtap_device_free(self.ptr)
             self.ptr = None
+
+
+def bfs_block(row, col, source: int = 0):
+    """Block-mode BFS depths (one vertex task per workgroup); returns (depth, stats)."""
+    import numpy as np
+    row = np.ascontiguousarray(row, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    nv = row.size - 1
+    drow, dcol, ddep = DeviceBuffer.from_array(row), DeviceBuffer.from_array(col), DeviceBuffer(nv * 4)
+    try:
+        s = RunStats()
+        _check(lib().gtap_bfs_block(drow.ptr, dcol.ptr, nv, col.size, source, ddep.ptr, ctypes.byref(s)),
+               "gtap_bfs_block")
+        depth = ddep.to_array(np.empty(nv, dtype=np.int32))
+    finally:
+        for b in (drow, dcol, ddep):
+            b.free()
+    return depth, s.as_dict()
 
 
 # ---------------------------------------------------------------- SpMV and the binary tree

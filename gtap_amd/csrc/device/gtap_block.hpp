@@ -248,9 +248,11 @@ struct BlockTask {
 };
 
 // Thread 0 only: pop the newest published task of this workgroup's deque.
+// A failed CAS means a thief took tasks, so the loop ends with a task or an empty deque: a
+// workgroup that counts itself idle has nothing left in its own deque.
 __device__ __forceinline__ uint32_t block_pop(const GtapDev& dev, BlockCtx* c, uint32_t me) {
     uint64_t* wp = dq_word_ptr(dev, 0, me);
-    for (int tries = 0; tries < 8; ++tries) {
+    while (true) {
         const uint64_t wv = ld64(wp);
         const uint32_t t = (uint32_t)(wv >> 32), b = (uint32_t)wv;
         if ((int32_t)(b - t) <= 0) return 0;
@@ -263,7 +265,6 @@ __device__ __forceinline__ uint32_t block_pop(const GtapDev& dev, BlockCtx* c, u
             return id;
         }
     }
-    return 0;
 }
 
 // Thread 0 only: steal the oldest half (at most kBlockStealMax, and at most the configured
@@ -380,6 +381,7 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
     BlockLoop& lp = c->lp;
     if (t == 0) {
         lp.t_begin = now_ticks();
+        if (dev.term_mode == GTAP_TERM_QUIESCE) atomic_add(dev.ctl + CTL_STARTED * 32, 1);
         lp.host_poll_last = 0;
         lp.rng = 0x9E3779B9u ^ (me * 0x85EBCA6Bu) ^ (uint32_t)lp.t_begin;
         if (lp.rng == 0) lp.rng = 1;
@@ -429,7 +431,9 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
                 if (dev.term_mode == GTAP_TERM_QUIESCE && !lp.idle_registered) {
                     lp.idle_registered = true;
                     const int32_t cnt = atomic_add(dev.ctl + CTL_IDLE * 32, 1) + 1;
-                    if ((uint32_t)cnt == dev.nworkers && (uint32_t)ld32(dev.ctl + CTL_JOB_NEXT * 32) >= dev.njobs)
+                    // every started workgroup is idle (one not yet resident holds no work)
+                    if (cnt == ld32(dev.ctl + CTL_STARTED * 32) &&
+                        (uint32_t)ld32(dev.ctl + CTL_JOB_NEXT * 32) >= dev.njobs)
                         st32(dev.ctl + CTL_DONE * 32, 1);
                 }
                 const uint32_t idle_iter = ++lp.idle_iter;
@@ -556,6 +560,7 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
         s[ST_STEAL_ATTEMPTS] = lp.attempts;
         s[ST_JOBS] = lp.jobs;
         s[ST_REFILLS] = c->refills;
+        s[ST_START_TICKS] = lp.t_begin;
     }
 }
 

@@ -35,6 +35,9 @@ enum CtlIndex : int {
     CTL_LIVE_JOBS = 6,  // admitted and not yet completed
     CTL_HOST_AVAIL = 7,   // serve mode: device mirror of HCTL_JOB_AVAIL
     CTL_HOST_STOP = 8,    // serve mode: device mirror of HCTL_STOP
+    CTL_STARTED = 9,      // workers that have entered the scheduler loop (quiescence counts
+                          // these, not the launch's worker count: a worker the hardware has
+                          // not made resident yet holds no work and must not be waited for)
     CTL_COUNT = 12
 };
 
@@ -49,6 +52,7 @@ enum HostCtlIndex : int {
 enum StatIndex : int {
     ST_TASKS = 0, ST_SPAWNS, ST_ROUNDS, ST_POPS, ST_STEALS, ST_STEAL_ATTEMPTS, ST_JOBS, ST_REFILLS,
     ST_PUBLISHED, ST_OVERFLOWED, ST_SUSPENDED,
+    ST_START_TICKS,  // wall-clock tick at which the worker entered the scheduler loop
     ST_COUNT = 16
 };
 

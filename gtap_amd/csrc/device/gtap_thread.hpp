@@ -1122,6 +1122,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             vst(&w->valid[q], (uint64_t)0);
             vst(&w->inlds[q], (uint64_t)0);
         }
+        if (dev.term_mode == GTAP_TERM_QUIESCE) atomic_add(dev.ctl + CTL_STARTED * 32, 1);
     }
     __builtin_amdgcn_wave_barrier();
 
@@ -1192,9 +1193,16 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered) {
                     idle_registered = true;
                     int32_t c = 0;
-                    if (lane == 0) c = atomic_add(dev.ctl + CTL_IDLE * 32, 1) + 1;
+                    int32_t started = 0;
+                    if (lane == 0) {
+                        c = atomic_add(dev.ctl + CTL_IDLE * 32, 1) + 1;
+                        started = ld32(dev.ctl + CTL_STARTED * 32);
+                    }
                     c = shfl(c, 0);
-                    if ((uint32_t)c == dev.nworkers) {
+                    started = shfl(started, 0);
+                    // every worker that has started is idle: no deque holds work, and a worker
+                    // starting later begins empty
+                    if (c == started) {
                         const uint32_t next = (uint32_t)ld32(dev.ctl + CTL_JOB_NEXT * 32);
                         if (next >= dev.njobs && lane == 0) st32(dev.ctl + CTL_DONE * 32, 1);
                     }
@@ -1480,6 +1488,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         s[ST_PUBLISHED] = st.published;
         s[ST_OVERFLOWED] = vld(&w->overflowed);
         s[ST_SUSPENDED] = st.suspended;
+        s[ST_START_TICKS] = t_begin;
     }
 }
 

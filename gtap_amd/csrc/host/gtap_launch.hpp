@@ -312,16 +312,17 @@ int launch_block(const typename P::Args& args, uint32_t term_mode, uint32_t njob
 }
 
 template <class P>
-int run_single_block_job(const typename P::Args& args, const gtap_job& job, int64_t* result, gtap_run_stats* stats) {
+int run_single_block_job(const typename P::Args& args, const gtap_job& job, int64_t* result, gtap_run_stats* stats,
+                         uint32_t term_mode = GTAP_TERM_JOBS) {
     Runtime& R = rt();
     if (!R.initialized) {
         int st = gtap_initialize();
         if (st) return st;
     }
-    int st = launch_block<P>(args, GTAP_TERM_JOBS, 1, R.jobs, 1, stats, nullptr, &job);
+    int st = launch_block<P>(args, term_mode, 1, R.jobs, 1, stats, nullptr, &job);
     if (st) return st;
     const gtap_job out = R.summary->job0;
-    if (out.status != GTAP_JOB_DONE) return fail(GTAP_ERROR_INVALID_TASKWAIT, "job did not complete");
+    if (term_mode == GTAP_TERM_JOBS && out.status != GTAP_JOB_DONE) return fail(GTAP_ERROR_INVALID_TASKWAIT, "job did not complete");
     if (result) *result = out.result;
     return 0;
 }

@@ -28,6 +28,7 @@
 // queue every task goes to queue 0. No task waits for its children (detached spawns); the
 // launch ends by quiescence. Distances are the BFS hop counts whatever the schedule.
 #include "../device/gtap_abi.hpp"
+#include "../device/gtap_block.hpp"
 #include "../host/gtap_launch.hpp"
 
 namespace gtap {
@@ -153,6 +154,58 @@ __device__ __forceinline__ void Program<NQ>::run(L& lane) {
 }  // namespace bfs
 }  // namespace gtap
 
+namespace gtap {
+namespace bfs {
+
+// Block-mode form (examples/asynchronous_bfs/asynchronous_bfs_block.cu:17-32): one vertex task
+// runs on a whole 256-thread workgroup, thread t taking edge s + t of the vertex's range; each
+// thread that lowers a neighbour's depth spawns that neighbour's task. A range longer than
+// kBlockEdges is split in halves first (a workgroup invocation spawns at most kBlockMaxSpawn
+// tasks; the reference walks the whole list with a stride instead).
+constexpr int32_t kBlockEdges = 128;
+
+struct BlockProgram {
+    static constexpr int kFrameWords = 4;
+    static constexpr uint32_t kMaxFunc = F_BFS;
+    static constexpr uint32_t kFrameBytes = 16;
+    using Args = BfsArgs;
+    struct Shared {
+        int32_t unused;
+    };
+    template <class B>
+    __device__ __forceinline__ static void run(B& blk) {
+        const BfsArgs& g = *blk.args;
+        const int32_t v = blk.template arg<int32_t>(0);
+        const int32_t s = blk.template arg<int32_t>(4);
+        const int32_t e = blk.template arg<int32_t>(8);
+        const int32_t d = blk.template arg<int32_t>(12);
+        const uint32_t t = threadIdx.x;
+        const bool stale = ld32(reinterpret_cast<const int32_t*>(g.depth) + v) < d;
+        if (!stale) {
+            if (e - s > kBlockEdges) {
+                if (t == 0) {
+                    const int32_t m = s + (e - s) / 2;
+                    blk.spawn_detached(F_BFS, BfsArg{v, s, m, d});
+                    blk.spawn_detached(F_BFS, BfsArg{v, m, e, d});
+                }
+            } else if ((int32_t)t < e - s) {
+                const int32_t u = g.col[s + (int32_t)t];
+                if (u < 0 || u >= g.nv) {
+                    GTAP_DEVICE_ERROR(*blk.dev, GTAP_ERROR_INVALID_ARGUMENT, blk.tid, -1, u, g.nv);
+                } else if (g.depth[u] > d + 1 &&
+                           __hip_atomic_fetch_min(g.depth + u, d + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > d + 1) {
+                    blk.spawn_detached(F_BFS, BfsArg{u, g.row[u], g.row[u + 1], d + 1});
+                }
+            }
+        }
+        blk.finish();
+    }
+};
+static_assert(kBlockEdges <= (int32_t)kBlockMaxSpawn, "one invocation spawns at most kBlockMaxSpawn tasks");
+
+}  // namespace bfs
+}  // namespace gtap
+
 using Bfs1 = gtap::bfs::Program<1>;
 using Bfs4 = gtap::bfs::Program<4>;
 
@@ -192,6 +245,29 @@ extern "C" int gtap_bfs(const int32_t* row, const int32_t* col, int32_t nv, int6
     bfs::BfsArgs a{row, col, depth, nv, split};
     if (num_queues == 4) return host::run_single_job<Bfs4>(a, job, nullptr, stats, GTAP_TERM_QUIESCE);
     return host::run_single_job<Bfs1>(a, job, nullptr, stats, GTAP_TERM_QUIESCE);
+}
+
+// The block-mode BFS (one vertex task per 256-thread workgroup) over a CSR graph in device
+// memory; same arguments and result as gtap_bfs. Replaces the program of
+// examples/asynchronous_bfs/asynchronous_bfs_block.cu:17-37.
+extern "C" int gtap_bfs_block(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne, int32_t source,
+                              int32_t* depth, gtap_run_stats* stats) {
+    using namespace gtap;
+    if (!row || !col || !depth || nv <= 0 || source < 0 || source >= nv || ne < 0)
+        return host::invalid_argument("gtap_bfs_block: bad graph arguments");
+    host::Runtime& R = host::rt();
+    if (!R.initialized) {
+        int st = gtap_initialize();
+        if (st) return st;
+    }
+    hipLaunchKernelGGL(bfs_init_depth, dim3(1024), dim3(256), 0, R.stream, depth, nv, source);
+    GTAP_HIP_TRY(hipGetLastError());
+    int32_t rs[2];
+    GTAP_HIP_TRY(hipMemcpy(rs, row + source, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    bfs::BfsArg root{source, rs[0], rs[1], 0};
+    gtap_job job = host::make_job(bfs::F_BFS, 0, &root, sizeof(root));
+    bfs::BfsArgs a{row, col, depth, nv, 0};
+    return host::run_single_block_job<bfs::BlockProgram>(a, job, nullptr, stats, GTAP_TERM_QUIESCE);
 }
 
 // Host-pointer convenience: copies the CSR in, runs gtap_bfs, copies depth out.

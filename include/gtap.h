@@ -70,7 +70,8 @@ int gtap_set_profiling(int enable, uint32_t max_samples);
 int gtap_get_last_stats(gtap_run_stats* out);
 /* Per-wave counters of the last launch: 16 uint64 per worker, in the order tasks executed,
  * tasks spawned, rounds, pops, steals, steal attempts, jobs admitted, pool refills, records
- * published, records pushed out of the LDS window, suspended taskwaits, then 5 unused. */
+ * published, records pushed out of the LDS window, suspended taskwaits, the 100 MHz
+ * wall-clock tick at which the worker started, then 4 unused. */
 int gtap_get_worker_stats(uint64_t* out, uint32_t max_workers, uint32_t* nworkers_out);
 int gtap_device_info(char* name, int name_len, int* num_cus, uint64_t* hbm_bytes);
 
@@ -153,6 +154,12 @@ int gtap_stream_serve(gtap_stream* stream, double rate, uint64_t arrival_seed, i
 /* Device time from admission to completion of each job of the last run (any of the three
  * forms above), in ms: the execution part of latency_ms. */
 int gtap_stream_exec_ms(gtap_stream* stream, double* exec_ms);
+
+/* Block-mode BFS (examples/asynchronous_bfs/asynchronous_bfs_block.cu:17-37): one vertex
+ * task per 256-thread workgroup, a thread per edge of ranges split to <= 128 edges. Same
+ * buffers and result as gtap_bfs. */
+int gtap_bfs_block(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne, int32_t source, int32_t* depth,
+                   gtap_run_stats* stats);
 
 /* ------------------------------------------------------------ more reference workloads */
 /* y = A x, CSR float64, divide-and-conquer over row ranges of at most `cutoff` rows (0 ->

@@ -1,6 +1,7 @@
-"""SpMV (examples/spmv/spmv_thread.cu:34-66) and the binary-tree workload
-(examples/binary_tree/binary_tree_thread.cu:63-120) on the GPU, bit-exact against the
-oracle; a graph read from a file in the reference's CSR format through BFS."""
+"""SpMV (examples/spmv/spmv_thread.cu:34-66), the binary-tree workload
+(examples/binary_tree/binary_tree_thread.cu:63-120) and the block-mode BFS
+(examples/asynchronous_bfs/asynchronous_bfs_block.cu:17-37) on the GPU, bit-exact against
+the oracle; a graph read from a file in the reference's CSR format through BFS."""
 import numpy as np
 import pytest
 
@@ -76,3 +77,24 @@ def test_bfs_on_graph_read_from_reference_format(tmp_path):
     r2, c2 = G.read_csr(p)
     depth, _ = G.bfs(r2, c2, 0, 4, 64)
     assert np.array_equal(depth, O.bfs(row, col, 0))
+
+
+@pytest.mark.parametrize("scale,ef", [(8, 4), (14, 8), (18, 16)])
+def test_block_bfs_matches_oracle(scale, ef):
+    print(f"case block bfs rmat scale={scale} ef={ef}", flush=True)
+    row, col = G.rmat_csr(scale, ef, scale)
+    depth, s = G.bfs_block(row, col, 0)
+    assert np.array_equal(depth, O.bfs(row, col, 0))
+
+
+def test_block_bfs_chain_and_star():
+    print("case block bfs chain and star", flush=True)
+    n = 3000
+    row = np.concatenate([np.arange(n), [n - 1]]).astype(np.int32)  # chain 0 -> 1 -> ... (the reference's default graph)
+    col = np.arange(1, n, dtype=np.int32)
+    depth, _ = G.bfs_block(row, col, 0)
+    assert depth.tolist() == list(range(n))
+    star_row = np.array([0, 5000] + [5000] * 5000, dtype=np.int32)  # vertex 0 -> 1..5000
+    star_col = np.arange(1, 5001, dtype=np.int32)
+    depth, _ = G.bfs_block(star_row, star_col, 0)
+    assert depth[0] == 0 and np.all(depth[1:] == 1)
