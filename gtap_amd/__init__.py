@@ -169,6 +169,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_csr_from_edges": ([ctypes.c_int32, _i64, _vp, _vp, _vp, _vp], _i),
     "gtap_rmat_csr": ([_i, _i, _u64, _u32, _u32, _u32, _vp, _vp], _i),
     "gtap_gen_keys": ([_u64, _u64, _vp], _i),
+    "gtap_nqueens_cutoff": ([_i, _i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
     "gtap_bfs_block": ([_vp, _vp, ctypes.c_int32, _i64, ctypes.c_int32, _vp, ctypes.POINTER(RunStats)], _i),
     "gtap_spmv": ([_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(RunStats)], _i),
     "gtap_spmv_host": ([_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -413,6 +414,15 @@ class DeviceBuffer:
         if self.ptr:
             lib().gtap_device_free(self.ptr)
             self.ptr = None
+
+
+def nqueens_cutoff(n: int, cutoff: int = 7) -> tuple[int, dict]:
+    """n-queens at the reference's benchmark settings (tasks down to row `cutoff`, sequential
+    search below, no taskwait); returns (solutions, stats)."""
+    r = ctypes.c_int64(0)
+    s = RunStats()
+    _check(lib().gtap_nqueens_cutoff(n, cutoff, ctypes.byref(r), ctypes.byref(s)), "gtap_nqueens_cutoff")
+    return r.value, s.as_dict()
 
 
 def bfs_block(row, col, source: int = 0):

@@ -38,6 +38,7 @@ enum CtlIndex : int {
     CTL_STARTED = 9,      // workers that have entered the scheduler loop (quiescence counts
                           // these, not the launch's worker count: a worker the hardware has
                           // not made resident yet holds no work and must not be waited for)
+    CTL_PROGRAM = 10,     // a counter a program may use (cleared with the others per launch)
     CTL_COUNT = 12
 };
 

@@ -110,6 +110,90 @@ __device__ __forceinline__ void Program::run(L& lane) {
 }  // namespace nq
 }  // namespace gtap
 
+namespace gtap {
+namespace nq {
+
+// The reference's evaluation program at its own settings (evaluation/2-comparison/nq/
+// gtap_nq.cu:10-56): tasks down to row `cutoff`, a sequential search below it, detached
+// spawns (no taskwait) and one shared int32 counter. The sequential search here is an
+// explicit-stack depth-first walk; each task adds its count with one atomic (the reference
+// adds 1 per solution). Unlike the reference, a task that reaches row N counts a solution
+// even above the cutoff, so N <= cutoff + 1 is also correct.
+struct CutoffArgs {
+    int32_t n;
+    int32_t cutoff;
+    int32_t* answer;
+};
+
+__device__ __forceinline__ int32_t nq_serial(int32_t row, int32_t n, uint32_t cols, uint32_t left, uint32_t right) {
+    const uint32_t mask = (n < 32) ? ((1u << n) - 1u) : 0xFFFFFFFFu;
+    uint32_t avail[kMaxN + 1], cs[kMaxN + 1], ls[kMaxN + 1], rs[kMaxN + 1];
+    int32_t count = 0;
+    int32_t d = 0;
+    cs[0] = cols;
+    ls[0] = left;
+    rs[0] = right;
+    avail[0] = mask & ~(cols | left | right);
+    if (row == n) return 1;
+    while (d >= 0) {
+        const uint32_t a = avail[d];
+        if (a == 0u) {
+            --d;
+            continue;
+        }
+        const uint32_t p = a & (0u - a);
+        avail[d] = a - p;
+        if (row + d + 1 == n) {
+            ++count;
+            continue;
+        }
+        const uint32_t c = cs[d] | p, l = (ls[d] | p) << 1, r = (rs[d] | p) >> 1;
+        ++d;
+        cs[d] = c;
+        ls[d] = l;
+        rs[d] = r;
+        avail[d] = mask & ~(c | l | r);
+    }
+    return count;
+}
+
+struct CutoffProgram {
+    static constexpr int kQueues = 1;
+    static constexpr bool kNoTaskwait = true;
+    static constexpr int kWavesPerSimd = 3;
+    static constexpr int kFrameWords = 4;  // row, cols, left, right
+    static constexpr uint32_t kMaxFunc = F_NQ;
+    static constexpr uint32_t kFrameBytes = 16;
+    using Args = CutoffArgs;
+    template <class L>
+    __device__ __forceinline__ static void run(L& lane) {
+        const CutoffArgs& g = *lane.args;
+        const int32_t row = lane.template arg<int32_t>(0);
+        const uint32_t cols = lane.template arg<uint32_t>(4);
+        const uint32_t left = lane.template arg<uint32_t>(8);
+        const uint32_t right = lane.template arg<uint32_t>(12);
+        int32_t found = 0;
+        if (row == g.n) {
+            found = 1;
+        } else if (row > g.cutoff) {
+            found = nq_serial(row, g.n, cols, left, right);
+        } else {
+            const uint32_t mask = (g.n < 32) ? ((1u << g.n) - 1u) : 0xFFFFFFFFu;
+            uint32_t avail = mask & ~(cols | left | right);
+            while (avail) {
+                const uint32_t p = avail & (0u - avail);
+                avail -= p;
+                __gtap_spawn_task_detached(lane, F_NQ, 0, NqArg{row + 1, cols | p, (left | p) << 1, (right | p) >> 1});
+            }
+        }
+        if (found) atomic_add(g.answer, found);
+        __gtap_finish_task(lane);
+    }
+};
+
+}  // namespace nq
+}  // namespace gtap
+
 using NqProgram = gtap::nq::Program;
 static_assert(NqProgram::kFrameBytes + gtap::kHdrBytes <= gtap::host::kBuiltinStride, "frame exceeds stride");
 
@@ -123,4 +207,28 @@ extern "C" int gtap_nqueens(int n, int64_t* result, gtap_run_stats* stats) {
     int st = gtap::host::run_single_job<NqProgram>(NqProgram::Args{n}, job, &r, stats, GTAP_TERM_JOBS);
     if (st == 0 && result) *result = r;
     return st;
+}
+
+// The reference's n-queens benchmark at its own settings (evaluation/2-comparison/nq/
+// gtap_nq.cu:57-99): tasks for rows 0..cutoff (the reference's default 7), a sequential search
+// below, no taskwait, one int32 counter. Returns the solution count in *result.
+extern "C" int gtap_nqueens_cutoff(int n, int cutoff, int64_t* result, gtap_run_stats* stats) {
+    using namespace gtap;
+    if (n < 1 || n > nq::kMaxN || cutoff < 0) return host::invalid_argument("gtap_nqueens_cutoff: n in [1, 16], cutoff >= 0");
+    host::Runtime& R = host::rt();
+    if (!R.initialized) {
+        int st = gtap_initialize();
+        if (st) return st;
+    }
+    // the counter: the launch's program control word, cleared by the per-launch reset
+    int32_t* answer = R.ctl + CTL_PROGRAM * 32;
+    nq::NqArg root{0, 0u, 0u, 0u};
+    gtap_job job = host::make_job(nq::F_NQ, 0, &root, sizeof(root));
+    int st = host::run_single_job<gtap::nq::CutoffProgram>(nq::CutoffArgs{n, cutoff, answer}, job, nullptr, stats,
+                                                            GTAP_TERM_QUIESCE);
+    if (st) return st;
+    int32_t v = 0;
+    GTAP_HIP_TRY(hipMemcpy(&v, answer, sizeof(v), hipMemcpyDeviceToHost));
+    if (result) *result = v;
+    return 0;
 }

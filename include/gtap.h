@@ -155,6 +155,11 @@ int gtap_stream_serve(gtap_stream* stream, double rate, uint64_t arrival_seed, i
  * forms above), in ms: the execution part of latency_ms. */
 int gtap_stream_exec_ms(gtap_stream* stream, double* exec_ms);
 
+/* n-queens at the reference's benchmark settings (evaluation/2-comparison/nq/gtap_nq.cu:10-99):
+ * tasks for rows 0..cutoff (the reference uses 7), a sequential search below, detached
+ * spawns, one int32 counter. */
+int gtap_nqueens_cutoff(int n, int cutoff, int64_t* result, gtap_run_stats* stats);
+
 /* Block-mode BFS (examples/asynchronous_bfs/asynchronous_bfs_block.cu:17-37): one vertex
  * task per 256-thread workgroup, a thread per edge of ranges split to <= 128 edges. Same
  * buffers and result as gtap_bfs. */
