@@ -255,6 +255,43 @@ def bench_spmv_tree(gtap_amd, reps: int = 3) -> dict:
     return out
 
 
+# The reference's published medians on 1x GH200 for the same programs at its own settings
+# (evaluation/2-comparison/nq/nq_performance_results.csv:15-17, fib_performance_results.csv:16,
+# 1-worker_scalability/fib/fib_scaling_results.csv:10).
+REFERENCE_GH200_MS = {"nq14_cutoff7": 3.0135, "nq15_cutoff7": 17.175, "nq16_cutoff7": 115.6665,
+                      "fib30": 0.958, "fib35": 7.778}
+
+
+def bench_reference_settings(gtap_amd, reps: int = 5) -> dict:
+    """The reference's own benchmark programs at its own settings, timed here, beside the
+    medians it publishes for 1x GH200: n-queens with tasks to row 7 and a sequential search
+    below (no taskwait, one counter), and fib(35)."""
+    out = {}
+    for n, exp in ((14, 365596), (15, 2279184), (16, 14772512)):
+        ms = []
+        for _ in range(reps):
+            v, s = gtap_amd.nqueens_cutoff(n, 7)
+            if v != exp:
+                raise RuntimeError(f"nqueens_cutoff({n}) = {v}")
+            ms.append(s["kernel_ms"])
+        key = f"nq{n}_cutoff7"
+        med = statistics.median(ms)
+        out[key] = {"kernel_ms_median": med, "reference_gh200_ms": REFERENCE_GH200_MS[key],
+                    "speedup_vs_reference": REFERENCE_GH200_MS[key] / med}
+    ms = []
+    for _ in range(reps):
+        v, s = gtap_amd.fib(35)
+        if v != 9227465:
+            raise RuntimeError(f"fib(35) = {v}")
+        ms.append(s["kernel_ms"])
+    med = statistics.median(ms)
+    out["fib35"] = {"kernel_ms_median": med, "reference_gh200_ms": REFERENCE_GH200_MS["fib35"],
+                    "speedup_vs_reference": REFERENCE_GH200_MS["fib35"] / med}
+    log("reference settings: " + ", ".join(f"{k} {v['kernel_ms_median']:.3f} ms "
+                                           f"({v['speedup_vs_reference']:.2f}x the GH200 median)" for k, v in out.items()))
+    return out
+
+
 def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024,
                  serve_rates=(500.0, 2000.0, 8000.0, 32000.0), serve_jobs: int = 1024) -> dict:
     """The 4096-job mixed stream (fib / n-queens / sort jobs, fixed seed): one resident batched
@@ -382,12 +419,13 @@ def main() -> int:
         fib_ms.append(fs["kernel_ms"])
     fib_med = statistics.median(fib_ms) if fib_ms else None
 
-    extra_sort = extra_bfs = extra_stream = fences = extra_more = None
+    extra_sort = extra_bfs = extra_stream = fences = extra_more = extra_ref = None
     if not args.headline_only:
         extra_sort = bench_cilksort(gtap_amd, roofline, hbm_peak_meas)
         extra_bfs = bench_bfs(gtap_amd, roofline, hbm_peak_meas)
         extra_stream = bench_stream(gtap_amd)
         extra_more = bench_spmv_tree(gtap_amd)
+        extra_ref = bench_reference_settings(gtap_amd)
         fences = gtap_amd.bench_fences(max(4, workers // 4 * 4), 64)
 
     cpu = None if args.no_cpu else cpu_baseline(nq_tasks, args.headline_only)
@@ -434,11 +472,13 @@ def main() -> int:
             },
             "hbm_measured_copy_gbps": hbm_peak_meas,
             "fib30": None if fib_med is None else {"kernel_ms_median": fib_med, "tasks": fib_tasks,
-                                                   "tasks_per_s": fib_tasks / (fib_med / 1e3)},
+                                                   "tasks_per_s": fib_tasks / (fib_med / 1e3),
+                                                   "reference_gh200_ms": REFERENCE_GH200_MS["fib30"]},
             "cilksort_2e26": extra_sort,
             "bfs_rmat22": extra_bfs,
             "stream_4096": extra_stream,
             "spmv_tree": extra_more,
+            "reference_settings": extra_ref,
             "publication_order_ns_per_wave": fences,
             "device": info,
         },
