@@ -224,6 +224,7 @@ def gtap_set_config(**kw) -> int:
 WORKER_STAT_FIELDS = ("tasks_executed", "tasks_spawned", "rounds", "pops", "steals", "steal_attempts",
                       "jobs_taken", "pool_refills", "published", "overflowed", "suspended")
 WORKER_START_TICKS = 11  # column of worker_stats(): 100 MHz wall-clock tick at which the worker started
+WORKER_END_TICKS = 12    # and the tick at which it stopped
 
 
 def worker_stats():

@@ -30,8 +30,10 @@ struct BlockLoop {
     uint32_t rng;
     uint32_t idle_iter;
     uint32_t ph_idx, pw_idx;  // timeline profiling
+    uint32_t seen_next;       // last value of the admission counter read
     bool idle_registered;
     bool had_task;
+    bool static_job;          // this workgroup still has to start job `me` (handed out by index)
     uint32_t tasks, pops, steals, attempts, jobs;
 };
 
@@ -57,10 +59,12 @@ struct BlockCtx {
 
 __device__ __forceinline__ uint32_t blk_thread() { return threadIdx.x; }
 
-// Block-collective refill of the LDS id stack with one chunk (thread 0 claims, all copy).
+// Block-collective refill of the LDS id stack: one chunk from the workgroup's HBM stack, else
+// one ring cell, else `want` chunks of never-used ids with one fetch-add. The first wave
+// probes every ring at once (one lane per ring); thread 0 claims; all copy.
 template <int DUMMY = 0>
-__device__ __forceinline__ void block_refill_chunk(const GtapDev& dev, BlockCtx* c, uint32_t me) {
-    __shared__ uint32_t s_kind, s_pos, s_shard, s_fresh;
+__device__ __forceinline__ void block_refill_chunk(const GtapDev& dev, BlockCtx* c, uint32_t me, uint32_t want) {
+    __shared__ uint32_t s_kind, s_pos, s_shard, s_fresh, s_nch;
     const uint32_t t = blk_thread();
     const uint32_t top = c->free_top;
     const uint32_t ptop = c->priv_top;
@@ -75,36 +79,64 @@ __device__ __forceinline__ void block_refill_chunk(const GtapDev& dev, BlockCtx*
         __syncthreads();
         return;
     }
-    if (t == 0) {
-        uint32_t kind = 0, pos = 0, shard = 0, fresh = 0;
-        for (uint32_t i = 0; i < dev.nshards && kind == 0; ++i) {
-            shard = (me + i) % dev.nshards;
-            if (ring_take_claim(dev, shard, pos)) kind = 1;
+    if (t < (uint32_t)kWave) {
+        uint32_t kind = 0, pos = 0, shard = 0, fresh = 0, nch = 0;
+        bool fresh_left = true;
+        for (int attempt = 0; attempt < 64; ++attempt) {
+            const bool seen = ring_probe(dev, me, (uint32_t)kWave, t, shard, pos);
+            if (seen) {
+                int ok = 0;
+                if (t == 0) {
+                    uint32_t expected = pos;
+                    ok = atomic_cas(pool_deq_ptr(dev, shard), expected, pos + 1) ? 1 : 0;
+                }
+                if (shfl((int32_t)ok, 0)) {
+                    kind = 1;
+                    nch = 1;
+                    break;
+                }
+            }
+            if (fresh_left) {
+                uint32_t f = 0, got = 0;
+                if (t == 0) {
+                    f = atomic_add(pool_fresh_ptr(dev), want * kChunk);
+                    const uint32_t left = f < dev.nrecs ? (dev.nrecs - f) / kChunk : 0u;
+                    got = left < want ? left : want;
+                }
+                fresh = shfl(f, 0);
+                nch = shfl(got, 0);
+                if (nch) {
+                    kind = 2;
+                    break;
+                }
+                fresh_left = false;
+            }
+            if (!seen) break;  // every ring empty and no never-used ids left
         }
-        if (kind == 0) {
-            fresh = atomic_add(pool_fresh_ptr(dev), kChunk);
-            if (fresh <= dev.nrecs - kChunk) kind = 2;
+        if (t == 0) {
+            s_kind = kind;
+            s_pos = pos;
+            s_shard = shard;
+            s_fresh = fresh;
+            s_nch = kind ? nch : 1u;
+            c->refills += 1;
         }
-        s_kind = kind;
-        s_pos = pos;
-        s_shard = shard;
-        s_fresh = fresh;
-        c->refills += 1;
     }
     __syncthreads();
     const uint32_t kind = s_kind;
-    if (t < kChunk) {
+    const uint32_t nids = s_nch * kChunk;
+    for (uint32_t i = t; i < nids; i += kBlockThreads) {
         uint32_t id = 0;
-        if (kind == 1) id = ld32(pool_cell_ptr(dev, s_shard, s_pos) + t);
-        else if (kind == 2) id = s_fresh + t;
-        c->free_ids[top + t] = id;
+        if (kind == 1) id = ld32(pool_cell_ptr(dev, s_shard, s_pos) + i);
+        else if (kind == 2) id = s_fresh + i;
+        c->free_ids[top + i] = id;
     }
     complete_stores();
     __syncthreads();
     if (t == 0) {
         if (kind == 1) st32(pool_seq_ptr(dev, s_shard, s_pos), s_pos + dev.ncells);
         if (kind == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_TASK_ID_POOL_LOW_HEADROOM, -1, -1, (int)s_fresh, (int)dev.nrecs);
-        c->free_top = top + kChunk;
+        c->free_top = top + nids;
     }
     __syncthreads();
 }
@@ -331,25 +363,11 @@ __device__ __forceinline__ uint32_t block_steal(const GtapDev& dev, uint32_t me,
     return 0;
 }
 
-// Thread 0 only: admit one job from the job table as a root task.
+// Thread 0 only: job j of the table becomes a root task (anchor record with join = 1, root
+// record with the job's arguments) and is marked running. Returns the root id.
 template <class P>
-__device__ __forceinline__ uint32_t block_admit(const GtapDev& dev, BlockCtx* c, bool& idle_registered) {
-    uint32_t avail = dev.njobs;
-    if (dev.term_mode == GTAP_TERM_SERVE) avail = (uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32);
-    int32_t* nextp = dev.ctl + CTL_JOB_NEXT * 32;
-    const uint32_t next = (uint32_t)ld32(nextp);
-    if (next >= avail) return 0;
-    const uint32_t live = (uint32_t)ld32(dev.ctl + CTL_LIVE_JOBS * 32);
-    if (live >= dev.max_live_jobs) return 0;
-    if (c->free_top == 0) return 0;
-    if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
-        atomic_add(dev.ctl + CTL_IDLE * 32, -1);
-        idle_registered = false;
-    }
-    int32_t expected = (int32_t)next;
-    if (!atomic_cas(nextp, expected, (int32_t)(next + 1))) return 0;
-    atomic_add(dev.ctl + CTL_LIVE_JOBS * 32, 1);
-    const uint32_t slot = next % dev.job_ring;
+__device__ __forceinline__ uint32_t block_start_job(const GtapDev& dev, BlockCtx* c, uint32_t j) {
+    const uint32_t slot = j % dev.job_ring;
     gtap_job* job = dev.jobs + slot;
     const uint32_t func = ld32_sys(&job->func);
     const uint32_t anchor = dev.nrecs + slot;
@@ -373,6 +391,32 @@ __device__ __forceinline__ uint32_t block_admit(const GtapDev& dev, BlockCtx* c,
     return id;
 }
 
+// Thread 0 only: admit one job from the job table as a root task. `seen_next` is the last
+// value of the admission counter this workgroup read; the counter only grows, so once it
+// was seen at the number of jobs available its line is not read again.
+template <class P>
+__device__ __forceinline__ uint32_t block_admit(const GtapDev& dev, BlockCtx* c, bool& idle_registered,
+                                                uint32_t& seen_next) {
+    uint32_t avail = dev.njobs;
+    if (dev.term_mode == GTAP_TERM_SERVE) avail = (uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32);
+    if (seen_next >= avail) return 0;
+    int32_t* nextp = dev.ctl + CTL_JOB_NEXT * 32;
+    const uint32_t next = (uint32_t)ld32(nextp);
+    seen_next = next;
+    if (next >= avail) return 0;
+    const uint32_t live = (uint32_t)ld32(dev.ctl + CTL_LIVE_JOBS * 32);
+    if (live >= dev.max_live_jobs) return 0;
+    if (c->free_top == 0) return 0;
+    if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
+        atomic_add(dev.ctl + CTL_IDLE * 32, -1);
+        idle_registered = false;
+    }
+    int32_t expected = (int32_t)next;
+    if (!atomic_cas(nextp, expected, (int32_t)(next + 1))) return 0;
+    atomic_add(dev.ctl + CTL_LIVE_JOBS * 32, 1);
+    return block_start_job<P>(dev, c, next);
+}
+
 template <class P>
 __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args, BlockCtx* c,
                                 typename P::Shared* sh) {
@@ -389,6 +433,8 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
         lp.ph_idx = lp.pw_idx = 0;
         lp.idle_registered = false;
         lp.had_task = false;
+        lp.seen_next = dev.jobs != nullptr ? dev.static_jobs : 0u;
+        lp.static_job = dev.jobs != nullptr && me < dev.static_jobs;
         lp.tasks = lp.pops = lp.steals = lp.attempts = lp.jobs = 0;
         c->cur = 0;
         c->exit = 0;
@@ -401,6 +447,14 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
         c->refills = 0;
     }
     __syncthreads();
+    // the workgroup's first ids are a fixed range (the shared fresh-id counter starts above
+    // every worker's range)
+    if (dev.static_ids) {
+        for (uint32_t i = t; i < dev.static_ids; i += kBlockThreads) c->free_ids[i] = 64u + me * dev.static_ids + i;
+        __syncthreads();
+        if (t == 0) c->free_top = dev.static_ids;
+        __syncthreads();
+    }
 
     BlockTask<P> T;
     T.dev = &dev;
@@ -414,12 +468,21 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
     while (true) {
         if (host_poller) host_poll(dev.ctl, dev.host_ctl, lp.host_poll_last);
         // keep enough ids in LDS for one invocation's spawns (collective)
-        while (c->free_top < kBlockMaxSpawn) block_refill_chunk(dev, c, me);
+        while (c->free_top < kBlockMaxSpawn)
+            block_refill_chunk(dev, c, me, (kBlockMaxSpawn - c->free_top + kChunk - 1) / kChunk);
         if (t == 0 && c->cur == 0) {
-            uint32_t id = block_pop(dev, c, me);
-            if (id) lp.pops += 1;
+            uint32_t id = 0;
+            if (lp.static_job) {
+                lp.static_job = false;
+                id = block_start_job<P>(dev, c, me);
+                lp.jobs += 1;
+            }
+            if (!id) {
+                id = block_pop(dev, c, me);
+                if (id) lp.pops += 1;
+            }
             if (!id && dev.jobs != nullptr) {
-                id = block_admit<P>(dev, c, lp.idle_registered);
+                id = block_admit<P>(dev, c, lp.idle_registered, lp.seen_next);
                 if (id) lp.jobs += 1;
             }
             if (!id) id = block_steal(dev, me, c);
@@ -561,6 +624,7 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
         s[ST_JOBS] = lp.jobs;
         s[ST_REFILLS] = c->refills;
         s[ST_START_TICKS] = lp.t_begin;
+        s[ST_END_TICKS] = now_ticks();
     }
 }
 

@@ -54,6 +54,7 @@ enum StatIndex : int {
     ST_TASKS = 0, ST_SPAWNS, ST_ROUNDS, ST_POPS, ST_STEALS, ST_STEAL_ATTEMPTS, ST_JOBS, ST_REFILLS,
     ST_PUBLISHED, ST_OVERFLOWED, ST_SUSPENDED,
     ST_START_TICKS,  // wall-clock tick at which the worker entered the scheduler loop
+    ST_END_TICKS,    // wall-clock tick at which it left the loop
     ST_COUNT = 16
 };
 
@@ -90,6 +91,9 @@ struct GtapDev {
     uint32_t job_ring;         // job-table entries (anchors are job index % job_ring)
     uint32_t admit_reserve;    // free ids required per admitted job
     uint32_t max_live_jobs;    // admission cap on concurrently running jobs
+    uint32_t static_jobs;      // jobs [0, static_jobs) start on workers [0, static_jobs), by index
+    uint32_t static_ids;       // ids [64 + w * static_ids, 64 + (w + 1) * static_ids) are worker w's
+                               // at launch start (0: none)
     uint32_t prof_max;         // samples per worker; 0 = profiling off
     uint32_t idle_level;       // deepest idle backoff level (1..6: s_sleep 1 .. 32)
     uint32_t gq_cap;           // global-queue scheduler: ring slots per EPAQ queue (power of two)

@@ -190,9 +190,36 @@ __device__ __forceinline__ bool ring_put_claim(const GtapDev& dev, uint32_t s, u
     return false;
 }
 
-// Move 64 ids into the wave's LDS stack. Called by all active lanes.
+// Active lanes look at the rings together, lane of rank r at ring (me + b + r) % nshards
+// for b = 0, cnt, 2 cnt, ...: its dequeue position and that cell's sequence number. Returns
+// (wave-uniform) the first ring seen with a full cell at its dequeue position.
+__device__ __forceinline__ bool ring_probe(const GtapDev& dev, uint32_t me, uint32_t cnt, uint32_t r,
+                                           uint32_t& shard_out, uint32_t& pos_out) {
+    for (uint32_t b = 0; b < dev.nshards; b += cnt) {
+        const uint32_t i = b + r;
+        bool full = false;
+        uint32_t sh = 0, pos = 0;
+        if (i < dev.nshards) {
+            sh = (me + i) % dev.nshards;
+            pos = ld32(pool_deq_ptr(dev, sh));
+            full = ld32(pool_seq_ptr(dev, sh, pos)) == pos + 1;
+        }
+        const uint64_t f = ballot(full);
+        if (f) {
+            const int src = first_lane(f);
+            shard_out = shfl(sh, src);
+            pos_out = shfl(pos, src);
+            return true;
+        }
+    }
+    return false;
+}
+
+// Add ids to the wave's LDS stack: one chunk from the wave's HBM stack, else one ring cell,
+// else `want` chunks (fewer when the pool runs out) of never-used ids with one fetch-add.
+// Called by all active lanes.
 template <class W>
-__device__ __forceinline__ void refill_chunk(const GtapDev& dev, W* w, uint32_t me) {
+__device__ __forceinline__ void refill_chunk(const GtapDev& dev, W* w, uint32_t me, uint32_t want) {
     const uint64_t m = active_mask();
     const uint32_t cnt = (uint32_t)popc(m);
     const uint32_t r = (uint32_t)rank_in(m);
@@ -203,38 +230,58 @@ __device__ __forceinline__ void refill_chunk(const GtapDev& dev, W* w, uint32_t 
         const uint32_t* src = dev.priv_ids + (size_t)me * kPrivCap + (ptop - kChunk);
         for (uint32_t i = r; i < kChunk; i += cnt) vst(&w->free_ids[top + i], src[i]);
         vst(&w->priv_top, ptop - kChunk);
-    } else {
-        uint32_t pos = 0, shard = 0, fresh = 0;
-        int kind = 0;  // 0 = exhausted, 1 = ring cell, 2 = fresh ids
-        if ((int)r == 0) {
-            for (uint32_t i = 0; i < dev.nshards && kind == 0; ++i) {
-                shard = (me + i) % dev.nshards;
-                if (ring_take_claim(dev, shard, pos)) kind = 1;
-            }
-            if (kind == 0) {
-                fresh = atomic_add(pool_fresh_ptr(dev), kChunk);
-                if (fresh <= dev.nrecs - kChunk) kind = 2;
-            }
-        }
-        kind = shfl((int32_t)kind, leader);
-        pos = shfl(pos, leader);
-        shard = shfl(shard, leader);
-        fresh = shfl(fresh, leader);
-        if (kind == 1) {
-            const uint32_t* ids = pool_cell_ptr(dev, shard, pos);
-            for (uint32_t i = r; i < kChunk; i += cnt) vst(&w->free_ids[top + i], ld32(ids + i));
-            complete_stores();  // the cell is read out before it is released
-            if ((int)r == 0) st32(pool_seq_ptr(dev, shard, pos), pos + dev.ncells);
-        } else if (kind == 2) {
-            for (uint32_t i = r; i < kChunk; i += cnt) vst(&w->free_ids[top + i], fresh + i);
-        } else {
-            if ((int)r == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_TASK_ID_POOL_LOW_HEADROOM, -1, -1, (int)fresh, (int)dev.nrecs);
-            // hand out the sink record (id 0) so the round can finish harmlessly
-            for (uint32_t i = r; i < kChunk; i += cnt) vst(&w->free_ids[top + i], 0u);
-        }
-        if ((int)r == 0) vst(&w->refills, vld(&w->refills) + 1u);
+        vst(&w->free_top, top + kChunk);
+        return;
     }
-    vst(&w->free_top, top + kChunk);
+    uint32_t pos = 0, shard = 0, fresh = 0, nch = 0;
+    int kind = 0;  // 0 = exhausted, 1 = ring cell, 2 = fresh ids
+    bool fresh_left = true;
+    for (int attempt = 0; attempt < 64; ++attempt) {
+        const bool seen = ring_probe(dev, me, cnt, r, shard, pos);
+        if (seen) {
+            int ok = 0;
+            if ((int)r == 0) {
+                uint32_t expected = pos;
+                ok = atomic_cas(pool_deq_ptr(dev, shard), expected, pos + 1) ? 1 : 0;
+            }
+            if (shfl((int32_t)ok, leader)) {
+                kind = 1;
+                nch = 1;
+                break;
+            }
+        }
+        if (fresh_left) {
+            uint32_t f = 0, got = 0;
+            if ((int)r == 0) {
+                f = atomic_add(pool_fresh_ptr(dev), want * kChunk);
+                const uint32_t left = f < dev.nrecs ? (dev.nrecs - f) / kChunk : 0u;
+                got = left < want ? left : want;
+            }
+            fresh = shfl(f, leader);
+            nch = shfl(got, leader);
+            if (nch) {
+                kind = 2;
+                break;
+            }
+            fresh_left = false;
+        }
+        if (!seen) break;  // every ring empty and no never-used ids left
+    }
+    if (kind == 1) {
+        const uint32_t* ids = pool_cell_ptr(dev, shard, pos);
+        for (uint32_t i = r; i < kChunk; i += cnt) vst(&w->free_ids[top + i], ld32(ids + i));
+        complete_stores();  // the cell is read out before it is released
+        if ((int)r == 0) st32(pool_seq_ptr(dev, shard, pos), pos + dev.ncells);
+    } else if (kind == 2) {
+        for (uint32_t i = r; i < nch * kChunk; i += cnt) vst(&w->free_ids[top + i], fresh + i);
+    } else {
+        if ((int)r == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_TASK_ID_POOL_LOW_HEADROOM, -1, -1, (int)fresh, (int)dev.nrecs);
+        // hand out the sink record (id 0) so the round can finish harmlessly
+        nch = 1;
+        for (uint32_t i = r; i < kChunk; i += cnt) vst(&w->free_ids[top + i], 0u);
+    }
+    if ((int)r == 0) vst(&w->refills, vld(&w->refills) + 1u);
+    vst(&w->free_top, top + nch * kChunk);
 }
 
 // Move the 64 most recently freed ids out of the wave's LDS stack.
@@ -277,7 +324,7 @@ __device__ __forceinline__ uint32_t alloc_id(const GtapDev& dev, W* w, uint32_t 
     uint32_t top = uniform(vld(&w->free_top));
     if (top < cnt) {
         while (top < kRefillTarget) {
-            refill_chunk(dev, w, me);
+            refill_chunk(dev, w, me, (kRefillTarget - top + kChunk - 1) / kChunk);
             top = uniform(vld(&w->free_top));
         }
     }
@@ -790,6 +837,34 @@ __device__ __forceinline__ void load_task_lds(Lane<P>& L, uint32_t id, const uin
     for (int i = P::kFrameWords; i < CtxShape<P>::kFWr; ++i) L.frame[i] = 0u;
 }
 
+// Job j of the table becomes a root task: its anchor record (join = 1, the root) and the
+// root record with the job's arguments; the job is marked running. Returns the root id.
+// Called by the lanes that start a job each.
+template <class P>
+__device__ __forceinline__ uint32_t start_job(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t j) {
+    const uint32_t slot = j % dev.job_ring;
+    gtap_job* job = dev.jobs + slot;
+    const uint32_t func = ld32_sys(&job->func);
+    const uint32_t q = ld32_sys(&job->queue);
+    const uint32_t anchor = dev.nrecs + slot;
+    const uint32_t id = alloc_id(dev, w, me);
+    uint8_t* arec = rec_ptr(dev, anchor);
+    st64(reinterpret_cast<uint64_t*>(arec), hdr_lo(kNoParent, 0u));
+    st64(reinterpret_cast<uint64_t*>(arec + kHdrJoin), hdr_hi(1u, hdr_word0(kAnchorFunc, 0, 0)));  // join = 1 (the root)
+    st64(reinterpret_cast<uint64_t*>(arec + kHdrBytes), 0ull);
+    uint8_t* rrec = rec_ptr(dev, id);
+    st64(reinterpret_cast<uint64_t*>(rrec), hdr_lo(anchor, kHdrBytes));
+    st64(reinterpret_cast<uint64_t*>(rrec + kHdrJoin), hdr_hi(0u, hdr_word0(func, 0, q < (uint32_t)P::kQueues ? q : 0)));
+    const uint64_t* ja = reinterpret_cast<const uint64_t*>(job->args);
+    uint64_t* ra = reinterpret_cast<uint64_t*>(rrec + kHdrBytes);
+    constexpr int kArgWords = (P::kFrameWords / 2) < 8 ? (P::kFrameWords / 2) : 8;
+#pragma unroll
+    for (int i = 0; i < kArgWords; ++i) st64(ra + i, ld64_sys(ja + i));
+    st64_sys(&job->t_start, now_ticks());
+    st32_sys(&job->status, (uint32_t)GTAP_JOB_RUNNING);
+    return id;
+}
+
 // Admit up to 64 jobs from the job table as root tasks. Returns the number admitted; lane
 // i < count receives the root id in `tid_out`.
 template <class P>
@@ -836,32 +911,20 @@ __device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtxOf<P>*
     }
     ok = shfl((int32_t)ok, 0);
     if (!ok) return 0;
-    if ((uint32_t)lane < k) {
-        const uint32_t j = next + (uint32_t)lane;
-        const uint32_t slot = j % dev.job_ring;
-        gtap_job* job = dev.jobs + slot;
-        const uint32_t func = ld32_sys(&job->func);
-        const uint32_t q = ld32_sys(&job->queue);
-        const uint32_t anchor = dev.nrecs + slot;
-        const uint32_t id = alloc_id(dev, w, me);
-        uint8_t* arec = rec_ptr(dev, anchor);
-        st64(reinterpret_cast<uint64_t*>(arec), hdr_lo(kNoParent, 0u));
-        st64(reinterpret_cast<uint64_t*>(arec + kHdrJoin), hdr_hi(1u, hdr_word0(kAnchorFunc, 0, 0)));  // join = 1 (the root)
-        st64(reinterpret_cast<uint64_t*>(arec + kHdrBytes), 0ull);
-        uint8_t* rrec = rec_ptr(dev, id);
-        st64(reinterpret_cast<uint64_t*>(rrec), hdr_lo(anchor, kHdrBytes));
-        st64(reinterpret_cast<uint64_t*>(rrec + kHdrJoin), hdr_hi(0u, hdr_word0(func, 0, q < (uint32_t)P::kQueues ? q : 0)));
-        const uint64_t* ja = reinterpret_cast<const uint64_t*>(job->args);
-        uint64_t* ra = reinterpret_cast<uint64_t*>(rrec + kHdrBytes);
-        constexpr int kArgWords = (P::kFrameWords / 2) < 8 ? (P::kFrameWords / 2) : 8;
-#pragma unroll
-        for (int i = 0; i < kArgWords; ++i) st64(ra + i, ld64_sys(ja + i));
-        st64_sys(&job->t_start, now_ticks());
-        st32_sys(&job->status, (uint32_t)GTAP_JOB_RUNNING);
-        tid_out = id;
-    }
+    if ((uint32_t)lane < k) tid_out = start_job<P>(dev, w, me, next + (uint32_t)lane);
     complete_stores();
     return k;
+}
+
+// Jobs [0, dev.static_jobs) are handed out by index at launch start: worker j starts job j
+// without touching the admission counter, which the reset sets to static_jobs. Lane 0 takes
+// the root; returns the batch size (1 or 0).
+template <class P>
+__device__ __forceinline__ uint32_t admit_static(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t& tid_out) {
+    if (me >= dev.static_jobs) return 0;
+    if (lane_id() == 0) tid_out = start_job<P>(dev, w, me, me);
+    complete_stores();
+    return 1;
 }
 
 // A job's root finished: publish its result to the job table.
@@ -1125,6 +1188,13 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         if (dev.term_mode == GTAP_TERM_QUIESCE) atomic_add(dev.ctl + CTL_STARTED * 32, 1);
     }
     __builtin_amdgcn_wave_barrier();
+    // the worker's first ids are a fixed range (the shared fresh-id counter starts above
+    // every worker's range), so starting needs no shared counter
+    if (dev.static_ids) {
+        for (uint32_t i = (uint32_t)lane; i < dev.static_ids; i += kWave) vst(&w->free_ids[i], 64u + me * dev.static_ids + i);
+        if (lane == 0) vst(&w->free_top, dev.static_ids);
+        __builtin_amdgcn_wave_barrier();
+    }
 
     Lane<P> L;
     L.dev = &dev;
@@ -1144,7 +1214,9 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     bool idle_registered = false;
     uint32_t idle_iter = 0;
     uint32_t busy_iter = 0;
-    uint32_t seen_next = 0;  // last value of the shared admission counter this wave read
+    // last value of the shared admission counter this wave read (the jobs handed out by
+    // index are taken)
+    uint32_t seen_next = dev.jobs != nullptr ? dev.static_jobs : 0u;
     LoopStats st = {0, 0, 0, 0, 0, 0, 0, 0};
     // timeline profiling (lane 0, when dev.prof_max > 0)
     uint32_t ph_idx = 0, pw_idx = 0;
@@ -1152,6 +1224,10 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
 
     uint64_t host_poll_last = 0;
     const bool host_poller = dev.term_mode == GTAP_TERM_SERVE && me == 0;
+    if (dev.jobs != nullptr) {
+        n = admit_static<P>(dev, w, me, my_tid);
+        st.jobs += n;
+    }
 
     while (true) {
         if (host_poller && lane == 0) host_poll(dev.ctl, dev.host_ctl, host_poll_last);
@@ -1489,6 +1565,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         s[ST_OVERFLOWED] = vld(&w->overflowed);
         s[ST_SUSPENDED] = st.suspended;
         s[ST_START_TICKS] = t_begin;
+        s[ST_END_TICKS] = now_ticks();
     }
 }
 

@@ -71,6 +71,10 @@ static uint32_t next_pow2(uint64_t v) {
     return (uint32_t)p;
 }
 
+// Ids each worker holds at launch start: a wave's refill target and more than a workgroup's
+// per-invocation spawn reserve.
+constexpr uint32_t kStaticIds = 192;
+
 GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job* jobs, uint32_t job_ring) {
     Runtime& R = rt();
     GtapDev d;
@@ -103,6 +107,12 @@ GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job
     d.job_ring = job_ring;
     d.admit_reserve = 0;
     d.max_live_jobs = R.nanchors;
+    // A launch's first jobs go to its first workers by index, and every worker starts with a
+    // fixed range of ids: starting touches neither the admission counter nor the id pool's
+    // shared counter, which thousands of workers would otherwise all hit at once.
+    if (term_mode != GTAP_TERM_SERVE && jobs != nullptr)
+        d.static_jobs = std::min(std::min(njobs, nworkers), R.nanchors);
+    if (64ull + (uint64_t)nworkers * kStaticIds <= R.nrecs / 2) d.static_ids = kStaticIds;
     d.prof_max = R.prof_max;
     d.gq_cap = R.gq_cap;
     d.gq_slots = R.gq_slots;
@@ -121,7 +131,8 @@ __global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0) {
     const uint64_t gs = (uint64_t)gridDim.x * blockDim.x;
     if (write_job0 && gid < sizeof(gtap_job) / 4)
         reinterpret_cast<uint32_t*>(dev.jobs)[gid] = reinterpret_cast<const uint32_t*>(&job0)[gid];
-    for (uint64_t i = gid; i < (uint64_t)CTL_COUNT * 32; i += gs) dev.ctl[i] = 0;
+    for (uint64_t i = gid; i < (uint64_t)CTL_COUNT * 32; i += gs)
+        dev.ctl[i] = (i == CTL_JOB_NEXT * 32 || i == CTL_LIVE_JOBS * 32) ? (int32_t)dev.static_jobs : 0;
     for (uint64_t i = gid; i < (uint64_t)dev.nqueues * dev.nworkers; i += gs) dev.dq_word[i * 16] = 0;
     for (uint64_t i = gid; i < (uint64_t)dev.nshards * dev.ncells; i += gs)
         dev.pool_seq[i] = (uint32_t)(i & (dev.ncells - 1));
@@ -136,9 +147,9 @@ __global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0) {
         }
     }
     // ring positions start at 0; the fresh-id counter starts at 64 (ids 0..63 are
-    // reserved, id 0 is the sink record)
+    // reserved, id 0 is the sink record), above the workers' fixed first ranges
     for (uint64_t i = gid; i < (uint64_t)dev.nshards * 32 + 32; i += gs)
-        dev.pool_ctl[i] = (i == (uint64_t)dev.nshards * 32) ? 64u : 0u;
+        dev.pool_ctl[i] = (i == (uint64_t)dev.nshards * 32) ? 64ull + (uint64_t)dev.nworkers * dev.static_ids : 0ull;
 }
 
 int enqueue_reset(const GtapDev& dev, hipStream_t stream, const gtap_job* job0) {

@@ -71,7 +71,7 @@ int gtap_get_last_stats(gtap_run_stats* out);
 /* Per-wave counters of the last launch: 16 uint64 per worker, in the order tasks executed,
  * tasks spawned, rounds, pops, steals, steal attempts, jobs admitted, pool refills, records
  * published, records pushed out of the LDS window, suspended taskwaits, the 100 MHz
- * wall-clock tick at which the worker started, then 4 unused. */
+ * wall-clock ticks at which the worker started and stopped, then 3 unused. */
 int gtap_get_worker_stats(uint64_t* out, uint32_t max_workers, uint32_t* nworkers_out);
 int gtap_device_info(char* name, int name_len, int* num_cus, uint64_t* hbm_bytes);
 
