@@ -149,6 +149,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_bench_atomics": ([_i, ctypes.POINTER(ctypes.c_double)], _i),
     "gtap_fib": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
     "gtap_nqueens": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
+    "gtap_nqueens_slots": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
     "gtap_bfs": ([_vp, _vp, ctypes.c_int32, _i64, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32,
                   ctypes.POINTER(RunStats)], _i),
     "gtap_bfs_host": ([_vp, _vp, ctypes.c_int32, _i64, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_int32,
@@ -330,6 +331,15 @@ def nqueens(n: int) -> tuple[int, dict]:
     r = ctypes.c_int64(0)
     s = RunStats()
     _check(lib().gtap_nqueens(n, ctypes.byref(r), ctypes.byref(s)), f"gtap_nqueens({n})")
+    return r.value, s.as_dict()
+
+
+def nqueens_slots(n: int) -> tuple[int, dict]:
+    """nqueens() with result slots instead of the additive join (each child stores its count
+    into the parent's frame, then counts the join down); returns (count, stats)."""
+    r = ctypes.c_int64(0)
+    s = RunStats()
+    _check(lib().gtap_nqueens_slots(n, ctypes.byref(r), ctypes.byref(s)), f"gtap_nqueens_slots({n})")
     return r.value, s.as_dict()
 
 

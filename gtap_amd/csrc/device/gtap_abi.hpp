@@ -15,6 +15,12 @@
 //   __gtap_set_state_for_join(tid, cnt, s, q)   __gtap_set_state_for_join(L, s, q)
 //   __gtap_finish_task(tid, ctx)                __gtap_finish_task(L) / __gtap_return(L, v)
 //   __gtap_get_task_data(tid)                   __gtap_get_task_data(L)
+//
+// The additive join has no reference counterpart: a parent that only sums its children's
+// integer results spawns them with __gtap_spawn_task_sum and waits with
+// __gtap_set_state_for_join_sum; after resuming, __gtap_get_join_sum(L) is the sum. Each
+// child's result then travels in the atomic that counts the join down (thread-mode lane
+// tasks; bounds in gtap_dev.hpp).
 #pragma once
 
 #include "gtap_thread.hpp"
@@ -35,6 +41,18 @@ __device__ __forceinline__ uint32_t __gtap_spawn_task_detached(L& lane, uint32_t
 template <class L>
 __device__ __forceinline__ bool __gtap_set_state_for_join(L& lane, uint32_t next_state, uint32_t queue_after) {
     return lane.taskwait(next_state, queue_after);
+}
+template <class L, class A>
+__device__ __forceinline__ uint32_t __gtap_spawn_task_sum(L& lane, uint32_t func, uint32_t queue, const A& args) {
+    return lane.spawn_sum(func, queue, args);
+}
+template <class L>
+__device__ __forceinline__ bool __gtap_set_state_for_join_sum(L& lane, uint32_t next_state, uint32_t queue_after) {
+    return lane.taskwait_sum(next_state, queue_after);
+}
+template <class L>
+__device__ __forceinline__ int64_t __gtap_get_join_sum(const L& lane) {
+    return lane.join_sum();
 }
 template <class L>
 __device__ __forceinline__ void __gtap_finish_task(L& lane) {

@@ -119,11 +119,26 @@ __device__ __forceinline__ void prof_stamp(uint64_t* base, int32_t* counts, uint
 //   word3: control word = func id [0..11] | state [12..23] | queue [24..27]
 // The join counter and the control word share one 8-byte word, so the child that takes the
 // counter to zero learns the parent's resume queue from the same atomic.
+//
+// Additive join (thread-mode lane tasks). A parent that only sums its children's integer
+// results takes them through its join word instead of result slots. Its suspended header is
+//   word1: result offset [0..15] | state [16..23] | func [24..30] | 1 << 31 (this layout)
+//   word2..3, one 8-byte word: count [0..11] | resume queue [12..15] | sum [16..63], signed
+// and each child, whose own result offset is kResSum, adds (value << 16) - 1 with the atomic
+// that counts it down: one memory operation per child instead of a store and an atomic.
+// Bounds: at most kSumMaxCount children per join, |value| < kSumMaxValue (so |sum| < 2^47),
+// func <= kSumMaxFunc, state <= kSumMaxState; the runtime reports anything outside them.
 constexpr uint32_t kHdrBytes = 16;
 constexpr uint32_t kHdrJoin = 8;  // byte offset of the {join, control} 8-byte word
 constexpr uint32_t kNoParent = 0xFFFFFFFFu;
 constexpr uint32_t kMaxFunc = 0xFFF;
 constexpr uint32_t kMaxState = 0xFFF;
+constexpr uint32_t kResSum = 0xFFFFu;          // result offset of a child of an additive join
+constexpr uint32_t kSumLayout = 0x80000000u;   // word1 flag of a parent waiting in an additive join
+constexpr uint32_t kSumMaxCount = 0xFFFu;
+constexpr uint32_t kSumMaxFunc = 0x7Fu;
+constexpr uint32_t kSumMaxState = 0xFFu;
+constexpr int64_t kSumMaxValue = (int64_t)1 << 35;
 
 __device__ __forceinline__ uint32_t hdr_word0(uint32_t func, uint32_t state, uint32_t queue) {
     return (func & 0xFFFu) | ((state & 0xFFFu) << 12) | ((queue & 0xFu) << 24);

@@ -29,6 +29,8 @@
 //    fences (see gtap_dev.hpp).
 #pragma once
 
+#include <type_traits>
+
 #include "gtap_dev.hpp"
 
 namespace gtap {
@@ -549,9 +551,13 @@ struct Lane {
     uint32_t parent;
     uint32_t res_off;
     uint32_t child_count;
+    uint32_t child_sum;     // of which children of an additive join
     uint32_t join_count;    // children to wait for, after a taskwait that suspends
     uint32_t outcome;
     bool lds;               // the record's header and first FW frame words are not in HBM yet
+    bool sum_join;          // the pending taskwait is an additive join
+    int64_t jsum;           // sum of the children's results, after an additive join
+    uint64_t join_delta;    // what finishing adds to the parent's join word
     uint32_t frame[FW];     // the first frame words, loaded when the round starts
 
     // -- reads of the loaded frame (arguments written by the spawner)
@@ -573,6 +579,8 @@ struct Lane {
     }
     // The 16-byte header this task leaves in its record when it suspends.
     __device__ __forceinline__ u32x4 suspended_header() const {
+        if (sum_join)
+            return u32x4{parent, (res_off & 0xFFFFu) | (state << 16) | (func << 24) | kSumLayout, join_count | (queue << 12), 0u};
         return u32x4{parent, res_off, join_count, hdr_word0(func, state, queue)};
     }
     // -- coherent access to any part of this task's frame (offsets relative to the frame)
@@ -630,6 +638,12 @@ struct Lane {
     __device__ __forceinline__ uint32_t spawn_detached(uint32_t f, uint32_t q, const A& a) {
         return spawn_impl(f, q, kNoParent, 0u, a);
     }
+    // Spawn a child whose integer result is added into this task's join sum (taskwait_sum).
+    template <typename A>
+    __device__ __forceinline__ uint32_t spawn_sum(uint32_t f, uint32_t q, const A& a) {
+        child_sum += 1;
+        return spawn_impl(f, q, tid, kResSum, a);
+    }
 
     template <typename A>
     __device__ __forceinline__ uint32_t spawn_impl(uint32_t f, uint32_t q, uint32_t par, uint32_t roff, const A& a) {
@@ -649,6 +663,11 @@ struct Lane {
             GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_QUEUE_IDX_AFTER_JOIN, tid, q_after, q_after, Q);
             q_after = 0;
         }
+        if (child_sum != 0) {
+            // children of an additive join add into the join word; only taskwait_sum waits for them
+            GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_TASKWAIT, tid, -1, child_sum, 0);
+            child_sum = 0;
+        }
         if (child_count == 0) {
             state = next_state;
             return false;
@@ -661,10 +680,49 @@ struct Lane {
         return true;
     }
 
-    // return value: written into the parent's result slot
+    // taskwait over children spawned with spawn_sum: when the task resumes at `next_state`,
+    // join_sum() is the sum of their results (0 when there were none). Every child joined
+    // here must be a spawn_sum child.
+    __device__ __forceinline__ bool taskwait_sum(uint32_t next_state, uint32_t q_after) {
+        static_assert(!NoTaskwait<P>::value, "a program declared kNoTaskwait calls taskwait");
+        if (q_after >= (uint32_t)Q) {
+            GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_QUEUE_IDX_AFTER_JOIN, tid, q_after, q_after, Q);
+            q_after = 0;
+        }
+        state = next_state;
+        jsum = 0;
+        if (child_count == 0) return false;
+        if (child_sum != child_count || child_count > kSumMaxCount || func > kSumMaxFunc || next_state > kSumMaxState) {
+            GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_TASKWAIT, tid, -1, child_sum, child_count);
+        }
+        queue = q_after;
+        join_count = child_count & kSumMaxCount;
+        child_count = 0;
+        child_sum = 0;
+        sum_join = true;
+        outcome = OUT_SUSPENDED;
+        return true;
+    }
+    __device__ __forceinline__ int64_t join_sum() const { return jsum; }
+
+    // return value: written into the parent's result slot, or (a child of an additive join)
+    // carried by the join atomic
     template <typename T>
     __device__ __forceinline__ void ret(T v) {
-        if (parent != kNoParent) stT<T>(rec_ptr(*dev, parent) + res_off, v);
+        if (parent != kNoParent) {
+            if (res_off == kResSum) {
+                if constexpr (std::is_integral<T>::value) {
+                    const int64_t x = (int64_t)v;
+                    if (x >= kSumMaxValue || x <= -kSumMaxValue)
+                        GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_ARGUMENT, tid, -1, (int)(x >> 32), (int)x);
+                    join_delta = ((uint64_t)x << 16) - 1u;
+                } else {
+                    GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_ARGUMENT, tid, -1, 0, 0);
+                }
+            } else {
+                stT<T>(rec_ptr(*dev, parent) + res_off, v);
+            }
+        }
         finish();
     }
     __device__ __forceinline__ void finish() {
@@ -735,7 +793,10 @@ struct WaveTask {
     }
     template <typename T>
     __device__ __forceinline__ void ret(T v) {
-        if (lane == 0 && parent != kNoParent) stT<T>(rec_ptr(*dev, parent) + res_off, v);
+        if (lane == 0 && parent != kNoParent) {
+            if (res_off == kResSum) GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_INVALID_TASKWAIT, tid, -1, 0, 0);  // lane tasks only
+            else stT<T>(rec_ptr(*dev, parent) + res_off, v);
+        }
         finish();
     }
     __device__ __forceinline__ void finish() {
@@ -806,12 +867,25 @@ __device__ __forceinline__ void load_task(Lane<P>& L, uint32_t id) {
     L.rec = rec_ptr(dev, id);
     const uint32_t off = id * dev.stride;
     const u32x4 h = ld128(L.rs, off);
-    L.func = hdr_func(h.w);
-    L.state = hdr_state(h.w);
-    L.queue = hdr_queue(h.w);
+    if (h.y & kSumLayout) {
+        // resumed after an additive join: the join word holds the children's sum
+        L.func = (h.y >> 24) & kSumMaxFunc;
+        L.state = (h.y >> 16) & kSumMaxState;
+        L.queue = (h.z >> 12) & 0xFu;
+        L.res_off = h.y & 0xFFFFu;
+        L.jsum = (int64_t)(((uint64_t)h.w << 32) | (uint64_t)h.z) >> 16;
+    } else {
+        L.func = hdr_func(h.w);
+        L.state = hdr_state(h.w);
+        L.queue = hdr_queue(h.w);
+        L.res_off = h.y;
+        L.jsum = 0;
+    }
     L.parent = h.x;
-    L.res_off = h.y;
     L.child_count = 0;
+    L.child_sum = 0;
+    L.sum_join = false;
+    L.join_delta = ~0ull;
     L.outcome = OUT_RUNNING;
     L.lds = false;
     frame_words_ld<CtxShape<P>::kFWr>(L.rs, off, L.frame);
@@ -829,6 +903,10 @@ __device__ __forceinline__ void load_task_lds(Lane<P>& L, uint32_t id, const uin
     L.parent = vld(&d[1]);
     L.res_off = vld(&d[2]);
     L.child_count = 0;
+    L.child_sum = 0;
+    L.sum_join = false;
+    L.jsum = 0;
+    L.join_delta = ~0ull;
     L.outcome = OUT_RUNNING;
     L.lds = true;
 #pragma unroll
@@ -1324,6 +1402,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         }
         had_tasks = true;
         uint32_t fin_tid = 0, fin_parent = kNoParent;
+        uint64_t fin_delta = ~0ull;  // added to the parent's join word: -1, plus value << 16 (additive join)
+        bool fin_sum = false;        // the parent waits in an additive join
         bool fin = false, sus = false;
         if (WQ >= 0 && batch_q == WQ) {
             // one wave-cooperative task on all lanes
@@ -1362,6 +1442,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             sus = (L.outcome == OUT_SUSPENDED);
             fin_tid = L.tid;
             fin_parent = L.parent;
+            fin_sum = L.res_off == kResSum;
+            fin_delta = L.join_delta;
             if (fin_parent != kNoParent && fin_parent >= dev.nrecs + dev.nanchors) {
                 GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_FUNCTION, fin_tid, -1, fin_parent, dev.nrecs + dev.nanchors);
                 fin_parent = kNoParent;
@@ -1398,21 +1480,22 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             }
         }
         complete_stores();  // results, suspended records and child records of this round
-        uint32_t ready = kNoParent, ready_ctl = 0;
+        uint32_t ready = kNoParent, ready_q = 0;
         if (fin && fin_parent != kNoParent) {
-            // {join, control} of the parent: the last child also reads the resume queue
-            const uint64_t old = atomic_sub(reinterpret_cast<uint64_t*>(rec_ptr(dev, fin_parent) + kHdrJoin), (uint64_t)1);
-            if ((uint32_t)old == 1u) {
+            // the parent's join word: the last child also reads the resume queue
+            const uint64_t old = atomic_add(reinterpret_cast<uint64_t*>(rec_ptr(dev, fin_parent) + kHdrJoin), fin_delta);
+            const uint32_t left = fin_sum ? (uint32_t)old & kSumMaxCount : (uint32_t)old;
+            if (left == 1u) {
                 ready = fin_parent;
-                ready_ctl = (uint32_t)(old >> 32);
-            } else if ((uint32_t)old == 0u) {
+                ready_q = fin_sum ? ((uint32_t)old >> 12) & 0xFu : hdr_queue((uint32_t)(old >> 32));
+            } else if (left == 0u) {
                 GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, fin_tid, -1, 0, 0);
             }
         }
         if (ballot(ready != kNoParent)) {
             if (ready != kNoParent && ready >= dev.nrecs) complete_job<P>(dev, ready);
             if (ready != kNoParent && ready < dev.nrecs)
-                queue_append<P, false>(dev, w, me, hdr_queue(ready_ctl) < (uint32_t)Q ? hdr_queue(ready_ctl) : 0u, ready, nullptr);
+                queue_append<P, false>(dev, w, me, ready_q < (uint32_t)Q ? ready_q : 0u, ready, nullptr);
         }
         if (fin && fin_tid != 0u) free_id(dev, w, me, fin_tid);  // id 0 is the sink record
 

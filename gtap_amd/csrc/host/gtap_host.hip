@@ -361,7 +361,8 @@ int gtap_initialize(void) {
     if (R.initialized) return 0;
     apply_defaults(&R.cfg);
     gtap_config& c = R.cfg;
-    if (c.record_stride % 16 != 0 || c.record_stride < 32) return invalid_argument("record_stride must be a multiple of 16 >= 32");
+    if (c.record_stride % 16 != 0 || c.record_stride < 32 || c.record_stride > 65520)
+        return invalid_argument("record_stride must be a multiple of 16 in [32, 65520]");  // result offsets < 0xFFFF
     if ((c.deque_capacity & (c.deque_capacity - 1)) != 0 || c.deque_capacity < 4096)
         return invalid_argument("deque_capacity must be a power of two >= 4096");
     if (c.num_queues < 1 || c.num_queues > 8) return invalid_argument("num_queues must be in [1, 8]");

@@ -15,7 +15,9 @@
 //       return a + b;
 //   }
 //
-// Lowered form below: frame = { int32 n; int32 pad; int64 a; int64 b; }, states 0 and 1.
+// Lowered form below, states 0 and 1, with the additive join: a and b are only summed after
+// the taskwait, so each child adds its value to the parent's join word with the atomic that
+// counts the join down, and the parent resumes with a + b. Frame = { int32 n; int32 pad; }.
 #include "../device/gtap_abi.hpp"
 #include "../host/gtap_launch.hpp"
 
@@ -27,15 +29,13 @@ struct NArg {
     int32_t n;
     int32_t pad;
 };
-constexpr int kOffA = 8;
-constexpr int kOffB = 16;
 
 struct Program {
     static constexpr int kQueues = 1;
     static constexpr bool kSchedVariants = true;  // also built with the global-queue and Chase-Lev schedulers
     static constexpr int kFrameWords = 2;  // n (+pad)
     static constexpr uint32_t kMaxFunc = F_FIB;
-    static constexpr uint32_t kFrameBytes = 24;
+    static constexpr uint32_t kFrameBytes = 16;  // covers the 4 frame words loaded with a task
     struct Args {
         int32_t unused;
     };
@@ -52,17 +52,14 @@ __device__ __forceinline__ void fib_task(L& lane) {
                 __gtap_return(lane, (int64_t)n);
                 return;
             }
-            __gtap_spawn_task(lane, F_FIB, 0, kOffA, NArg{n - 1, 0});
-            __gtap_spawn_task(lane, F_FIB, 0, kOffB, NArg{n - 2, 0});
-            if (__gtap_set_state_for_join(lane, 1, 0)) return;
+            __gtap_spawn_task_sum(lane, F_FIB, 0, NArg{n - 1, 0});
+            __gtap_spawn_task_sum(lane, F_FIB, 0, NArg{n - 2, 0});
+            if (__gtap_set_state_for_join_sum(lane, 1, 0)) return;
         }
             [[fallthrough]];
-        case 1: {
-            const int64_t a = lane.template frame_ld<int64_t>(kOffA);
-            const int64_t b = lane.template frame_ld<int64_t>(kOffB);
-            __gtap_return(lane, a + b);
+        case 1:
+            __gtap_return(lane, __gtap_get_join_sum(lane));
             return;
-        }
         default:
             GTAP_DEVICE_ERROR(*lane.dev, GTAP_ERROR_INVALID_TASKWAIT, lane.tid, 0, lane.state, 1);
             lane.finish();

@@ -19,10 +19,16 @@
 //       return s;
 //   }
 //
-// Lowered frame: { int32 row; uint32 cols, diag, anti; int32 k; int32 pad[3]; int64 counts[16]; }.
-// The first six words (the arguments and k) are the frame words loaded when the task starts,
-// so the resumed task knows k without another load; counts[] is 16-byte aligned and read
-// two at a time.
+// Two lowerings of this program:
+//  * additive join (Program, the default): the counts are only summed after the taskwait,
+//    so each child adds its count to the parent's join word with the atomic that counts the
+//    join down; frame = { int32 row; uint32 cols, diag, anti; }, the parent resumes with
+//    the sum.
+//  * result slots (SlotProgram): frame = { int32 row; uint32 cols, diag, anti; int32 k;
+//    int32 pad[3]; int64 counts[16]; }; each child stores its count into counts[i], then
+//    counts the join down. The first six words (the arguments and k) are loaded when the
+//    task starts, so the resumed task knows k without another load; counts[] is 16-byte
+//    aligned and read two at a time.
 #include "../device/gtap_abi.hpp"
 #include "../host/gtap_launch.hpp"
 
@@ -40,20 +46,24 @@ constexpr int kOffK = 16;
 constexpr int kOffCounts = 32;
 constexpr int kMaxN = 16;
 
-struct Program {
+template <bool kSum>
+struct ProgramT {
     static constexpr int kQueues = 1;
-    static constexpr bool kSchedVariants = true;  // also built with the global-queue and Chase-Lev schedulers
-    static constexpr int kFrameWords = 6;  // row, cols, diag, anti, k, pad
+    // the default form is also built with the global-queue and Chase-Lev schedulers
+    static constexpr bool kSchedVariants = kSum;
+    static constexpr int kFrameWords = kSum ? 4 : 6;  // row, cols, diag, anti (, k, pad)
     static constexpr uint32_t kMaxFunc = F_NQ;
-    static constexpr uint32_t kFrameBytes = kOffCounts + 8 * kMaxN;
+    static constexpr uint32_t kFrameBytes = kSum ? 16 : kOffCounts + 8 * kMaxN;
     struct Args {
         int32_t n;
     };
     template <class L>
     __device__ __forceinline__ static void run(L& lane);
 };
+using Program = ProgramT<true>;
+using SlotProgram = ProgramT<false>;
 
-template <class L>
+template <bool kSum, class L>
 __device__ __forceinline__ void nq_task(L& lane) {
     const int32_t N = lane.args->n;
     switch (__gtap_get_task_state(lane)) {
@@ -72,27 +82,36 @@ __device__ __forceinline__ void nq_task(L& lane) {
             while (avail) {
                 const uint32_t p = avail & (0u - avail);
                 avail -= p;
-                __gtap_spawn_task(lane, F_NQ, 0, kOffCounts + 8 * k,
-                                  NqArg{row + 1, cols | p, (diag | p) << 1, (anti | p) >> 1});
+                const NqArg child{row + 1, cols | p, (diag | p) << 1, (anti | p) >> 1};
+                if constexpr (kSum) __gtap_spawn_task_sum(lane, F_NQ, 0, child);
+                else __gtap_spawn_task(lane, F_NQ, 0, kOffCounts + 8 * k, child);
                 ++k;
             }
             if (k == 0) {  // dead end: no legal column in this row
                 __gtap_return(lane, (int64_t)0);
                 return;
             }
-            lane.template frame_st<int32_t>(kOffK, k);
-            if (__gtap_set_state_for_join(lane, 1, 0)) return;
+            if constexpr (kSum) {
+                if (__gtap_set_state_for_join_sum(lane, 1, 0)) return;
+            } else {
+                lane.template frame_st<int32_t>(kOffK, k);
+                if (__gtap_set_state_for_join(lane, 1, 0)) return;
+            }
         }
             [[fallthrough]];
         case 1: {
-            const int32_t k = lane.template arg<int32_t>(kOffK);
-            int64_t s = 0;
-            for (int32_t i = 0; i < k; i += 2) {
-                int64_t a, b;
-                lane.template frame_ld2<int64_t>(kOffCounts + 8 * i, a, b);
-                s += a + (i + 1 < k ? b : 0);
+            if constexpr (kSum) {
+                __gtap_return(lane, __gtap_get_join_sum(lane));
+            } else {
+                const int32_t k = lane.template arg<int32_t>(kOffK);
+                int64_t s = 0;
+                for (int32_t i = 0; i < k; i += 2) {
+                    int64_t a, b;
+                    lane.template frame_ld2<int64_t>(kOffCounts + 8 * i, a, b);
+                    s += a + (i + 1 < k ? b : 0);
+                }
+                __gtap_return(lane, s);
             }
-            __gtap_return(lane, s);
             return;
         }
         default:
@@ -102,9 +121,10 @@ __device__ __forceinline__ void nq_task(L& lane) {
     }
 }
 
+template <bool kSum>
 template <class L>
-__device__ __forceinline__ void Program::run(L& lane) {
-    nq_task(lane);
+__device__ __forceinline__ void ProgramT<kSum>::run(L& lane) {
+    nq_task<kSum>(lane);
 }
 
 }  // namespace nq
@@ -195,7 +215,8 @@ struct CutoffProgram {
 }  // namespace gtap
 
 using NqProgram = gtap::nq::Program;
-static_assert(NqProgram::kFrameBytes + gtap::kHdrBytes <= gtap::host::kBuiltinStride, "frame exceeds stride");
+using NqSlotProgram = gtap::nq::SlotProgram;
+static_assert(NqSlotProgram::kFrameBytes + gtap::kHdrBytes <= gtap::host::kBuiltinStride, "frame exceeds stride");
 
 // Counts the n-queens placements of an n x n board as one persistent launch
 // (reference program: evaluation/2-comparison/nq/gtap_nq.cu:57-99).
@@ -205,6 +226,19 @@ extern "C" int gtap_nqueens(int n, int64_t* result, gtap_run_stats* stats) {
     gtap_job job = gtap::host::make_job(gtap::nq::F_NQ, 0, &root, sizeof(root));
     int64_t r = 0;
     int st = gtap::host::run_single_job<NqProgram>(NqProgram::Args{n}, job, &r, stats, GTAP_TERM_JOBS);
+    if (st == 0 && result) *result = r;
+    return st;
+}
+
+// The same count with the result-slot join (each child stores its count into the parent's
+// frame, then counts the join down), the form a slot-per-call lowering produces; work-
+// stealing scheduler only.
+extern "C" int gtap_nqueens_slots(int n, int64_t* result, gtap_run_stats* stats) {
+    if (n < 1 || n > gtap::nq::kMaxN) return gtap::host::invalid_argument("gtap_nqueens_slots: n must be in [1, 16]");
+    gtap::nq::NqArg root{0, 0u, 0u, 0u};
+    gtap_job job = gtap::host::make_job(gtap::nq::F_NQ, 0, &root, sizeof(root));
+    int64_t r = 0;
+    int st = gtap::host::run_single_job<NqSlotProgram>(NqSlotProgram::Args{n}, job, &r, stats, GTAP_TERM_JOBS);
     if (st == 0 && result) *result = r;
     return st;
 }

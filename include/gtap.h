@@ -104,8 +104,12 @@ int gtap_fib(int n, int64_t* result, gtap_run_stats* stats);
 
 /* Number of n-queens placements, n in [1, 16]: one task per legal placement, each
  * returning its int64 count through a taskwait; replaces the program of
- * evaluation/2-comparison/nq/gtap_nq.cu:25-99 (which counts with a global atomicAdd). */
+ * evaluation/2-comparison/nq/gtap_nq.cu:25-99 (which counts with a global atomicAdd).
+ * The counts are summed by the additive join (each child's count rides on its join atomic). */
 int gtap_nqueens(int n, int64_t* result, gtap_run_stats* stats);
+/* The same tasks and count with result slots: each child stores its count into the parent's
+ * frame, then counts the join down (no reference counterpart; work-stealing scheduler only). */
+int gtap_nqueens_slots(int n, int64_t* result, gtap_run_stats* stats);
 
 /* Asynchronous BFS from `source` over a CSR graph in device memory (row: nv+1 int32,
  * col: ne int32; depth: nv int32 out, 0x3f3f3f3f = unreached). num_queues 1 = one queue,

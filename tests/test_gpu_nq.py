@@ -27,8 +27,21 @@ def test_nqueens_parity(n):
     assert st["error_code"] == 0
 
 
+@pytest.mark.parametrize("n", [1, 4, 8, 10, 12])
+def test_nqueens_slot_join_parity(n):
+    """The result-slot lowering: same counts and task executions as the additive join."""
+    print(f"case nqueens_slots({n})", flush=True)
+    v, st = gtap_amd.nqueens_slots(n)
+    sols, tasks = oracle.nq(n)
+    assert v == sols
+    assert st["tasks_executed"] == tasks + oracle.nq_internal(n)
+    assert st["error_code"] == 0
+
+
 def test_nqueens_rejects_out_of_range():
     with pytest.raises(gtap_amd.GtapError):
         gtap_amd.nqueens(0)
     with pytest.raises(gtap_amd.GtapError):
         gtap_amd.nqueens(17)
+    with pytest.raises(gtap_amd.GtapError):
+        gtap_amd.nqueens_slots(17)
