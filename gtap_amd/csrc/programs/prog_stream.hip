@@ -3,9 +3,10 @@
 // loop while the kernel runs), and the plain form, one launch per job.
 //
 // Task program (two EPAQ queues: queue 0 = per-lane tasks, queue 1 = wave-cooperative tasks):
-//   fib(n)                       lane  (evaluation/2-comparison/fib/gtap_fib.cu:10-22, int64)
+//   fib(n)                       lane  (evaluation/2-comparison/fib/gtap_fib.cu:10-22, int64,
+//                                       children summed by the additive join)
 //   nq(row, cols, diag, anti, N) lane  (evaluation/2-comparison/nq/gtap_nq.cu:25-55 shape,
-//                                       taskwait + int64 sum, N per job)
+//                                       taskwait + int64 sum by the additive join, N per job)
 //   sort_root(arr, tmp, n)       lane  spawns msort on queue 1, taskwait, returns n
 //   msort(a, t, n, into_t)       wave  2-way mergesort of examples/mergesort/mergesort.cu:34-50:
 //                                      n <= 1024 -> the wave sorts in LDS; else two msort
@@ -61,8 +62,6 @@ struct MergeArg {
     int64_t na;
     int64_t nb;
 };
-constexpr int kFibA = 8, kFibB = 16;
-constexpr int kNqK = 24, kNqCounts = 32;
 constexpr int kDummySlot = 40;  // result slot of void children (never read)
 
 struct StreamArgs {
@@ -74,7 +73,7 @@ struct Program {
     static constexpr int kWaveQueue = Q_WAVE;
     static constexpr int kFrameWords = 10;
     static constexpr uint32_t kMaxFunc = F_MERGE;
-    static constexpr uint32_t kFrameBytes = kNqCounts + 8 * 16;
+    static constexpr uint32_t kFrameBytes = 48;  // the largest arguments (merge: 40 bytes) and the dummy slot
     using Args = StreamArgs;
     struct WaveShared {
         uint32_t keys[kWaveLeaf];
@@ -97,13 +96,13 @@ __device__ __forceinline__ void fib_task(L& lane) {
                 lane.ret((int64_t)n);
                 return;
             }
-            lane.spawn(F_FIB, Q_LANE, kFibA, FibArg{n - 1, 0});
-            lane.spawn(F_FIB, Q_LANE, kFibB, FibArg{n - 2, 0});
-            if (lane.taskwait(1, Q_LANE)) return;
+            lane.spawn_sum(F_FIB, Q_LANE, FibArg{n - 1, 0});
+            lane.spawn_sum(F_FIB, Q_LANE, FibArg{n - 2, 0});
+            if (lane.taskwait_sum(1, Q_LANE)) return;
         }
             [[fallthrough]];
         case 1:
-            lane.ret(lane.template frame_ld<int64_t>(kFibA) + lane.template frame_ld<int64_t>(kFibB));
+            lane.ret(lane.join_sum());
             return;
         default:
             lane.finish();
@@ -130,24 +129,19 @@ __device__ __forceinline__ void nq_task(L& lane) {
             while (avail) {
                 const uint32_t p = avail & (0u - avail);
                 avail -= p;
-                lane.spawn(F_NQ, Q_LANE, kNqCounts + 8 * k, NqArg{row + 1, cols | p, (diag | p) << 1, (anti | p) >> 1, N, 0});
+                lane.spawn_sum(F_NQ, Q_LANE, NqArg{row + 1, cols | p, (diag | p) << 1, (anti | p) >> 1, N, 0});
                 ++k;
             }
             if (k == 0) {
                 lane.ret((int64_t)0);
                 return;
             }
-            lane.template frame_st<int32_t>(kNqK, k);
-            if (lane.taskwait(1, Q_LANE)) return;
+            if (lane.taskwait_sum(1, Q_LANE)) return;
         }
             [[fallthrough]];
-        case 1: {
-            const int32_t k = lane.template frame_ld<int32_t>(kNqK);
-            int64_t s = 0;
-            for (int32_t i = 0; i < k; ++i) s += lane.template frame_ld<int64_t>(kNqCounts + 8 * i);
-            lane.ret(s);
+        case 1:
+            lane.ret(lane.join_sum());
             return;
-        }
         default:
             lane.finish();
             return;
