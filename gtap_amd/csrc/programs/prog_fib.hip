@@ -33,6 +33,7 @@ struct NArg {
 struct Program {
     static constexpr int kQueues = 1;
     static constexpr bool kSchedVariants = true;  // also built with the global-queue and Chase-Lev schedulers
+    static constexpr int kWavesPerSimd = 4;
     static constexpr int kFrameWords = 2;  // n (+pad)
     static constexpr uint32_t kMaxFunc = F_FIB;
     static constexpr uint32_t kFrameBytes = 16;  // covers the 4 frame words loaded with a task

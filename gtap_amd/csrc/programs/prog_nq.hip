@@ -51,6 +51,7 @@ struct ProgramT {
     static constexpr int kQueues = 1;
     // the default form is also built with the global-queue and Chase-Lev schedulers
     static constexpr bool kSchedVariants = kSum;
+    static constexpr int kWavesPerSimd = kSum ? 4 : 1;
     static constexpr int kFrameWords = kSum ? 4 : 6;  // row, cols, diag, anti (, k, pad)
     static constexpr uint32_t kMaxFunc = F_NQ;
     static constexpr uint32_t kFrameBytes = kSum ? 16 : kOffCounts + 8 * kMaxN;
