@@ -12,7 +12,7 @@
 //                                      n <= 1024 -> the wave sorts in LDS; else two msort
 //                                      children (ping-pong buffers), taskwait, then merge
 //   merge(a, na, b, nb, dst)     wave  cilkmerge split of gtap_cilksort.cu:89-127 down to
-//                                      16384 keys, merged by the wave with merge-path tiles
+//                                      4096 keys, merged by the wave with merge-path tiles
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -29,7 +29,7 @@ namespace stream {
 enum : uint32_t { F_FIB = 1, F_NQ = 2, F_SROOT = 3, F_MSORT = 4, F_MERGE = 5 };
 enum : uint32_t { Q_LANE = 0, Q_WAVE = 1 };
 constexpr int kWaveLeaf = 1024;       // keys sorted by one wave in LDS
-constexpr int kWaveMergeLeaf = 16384; // keys merged by one wave
+constexpr int kWaveMergeLeaf = 4096;  // keys merged by one wave
 
 struct FibArg {
     int32_t n;
