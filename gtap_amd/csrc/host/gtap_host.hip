@@ -449,7 +449,7 @@ int gtap_get_runtime_error_code(int* code) {
     Runtime& R = rt();
     if (!code) return invalid_argument("null pointer");
     if (!R.initialized) {
-        *code = 0;
+        *code = R.last_error_code;  // a host-side error (e.g. a refused configuration)
         return 0;
     }
     int32_t e = 0;

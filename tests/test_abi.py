@@ -68,3 +68,17 @@ def test_config_fields_and_scheduler_check_without_gpu():
     assert gtap_amd.gtap_set_config() == 0
     c = gtap_amd.gtap_get_config()
     assert c["scheduler"] == 1 and c["idle_sleep"] == 32 and c["probe_lanes"] == 8
+
+
+def test_initialize_rejects_bad_layout_without_gpu():
+    """Record strides beyond 65520 bytes (result offsets must stay below the additive join's
+    16-bit marker), odd deque capacities and queue counts are refused before any device call."""
+    if not os.path.exists(gtap_amd.LIB_PATH):
+        pytest.skip("libgtap_amd.so not built")
+    try:
+        for bad in ({"record_stride": 65536}, {"record_stride": 40}, {"deque_capacity": 5000}, {"num_queues": 9}):
+            assert gtap_amd.gtap_set_config(**bad) == 0
+            assert gtap_amd.gtap_initialize() == 1, bad  # hipErrorInvalidValue
+            assert gtap_amd.gtap_get_runtime_error_code() == 11, bad  # GTAP_ERROR_INVALID_ARGUMENT
+    finally:
+        assert gtap_amd.gtap_set_config() == 0
