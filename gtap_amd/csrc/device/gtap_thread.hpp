@@ -1238,7 +1238,9 @@ __device__ __forceinline__ void backoff_sleep(uint32_t level) {
     }
 }
 
-template <class P, int Sched>
+// Prof: the kernel records the timeline (gtap_set_profiling). It is a separate build of the
+// kernel, because the timeline's loop state costs registers every launch would pay for.
+template <class P, int Sched, bool Prof>
 __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& args, WaveCtxOf<P>* w,
                                  void* wave_shared) {
     constexpr int Q = P::kQueues;
@@ -1296,7 +1298,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     // index are taken)
     uint32_t seen_next = dev.jobs != nullptr ? dev.static_jobs : 0u;
     LoopStats st = {0, 0, 0, 0, 0, 0, 0, 0};
-    // timeline profiling (lane 0, when dev.prof_max > 0)
+    // timeline profiling (lane 0, in the Prof build when dev.prof_max > 0)
     uint32_t ph_idx = 0, pw_idx = 0;
     bool had_tasks = false;
 
@@ -1339,7 +1341,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 }
             }
             if (n == 0) {
-                if (had_tasks) {
+                if (Prof && had_tasks) {
                     had_tasks = false;
                     if (dev.prof_max && lane == 0) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, ph_idx, 0);
                 }
@@ -1396,11 +1398,11 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         st.rounds += 1;
         st.tasks += n;
         const uint32_t round_n = n;
-        if (dev.prof_max && lane == 0) {
+        if (Prof && dev.prof_max && lane == 0) {
             if (!had_tasks) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, ph_idx, 0);
             prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, (int32_t)round_n);
         }
-        had_tasks = true;
+        if (Prof) had_tasks = true;
         uint32_t fin_tid = 0, fin_parent = kNoParent;
         uint64_t fin_delta = ~0ull;  // added to the parent's join word: -1, plus value << 16 (additive join)
         bool fin_sum = false;        // the parent waits in an additive join
@@ -1616,7 +1618,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         my_slot = next_slot;
         my_lds = next_lds;
         batch_q = (int)kmax;
-        if (dev.prof_max && lane == 0) prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, (int32_t)round_n);
+        if (Prof && dev.prof_max && lane == 0) prof_stamp(dev.prof_working, dev.prof_counts, dev.prof_max, me, pw_idx, (int32_t)round_n);
 
         busy_iter += 1;
         if ((busy_iter & 63u) == 0) {
@@ -1664,12 +1666,12 @@ struct WaveSharedOf<P, decltype((void)sizeof(typename P::WaveShared))> {
     using type = typename P::WaveShared;
 };
 
-template <class P, int Sched = kSchedWS>
+template <class P, int Sched = kSchedWS, bool Prof = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WavesPerSimd<P>::value)))
 void thread_kernel(GtapDev dev, typename P::Args args) {
     __shared__ WaveCtxOf<P> ctxs[4];
     __shared__ typename WaveSharedOf<P>::type wsh[4];
-    thread_scheduler<P, Sched>(dev, args, &ctxs[threadIdx.x / kWave], &wsh[threadIdx.x / kWave]);
+    thread_scheduler<P, Sched, Prof>(dev, args, &ctxs[threadIdx.x / kWave], &wsh[threadIdx.x / kWave]);
 }
 
 }  // namespace gtap

@@ -118,7 +118,7 @@ int occupancy_blocks(const void* kernel, uint32_t requested, uint32_t* blocks_ou
 // Persistent launch of a thread-mode program over `njobs` jobs of the job table `jobs`.
 // Checks a thread-mode launch against the provisioned runtime and picks its workgroup count
 // (every co-resident workgroup unless gtap_config.blocks asks for fewer).
-template <class P, int Sched = kSchedWS>
+template <class P, int Sched = kSchedWS, bool Prof = false>
 int thread_launch_shape(uint32_t job_ring, uint32_t* blocks_out) {
     Runtime& R = rt();
     if (!R.initialized) {
@@ -129,7 +129,7 @@ int thread_launch_shape(uint32_t job_ring, uint32_t* blocks_out) {
     if (P::kFrameBytes + kHdrBytes > R.stride) return invalid_argument("task frame exceeds the record stride");
     if (job_ring == 0 || job_ring > R.nanchors) return invalid_argument("job ring exceeds the anchor capacity");
     static uint32_t max_blocks = 0;
-    const void* kfn = reinterpret_cast<const void*>(&thread_kernel<P, Sched>);
+    const void* kfn = reinterpret_cast<const void*>(&thread_kernel<P, Sched, Prof>);
     if (max_blocks == 0) {
         int st = occupancy_blocks(kfn, 0, &max_blocks);
         if (st) return st;
@@ -154,12 +154,12 @@ struct HasSchedVariants<P, decltype((void)P::kSchedVariants)> {
     static constexpr bool value = P::kSchedVariants;
 };
 
-template <class P, int Sched>
-int launch_thread_as(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs,
-                     uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream, const gtap_job* job0) {
+template <class P, int Sched, bool Prof>
+int launch_thread_prof(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs,
+                       uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream, const gtap_job* job0) {
     Runtime& R = rt();
     uint32_t blocks = 0;
-    if (int st = thread_launch_shape<P, Sched>(job_ring, &blocks)) return st;
+    if (int st = thread_launch_shape<P, Sched, Prof>(job_ring, &blocks)) return st;
     if (Sched == kSchedGQ) {
         if (int st = ensure_global_queues()) return st;
     }
@@ -168,12 +168,20 @@ int launch_thread_as(const typename P::Args& args, uint32_t term_mode, uint32_t 
     int st = reset_launch_state(dev, stream, job0);
     if (st) return st;
     GTAP_HIP_TRY(hipEventRecord(R.ev0, stream));
-    hipLaunchKernelGGL((thread_kernel<P, Sched>), dim3(blocks), dim3(256), 0, stream, dev, args);
+    hipLaunchKernelGGL((thread_kernel<P, Sched, Prof>), dim3(blocks), dim3(256), 0, stream, dev, args);
     GTAP_HIP_TRY(hipGetLastError());
     GTAP_HIP_TRY(hipEventRecord(R.ev1, stream));
     if ((st = enqueue_summary(dev, stream, job0 != nullptr))) return st;
     GTAP_HIP_TRY(hipStreamSynchronize(stream));
     return finish_launch(dev, blocks, stats);
+}
+
+// The timeline-profiling build of the kernel when profiling is on, the plain one otherwise.
+template <class P, int Sched>
+int launch_thread_as(const typename P::Args& args, uint32_t term_mode, uint32_t njobs, gtap_job* jobs,
+                     uint32_t job_ring, gtap_run_stats* stats, hipStream_t stream, const gtap_job* job0) {
+    if (rt().prof_max) return launch_thread_prof<P, Sched, true>(args, term_mode, njobs, jobs, job_ring, stats, stream, job0);
+    return launch_thread_prof<P, Sched, false>(args, term_mode, njobs, jobs, job_ring, stats, stream, job0);
 }
 
 // Launches with the scheduler gtap_config.scheduler names (batched work stealing unless a
@@ -215,7 +223,10 @@ int capture_thread(ThreadGraph& g, const typename P::Args& args, uint32_t term_m
                    const gtap_job* jobs_template_dev, uint32_t job_ring) {
     Runtime& R = rt();
     uint32_t blocks = 0;
-    if (int st = thread_launch_shape<P>(job_ring, &blocks)) return st;
+    const bool prof = R.prof_max != 0;
+    if (int st = prof ? thread_launch_shape<P, kSchedWS, true>(job_ring, &blocks)
+                      : thread_launch_shape<P, kSchedWS, false>(job_ring, &blocks))
+        return st;
     GtapDev dev = make_dev(blocks * 4, term_mode, njobs, R.jobs, job_ring);
     if (g.exec && g.blocks == blocks && memcmp(&g.dev, &dev, sizeof(GtapDev)) == 0) return 0;
     destroy_graph(g);
@@ -225,7 +236,8 @@ int capture_thread(ThreadGraph& g, const typename P::Args& args, uint32_t term_m
                                   hipMemcpyDeviceToDevice, s);
     int st = (e == hipSuccess) ? enqueue_reset(dev, s) : hip_status(e, "hipMemcpyAsync (capture)");
     if (st == 0) {
-        hipLaunchKernelGGL(thread_kernel<P>, dim3(blocks), dim3(256), 0, s, dev, args);
+        if (prof) hipLaunchKernelGGL((thread_kernel<P, kSchedWS, true>), dim3(blocks), dim3(256), 0, s, dev, args);
+        else hipLaunchKernelGGL((thread_kernel<P, kSchedWS, false>), dim3(blocks), dim3(256), 0, s, dev, args);
         e = hipGetLastError();
         if (e != hipSuccess) st = hip_status(e, "thread_kernel (capture)");
     }

@@ -670,13 +670,16 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
         hctl[HCTL_STOP * 32] = 0;
         __atomic_thread_fence(__ATOMIC_SEQ_CST);
         // the launch itself (asynchronous): served mode, admission by live jobs and free ids
-        static uint32_t max_blocks = 0;
-        const void* kfn = reinterpret_cast<const void*>(&thread_kernel<StreamProgram>);
-        if (max_blocks == 0) {
-            st = host::occupancy_blocks(kfn, 0, &max_blocks);
+        // the timeline-profiling build of the kernel when profiling is on
+        const bool prof = R.prof_max != 0;
+        static uint32_t max_blocks[2] = {0, 0};
+        const void* kfn = prof ? reinterpret_cast<const void*>(&thread_kernel<StreamProgram, kSchedWS, true>)
+                               : reinterpret_cast<const void*>(&thread_kernel<StreamProgram, kSchedWS, false>);
+        if (max_blocks[prof] == 0) {
+            st = host::occupancy_blocks(kfn, 0, &max_blocks[prof]);
             if (st) return st;
         }
-        uint32_t blocks = max_blocks;
+        uint32_t blocks = max_blocks[prof];
         if (R.cfg.blocks && R.cfg.blocks < blocks) blocks = R.cfg.blocks;
         if (blocks * 4 > R.max_workers) blocks = R.max_workers / 4;
         GtapDev dev = host::make_dev(blocks * 4, GTAP_TERM_SERVE, (uint32_t)n, R.jobs_host_dev, (uint32_t)n);
@@ -688,8 +691,12 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
         st = host::reset_launch_state(dev, R.stream);
         if (st) return st;
         GTAP_HIP_TRY(hipEventRecord(R.ev0, R.stream));
-        hipLaunchKernelGGL(thread_kernel<StreamProgram>, dim3(blocks), dim3(256), 0, R.stream, dev,
-                           StreamProgram::Args{0});
+        if (prof)
+            hipLaunchKernelGGL((thread_kernel<StreamProgram, kSchedWS, true>), dim3(blocks), dim3(256), 0, R.stream, dev,
+                               StreamProgram::Args{0});
+        else
+            hipLaunchKernelGGL((thread_kernel<StreamProgram, kSchedWS, false>), dim3(blocks), dim3(256), 0, R.stream, dev,
+                               StreamProgram::Args{0});
         GTAP_HIP_TRY(hipGetLastError());
         GTAP_HIP_TRY(hipEventRecord(R.ev1, R.stream));
         st = host::enqueue_summary(dev, R.stream, false);
