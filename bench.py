@@ -165,7 +165,8 @@ def bench_cilksort(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
     return {"kernel_ms_median": med, "keys_per_s": n / (med / 1e3), "bit_exact": ok,
             "tasks_per_s": s.tasks_executed / (med / 1e3),
             "roofline": {"bound": "hbm", "achieved": gbps, "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
-                         "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas}}
+                         "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas,
+                         "traffic": load_traffic("cilksort_2e26")}}
 
 
 def bench_bfs(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
@@ -203,7 +204,8 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
             "epaq_speedup": res[1][0] / med4, "reached_vertices": vr, "reached_edges": er,
             "edges_per_s": er / (med4 / 1e3), "tasks_executed_epaq4": tasks4,
             "roofline": {"bound": "hbm", "achieved": gbps, "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
-                         "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas}}
+                         "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas,
+                         "algorithmic_bytes": roofline.bfs_bytes(vr, er), "traffic": load_traffic("bfs_rmat22")}}
 
 
 def bench_spmv_tree(gtap_amd, reps: int = 3) -> dict:
