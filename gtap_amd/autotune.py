@@ -32,7 +32,7 @@ DEFAULT_SPACE: Dict[str, Sequence[int]] = {
     "blocks": (0, 256, 512, 1024, 1536),
     "probe_lanes": (8, 1, 4, 16, 32),
     "steal_batch": (64, 8, 16, 32),
-    "deque_capacity": (16384, 65536),
+    "deque_capacity": (32768, 65536),
 }
 
 Measure = Callable[[Dict[str, int]], Tuple[float, object]]

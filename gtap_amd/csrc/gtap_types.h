@@ -105,7 +105,7 @@ enum {
 typedef struct gtap_config {
     uint64_t pool_bytes;      /* bytes of task records (default and maximum 4 GiB) */
     uint32_t record_stride;   /* bytes per task record (default: max over built-in programs) */
-    uint32_t deque_capacity;  /* slots per worker per queue, power of two (default 16384) */
+    uint32_t deque_capacity;  /* slots per worker per queue, power of two (default 32768) */
     uint32_t num_queues;      /* EPAQ queues provisioned (default 4) */
     uint32_t blocks;          /* workgroups per launch (default: all co-resident) */
     uint32_t steal_batch;     /* max tasks per steal, <= 64 (default 64) */

@@ -327,7 +327,7 @@ using namespace gtap;
 static void apply_defaults(gtap_config* c) {
     if (!c->pool_bytes) c->pool_bytes = 4ull << 30;
     if (!c->record_stride) c->record_stride = builtin_record_stride();
-    if (!c->deque_capacity) c->deque_capacity = 16384;
+    if (!c->deque_capacity) c->deque_capacity = 32768;
     if (!c->num_queues) c->num_queues = 4;
     if (!c->steal_batch || c->steal_batch > 64) c->steal_batch = 64;
     if (!c->probe_lanes || c->probe_lanes > 64) c->probe_lanes = 8;
