@@ -374,7 +374,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
-              f"the workload would shard by subtrees, see DESIGN.md)", file=sys.stderr)
+              f"the workload would shard by independent subtrees of the task tree)", file=sys.stderr)
         return 2
 
     import gtap_amd
