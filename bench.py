@@ -373,8 +373,8 @@ def main() -> int:
                     help="time the n-queens workload only (no fib/cilksort/BFS/stream extras); for profiler runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here has one GPU; "
-              f"the workload would shard by independent subtrees of the task tree)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the workload would shard by "
+              f"independent subtrees of the task tree)", file=sys.stderr)
         return 2
 
     import gtap_amd
