@@ -379,6 +379,7 @@ def main() -> int:
 
     import gtap_amd
     from gtap_amd import roofline
+    from gtap_amd.scaling import median_iqr
 
     nq_solutions, nq_tasks = NQ14_SOLUTIONS, NQ14_TASKS
 
@@ -478,6 +479,9 @@ def main() -> int:
                                                   if k in cpu},
         "extra": {
             "kernel_ms_median": statistics.median(kernel_ms),
+            # distances of the nearest-rank quartiles from the median over the timed steps
+            # (evaluation/1-worker_scalability/fib/worker_scaling_fib.sh:66-104)
+            "kernel_ms_err_low_high": list(median_iqr(kernel_ms)[1:]),
             "kernel_tasks_per_s": nq_tasks / (statistics.median(kernel_ms) / 1e3),
             "atomic_roofline": {
                 "achieved_gops": atomics_per_s / 1e9,
