@@ -411,9 +411,15 @@ __device__ __forceinline__ uint32_t block_admit(const GtapDev& dev, BlockCtx* c,
         atomic_add(dev.ctl + CTL_IDLE * 32, -1);
         idle_registered = false;
     }
-    int32_t expected = (int32_t)next;
-    if (!atomic_cas(nextp, expected, (int32_t)(next + 1))) return 0;
+    // live count first, admission counter second (undone if the claim fails), so a stop rule
+    // reading them in the other order never misses an admitted job
     atomic_add(dev.ctl + CTL_LIVE_JOBS * 32, 1);
+    complete_stores();
+    int32_t expected = (int32_t)next;
+    if (!atomic_cas(nextp, expected, (int32_t)(next + 1))) {
+        atomic_add_noret(dev.ctl + CTL_LIVE_JOBS * 32, -1);
+        return 0;
+    }
     return block_start_job<P>(dev, c, next);
 }
 
@@ -425,7 +431,8 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
     BlockLoop& lp = c->lp;
     if (t == 0) {
         lp.t_begin = now_ticks();
-        if (dev.term_mode == GTAP_TERM_QUIESCE) atomic_add(dev.ctl + CTL_STARTED * 32, 1);
+        // a workgroup that starts a job by index was counted as started by the reset
+        if (dev.term_mode == GTAP_TERM_QUIESCE && me >= dev.static_jobs) atomic_add(dev.ctl + CTL_STARTED * 32, 1);
         lp.host_poll_last = 0;
         lp.rng = 0x9E3779B9u ^ (me * 0x85EBCA6Bu) ^ (uint32_t)lp.t_begin;
         if (lp.rng == 0) lp.rng = 1;

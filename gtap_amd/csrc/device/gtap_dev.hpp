@@ -35,9 +35,11 @@ enum CtlIndex : int {
     CTL_LIVE_JOBS = 6,  // admitted and not yet completed
     CTL_HOST_AVAIL = 7,   // serve mode: device mirror of HCTL_JOB_AVAIL
     CTL_HOST_STOP = 8,    // serve mode: device mirror of HCTL_STOP
-    CTL_STARTED = 9,      // workers that have entered the scheduler loop (quiescence counts
-                          // these, not the launch's worker count: a worker the hardware has
-                          // not made resident yet holds no work and must not be waited for)
+    CTL_STARTED = 9,      // workers that have entered the scheduler loop, plus the workers
+                          // that start a job by index, counted from the reset (quiescence
+                          // counts these, not the launch's worker count: a worker the hardware
+                          // has not made resident yet and that owns no job holds no work and
+                          // must not be waited for; one that owns a job must)
     CTL_PROGRAM = 10,     // a counter a program may use (cleared with the others per launch)
     CTL_COUNT = 12
 };

@@ -983,9 +983,14 @@ __device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtxOf<P>*
     }
     int ok = 0;
     if (lane == 0) {
+        // the live count is raised before the admission counter moves (and lowered again if
+        // the claim fails): a served run's stop rule reads the counter, then the live count,
+        // and must not see the jobs admitted but not yet live
+        atomic_add(dev.ctl + CTL_LIVE_JOBS * 32, (int32_t)k);
+        complete_stores();
         int32_t expected = (int32_t)next;
         ok = atomic_cas(nextp, expected, (int32_t)(next + k)) ? 1 : 0;
-        if (ok) atomic_add(dev.ctl + CTL_LIVE_JOBS * 32, (int32_t)k);
+        if (!ok) atomic_add_noret(dev.ctl + CTL_LIVE_JOBS * 32, -(int32_t)k);
     }
     ok = shfl((int32_t)ok, 0);
     if (!ok) return 0;
@@ -1265,7 +1270,8 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             vst(&w->valid[q], (uint64_t)0);
             vst(&w->inlds[q], (uint64_t)0);
         }
-        if (dev.term_mode == GTAP_TERM_QUIESCE) atomic_add(dev.ctl + CTL_STARTED * 32, 1);
+        // a worker that starts a job by index was counted as started by the reset
+        if (dev.term_mode == GTAP_TERM_QUIESCE && me >= dev.static_jobs) atomic_add(dev.ctl + CTL_STARTED * 32, 1);
     }
     __builtin_amdgcn_wave_barrier();
     // the worker's first ids are a fixed range (the shared fresh-id counter starts above
@@ -1372,6 +1378,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                         if (stop) {
                             const uint32_t avail = (uint32_t)ld32(dev.ctl + CTL_HOST_AVAIL * 32);
                             const uint32_t next = (uint32_t)ld32(dev.ctl + CTL_JOB_NEXT * 32);
+                            complete_stores();  // the live count is read after the admission counter
                             const int32_t live = ld32(dev.ctl + CTL_LIVE_JOBS * 32);
                             if (next >= avail && live == 0) {
                                 st32(dev.ctl + CTL_DONE * 32, 1);

@@ -49,6 +49,30 @@ int invalid_argument(const char* msg) {
     return (int)hipErrorInvalidValue;
 }
 
+int check_csr_rows(const int32_t* row, int64_t n, int64_t ne, const char* who) {
+    char msg[160];
+    if (row == nullptr || n < 0 || ne < 0) {
+        snprintf(msg, sizeof(msg), "%s: bad CSR arguments", who);
+        return invalid_argument(msg);
+    }
+    if (row[0] != 0) {
+        snprintf(msg, sizeof(msg), "%s: row offsets must start at 0 (found %d)", who, row[0]);
+        return invalid_argument(msg);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        if (row[i + 1] < row[i]) {
+            snprintf(msg, sizeof(msg), "%s: row offsets decrease at row %lld", who, (long long)i);
+            return invalid_argument(msg);
+        }
+    }
+    if ((int64_t)row[n] != ne) {
+        snprintf(msg, sizeof(msg), "%s: last row offset %d differs from the entry count %lld", who, row[n],
+                 (long long)ne);
+        return invalid_argument(msg);
+    }
+    return 0;
+}
+
 int hip_status(hipError_t e, const char* what) {
     fprintf(stderr, "GTaP HIP error: %s in %s\n", hipGetErrorString(e), what);
     return (int)e;
@@ -131,8 +155,14 @@ __global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0) {
     const uint64_t gs = (uint64_t)gridDim.x * blockDim.x;
     if (write_job0 && gid < sizeof(gtap_job) / 4)
         reinterpret_cast<uint32_t*>(dev.jobs)[gid] = reinterpret_cast<const uint32_t*>(&job0)[gid];
+    // The workers that start a job by index count as started from the outset (quiescence
+    // compares the idle count with the started count): a worker that owns a job but is not
+    // resident yet must keep every other worker from taking the launch as finished.
     for (uint64_t i = gid; i < (uint64_t)CTL_COUNT * 32; i += gs)
-        dev.ctl[i] = (i == CTL_JOB_NEXT * 32 || i == CTL_LIVE_JOBS * 32) ? (int32_t)dev.static_jobs : 0;
+        dev.ctl[i] = (i == CTL_JOB_NEXT * 32 || i == CTL_LIVE_JOBS * 32 ||
+                      (i == CTL_STARTED * 32 && dev.term_mode == GTAP_TERM_QUIESCE))
+                         ? (int32_t)dev.static_jobs
+                         : 0;
     for (uint64_t i = gid; i < (uint64_t)dev.nqueues * dev.nworkers; i += gs) dev.dq_word[i * 16] = 0;
     for (uint64_t i = gid; i < (uint64_t)dev.nshards * dev.ncells; i += gs)
         dev.pool_seq[i] = (uint32_t)(i & (dev.ncells - 1));

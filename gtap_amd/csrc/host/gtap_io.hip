@@ -60,6 +60,9 @@ int gtap_csr_file_read(const char* path, int32_t* row, int32_t* col) {
     File F(path, "rb");
     if (!F.f || fseek(F.f, 8, SEEK_SET) != 0) return bad("gtap_csr_file_read: cannot open file");
     if (!F.read(row, 4 * ((size_t)nv + 1)) || !F.read(col, 4 * (size_t)ne)) return bad("gtap_csr_file_read: short read");
+    if (int st = gtap::host::check_csr_rows(row, nv, ne, "gtap_csr_file_read")) return st;
+    for (int64_t i = 0; i < ne; ++i)
+        if (col[i] < 0 || col[i] >= nv) return bad("gtap_csr_file_read: column index out of range");
     return 0;
 }
 
@@ -122,6 +125,9 @@ int gtap_spmv_file_read(const char* path, int32_t* row_ptr, int32_t* col_idx, do
     if (!F.f || fseek(F.f, 12, SEEK_SET) != 0) return bad("gtap_spmv_file_read: cannot open file");
     if (!F.read(row_ptr, 4 * ((size_t)rows + 1)) || !F.read(col_idx, 4 * (size_t)nnz) || !F.read(val, 8 * (size_t)nnz))
         return bad("gtap_spmv_file_read: short read");
+    if (int st = gtap::host::check_csr_rows(row_ptr, rows, nnz, "gtap_spmv_file_read")) return st;
+    for (int64_t i = 0; i < nnz; ++i)
+        if (col_idx[i] < 0 || col_idx[i] >= cols) return bad("gtap_spmv_file_read: column index out of range");
     return 0;
 }
 

@@ -47,6 +47,7 @@ struct BfsArgs {
     int32_t* depth;
     int32_t nv;
     int32_t split;
+    int64_t ne;  // column entries: a vertex's offsets must satisfy 0 <= row[u] <= row[u+1] <= ne
 };
 
 __host__ __device__ __forceinline__ uint32_t degree_bucket(int32_t deg) {
@@ -139,6 +140,10 @@ __device__ __forceinline__ void bfs_task(L& lane) {
             for (int k = 1; k < kChunkEdges; ++k) uj = (j == k) ? u[k] : uj;
             const int32_t us = g.row[uj];
             const int32_t ue = g.row[uj + 1];
+            if (us < 0 || ue < us || (int64_t)ue > g.ne) {  // malformed offsets: never read past col
+                GTAP_DEVICE_ERROR(*lane.dev, GTAP_ERROR_INVALID_ARGUMENT, lane.tid, -1, uj, g.nv);
+                continue;
+            }
             __gtap_spawn_task_detached(lane, F_BFS, route<NQ>(ue - us), BfsArg{uj, us, ue, nd});
         }
     }
@@ -194,7 +199,11 @@ struct BlockProgram {
                     GTAP_DEVICE_ERROR(*blk.dev, GTAP_ERROR_INVALID_ARGUMENT, blk.tid, -1, u, g.nv);
                 } else if (g.depth[u] > d + 1 &&
                            __hip_atomic_fetch_min(g.depth + u, d + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > d + 1) {
-                    blk.spawn_detached(F_BFS, BfsArg{u, g.row[u], g.row[u + 1], d + 1});
+                    const int32_t us = g.row[u], ue = g.row[u + 1];
+                    if (us < 0 || ue < us || (int64_t)ue > g.ne)
+                        GTAP_DEVICE_ERROR(*blk.dev, GTAP_ERROR_INVALID_ARGUMENT, blk.tid, -1, u, g.nv);
+                    else
+                        blk.spawn_detached(F_BFS, BfsArg{u, us, ue, d + 1});
                 }
             }
         }
@@ -239,10 +248,11 @@ extern "C" int gtap_bfs(const int32_t* row, const int32_t* col, int32_t nv, int6
     GTAP_HIP_TRY(hipGetLastError());
     int32_t rs[2];
     GTAP_HIP_TRY(hipMemcpy(rs, row + source, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (rs[0] < 0 || rs[1] < rs[0] || (int64_t)rs[1] > ne) return host::invalid_argument("gtap_bfs: bad source row offsets");
     bfs::BfsArg root{source, rs[0], rs[1], 0};
     const uint32_t q = (num_queues == 4) ? bfs::degree_bucket(rs[1] - rs[0]) : 0u;
     gtap_job job = host::make_job(bfs::F_BFS, q, &root, sizeof(root));
-    bfs::BfsArgs a{row, col, depth, nv, split};
+    bfs::BfsArgs a{row, col, depth, nv, split, ne};
     if (num_queues == 4) return host::run_single_job<Bfs4>(a, job, nullptr, stats, GTAP_TERM_QUIESCE);
     return host::run_single_job<Bfs1>(a, job, nullptr, stats, GTAP_TERM_QUIESCE);
 }
@@ -264,9 +274,11 @@ extern "C" int gtap_bfs_block(const int32_t* row, const int32_t* col, int32_t nv
     GTAP_HIP_TRY(hipGetLastError());
     int32_t rs[2];
     GTAP_HIP_TRY(hipMemcpy(rs, row + source, 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (rs[0] < 0 || rs[1] < rs[0] || (int64_t)rs[1] > ne)
+        return host::invalid_argument("gtap_bfs_block: bad source row offsets");
     bfs::BfsArg root{source, rs[0], rs[1], 0};
     gtap_job job = host::make_job(bfs::F_BFS, 0, &root, sizeof(root));
-    bfs::BfsArgs a{row, col, depth, nv, 0};
+    bfs::BfsArgs a{row, col, depth, nv, 0, ne};
     return host::run_single_block_job<bfs::BlockProgram>(a, job, nullptr, stats, GTAP_TERM_QUIESCE);
 }
 
@@ -274,6 +286,7 @@ extern "C" int gtap_bfs_block(const int32_t* row, const int32_t* col, int32_t nv
 extern "C" int gtap_bfs_host(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne, int32_t source,
                              int32_t* depth_out, int32_t num_queues, int32_t split, gtap_run_stats* stats) {
     if (!row || !col || !depth_out || nv <= 0 || ne < 0) return gtap::host::invalid_argument("gtap_bfs_host");
+    if (int st = gtap::host::check_csr_rows(row, nv, ne, "gtap_bfs_host")) return st;
     int32_t *drow = nullptr, *dcol = nullptr, *ddep = nullptr;
     GTAP_HIP_TRY(hipMalloc(&drow, (size_t)(nv + 1) * sizeof(int32_t)));
     GTAP_HIP_TRY(hipMalloc(&dcol, (size_t)(ne > 0 ? ne : 1) * sizeof(int32_t)));

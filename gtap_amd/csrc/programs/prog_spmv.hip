@@ -54,6 +54,7 @@ struct SpmvArgs {
     int32_t rows;
     int32_t cols;
     int32_t cutoff;
+    int32_t nnz;  // row_ptr[rows]: a row's offsets must satisfy 0 <= row_ptr[i] <= row_ptr[i+1] <= nnz
 };
 
 struct Program {
@@ -79,6 +80,11 @@ __device__ __forceinline__ void row_task(L& lane) {
         return;
     }
     const int32_t kb = g.row_ptr[i], ke = g.row_ptr[i + 1];
+    if (kb < 0 || ke < kb || ke > g.nnz) {  // malformed offsets: never read past col_idx / val
+        GTAP_DEVICE_ERROR(*lane.dev, GTAP_ERROR_INVALID_ARGUMENT, lane.tid, -1, i, g.nnz);
+        __gtap_finish_task(lane);
+        return;
+    }
     double sum = 0.0;
     constexpr int kChunk = 8;
     for (int32_t b = kb; b < ke; b += kChunk) {
@@ -148,9 +154,13 @@ extern "C" int gtap_spmv(const int32_t* row_ptr, const int32_t* col_idx, const d
         if (stats) memset(stats, 0, sizeof(*stats));
         return 0;
     }
+    int32_t nnz = 0;  // the entry count is row_ptr[rows]; every row range is checked against it
+    GTAP_HIP_TRY(hipMemcpy(&nnz, row_ptr + rows, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (nnz < 0) return host::invalid_argument("gtap_spmv: negative entry count");
+    if (nnz > 0 && (!col_idx || !val)) return host::invalid_argument("gtap_spmv: bad arguments");
     spmv::RangeArg root{0, rows};
     gtap_job job = host::make_job(spmv::F_RANGE, 0, &root, sizeof(root));
-    spmv::SpmvArgs a{row_ptr, col_idx, val, x, y, rows, cols, cutoff};
+    spmv::SpmvArgs a{row_ptr, col_idx, val, x, y, rows, cols, cutoff, nnz};
     return host::run_single_job<SpmvProgram>(a, job, nullptr, stats, GTAP_TERM_QUIESCE);
 }
 
@@ -163,6 +173,8 @@ extern "C" int gtap_spmv_host(const int32_t* row_ptr, const int32_t* col_idx, co
     if (rows == 0) return gtap_spmv(row_ptr, col_idx, val, x, y, 0, cols, cutoff, stats);
     const int64_t nnz = row_ptr[rows];
     if (nnz < 0) return host::invalid_argument("gtap_spmv_host: negative entry count");
+    if (int st = host::check_csr_rows(row_ptr, rows, nnz, "gtap_spmv_host")) return st;
+    if (nnz > 0 && (!col_idx || !val)) return host::invalid_argument("gtap_spmv_host: bad arguments");
     int32_t *drp = nullptr, *dci = nullptr;
     double *dv = nullptr, *dx = nullptr, *dy = nullptr;
     int st = 0;

@@ -1,6 +1,7 @@
 """The reference's on-disk formats (graph CSR, key vector, SpMV CSR): the library's readers
 and writers against bytes packed independently with struct, against the key files the
 reference's own gen_vector wrote (tests/golden), and on malformed files. CPU only."""
+import ctypes
 import os
 import struct
 
@@ -113,3 +114,43 @@ def test_tree_input_range_and_determinism():
     a = gtap_amd.tree_input(4096, 1)
     assert a.min() >= -1.0 and a.max() < 1.0 and abs(a.mean()) < 0.05
     assert np.array_equal(a, gtap_amd.tree_input(4096, 1)) and not np.array_equal(a, gtap_amd.tree_input(4096, 2))
+
+
+@pytest.mark.parametrize("row", [
+    [1, 2, 3],        # row[0] != 0
+    [0, 3, 2],        # decreasing offsets
+    [0, 1, 5],        # last offset beyond the entry count
+    [0, 1, 2, 1],     # decreasing at the end
+])
+def test_malformed_csr_offsets_are_rejected_before_any_launch(tmp_path, row):
+    """Row offsets are checked on the host (row[0] == 0, non-decreasing, row[n] == entries)
+    before anything is copied or launched, so no device read can run past the columns."""
+    row = np.array(row, dtype=np.int32)
+    nv = row.size - 1
+    col = np.zeros(3, dtype=np.int32)
+    depth = np.empty(nv, dtype=np.int32)
+    st = gtap_amd.lib().gtap_bfs_host(row.ctypes.data, col.ctypes.data, nv, col.size, 0, depth.ctypes.data, 1, 64,
+                                      None)
+    assert st != 0
+    code = ctypes.c_int(0)
+    gtap_amd.lib().gtap_get_runtime_error_code(ctypes.byref(code))
+    assert code.value == 11  # GTAP_ERROR_INVALID_ARGUMENT
+    x = np.ones(nv, dtype=np.float64)
+    y = np.empty(nv, dtype=np.float64)
+    val = np.ones(3, dtype=np.float64)
+    assert gtap_amd.lib().gtap_spmv_host(row.ctypes.data, col.ctypes.data, val.ctypes.data, x.ctypes.data,
+                                         y.ctypes.data, nv, nv, 16, None) != 0
+    # the same offsets in a graph file are refused by the reader
+    p = str(tmp_path / "bad_rows.bin")
+    with open(p, "wb") as f:
+        f.write(struct.pack("<2i", nv, 3) + row.tobytes() + col.tobytes())
+    with pytest.raises(RuntimeError):
+        gtap_amd.read_csr(p)
+
+
+def test_csr_file_column_out_of_range_is_rejected(tmp_path):
+    p = str(tmp_path / "bad_col.bin")
+    with open(p, "wb") as f:
+        f.write(struct.pack("<2i", 2, 2) + struct.pack("<3i", 0, 1, 2) + struct.pack("<2i", 1, 7))
+    with pytest.raises(RuntimeError):
+        gtap_amd.read_csr(p)
