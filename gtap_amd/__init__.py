@@ -156,6 +156,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
                        ctypes.POINTER(RunStats)], _i),
     "gtap_cilksort": ([_vp, _vp, _i64, _i64, _i64, ctypes.POINTER(RunStats)], _i),
     "gtap_cilksort_host": ([_vp, _i64, _i64, _i64, ctypes.POINTER(RunStats)], _i),
+    "gtap_cilksort_noalloc": ([_vp, _i64, _i64, _i64, ctypes.POINTER(RunStats)], _i),
     "gtap_jobs_generate": ([_u64, ctypes.c_int32, _vp, _vp, _vp], _i),
     "gtap_stream_create": ([_u64, ctypes.c_int32, ctypes.POINTER(_vp)], _i),
     "gtap_stream_destroy": ([_vp], _i),

@@ -400,6 +400,7 @@ int gtap_initialize(void) {
     hipDeviceProp_t prop;
     GTAP_HIP_TRY(hipGetDeviceProperties(&prop, R.device));
     R.num_cus = prop.multiProcessorCount;
+    if (int st = cilksort_scratch_setup(R.device)) return st;
     GTAP_HIP_TRY(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
     GTAP_HIP_TRY(hipEventCreate(&R.ev0));
     GTAP_HIP_TRY(hipEventCreate(&R.ev1));

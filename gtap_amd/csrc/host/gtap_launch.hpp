@@ -97,6 +97,9 @@ int hip_status(hipError_t e, const char* what);
 // Row offsets of a CSR structure in host memory: row[0] == 0, non-decreasing, row[n] == ne.
 // Returns 0, or GTAP_ERROR_INVALID_ARGUMENT's status with `who` in the message.
 int check_csr_rows(const int32_t* row, int64_t n, int64_t ne, const char* who);
+// Resolves the device address of gtap_cilksort_noalloc's static workspace on the current
+// device (prog_cilksort.hip); gtap_initialize calls it.
+int cilksort_scratch_setup(int device);
 gtap_job make_job(uint32_t func, uint32_t queue, const void* args, size_t nbytes);
 GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job* jobs, uint32_t job_ring);
 // Resets the per-launch device state; a non-null job0 is written into job-table slot 0 by

@@ -517,6 +517,55 @@ extern "C" int gtap_cilksort(uint32_t* keys, uint32_t* tmp, int64_t n, int64_t l
     return host::run_single_block_job<CilkProgram>(cilk::CilkArgs{leaf, merge_leaf}, job, nullptr, stats);
 }
 
+namespace gtap {
+namespace cilk {
+
+// Workspace of gtap_cilksort_noalloc: the merge buffer (`tmp` of gtap_cilksort) as a
+// file-scope device array, so the call allocates nothing. Every GPU the library runs on holds
+// its own copy until the process ends. In host code `scratch` names a host-side stand-in, not
+// device memory: kernels get the address hipGetSymbolAddress returns for the device
+// (g_scratch_dev), never the array's name.
+constexpr int64_t kNoallocMaxKeys = (int64_t)1 << 22;  // 16 MiB of uint32
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint32_t scratch[kNoallocMaxKeys];
+static uint32_t* g_scratch_dev[kMaxDevices];  // the device address of `scratch`, per device
+
+}  // namespace cilk
+
+namespace host {
+
+// Resolves the device address of the no-allocation workspace on `device`, which must be the
+// current device. Called once per device by gtap_initialize.
+int cilksort_scratch_setup(int device) {
+    if (device < 0 || device >= cilk::kMaxDevices) return invalid_argument("cilksort_scratch_setup: device index");
+    if (cilk::g_scratch_dev[device]) return 0;
+    void* p = nullptr;
+    GTAP_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(cilk::scratch)));
+    if (p == nullptr) return fail(GTAP_ERROR_HIP, "hipGetSymbolAddress returned no address for the cilksort workspace");
+    cilk::g_scratch_dev[device] = static_cast<uint32_t*>(p);
+    return 0;
+}
+
+}  // namespace host
+}  // namespace gtap
+
+// gtap_cilksort with the library's static workspace instead of a caller's tmp buffer: no
+// allocation of any kind, for callers that cannot allocate (under HIP graph capture, in a
+// serving loop). The runtime must be initialized; n <= 2^22 keys. Calls on one GPU share the
+// workspace, so calls that overlap on one GPU race.
+extern "C" int gtap_cilksort_noalloc(uint32_t* keys, int64_t n, int64_t leaf, int64_t merge_leaf,
+                                     gtap_run_stats* stats) {
+    using namespace gtap;
+    host::Runtime& R = host::rt();
+    if (!R.initialized) return host::fail(GTAP_ERROR_NOT_INITIALIZED, "gtap_cilksort_noalloc before gtap_initialize");
+    if (!keys || n < 0) return host::invalid_argument("gtap_cilksort_noalloc: bad arguments");
+    if (n > cilk::kNoallocMaxKeys)
+        return host::invalid_argument("gtap_cilksort_noalloc: n exceeds the workspace of 4194304 keys");
+    uint32_t* tmp = (R.device >= 0 && R.device < cilk::kMaxDevices) ? cilk::g_scratch_dev[R.device] : nullptr;
+    if (tmp == nullptr) return host::fail(GTAP_ERROR_NOT_INITIALIZED, "gtap_cilksort_noalloc: workspace not resolved");
+    return gtap_cilksort(keys, tmp, n, leaf, merge_leaf, stats);
+}
+
 // Host-buffer convenience: copies keys in, sorts, copies the sorted keys back.
 extern "C" int gtap_cilksort_host(uint32_t* keys, int64_t n, int64_t leaf, int64_t merge_leaf, gtap_run_stats* stats) {
     if (n < 0 || (n > 0 && !keys)) return gtap::host::invalid_argument("gtap_cilksort_host");

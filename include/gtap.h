@@ -128,6 +128,13 @@ int gtap_bfs_host(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne
  * are merged by one workgroup. Replaces evaluation/2-comparison/cilksort/gtap_cilksort.cu:89-281. */
 int gtap_cilksort(uint32_t* keys, uint32_t* tmp, int64_t n, int64_t leaf, int64_t merge_leaf,
                   gtap_run_stats* stats);
+/* gtap_cilksort without a tmp argument and without allocating anything: the merge buffer is
+ * the library's static device workspace (one per GPU, held until the process ends). No
+ * counterpart in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). Limit: n <= 4194304 keys (16 MiB);
+ * larger n is refused with GTAP_ERROR_INVALID_ARGUMENT, and a call before gtap_initialize with
+ * GTAP_ERROR_NOT_INITIALIZED. Calls that overlap on one GPU share the workspace and race. */
+int gtap_cilksort_noalloc(uint32_t* keys, int64_t n, int64_t leaf, int64_t merge_leaf, gtap_run_stats* stats);
 /* The same with a host buffer (copied in and out around the launch). */
 int gtap_cilksort_host(uint32_t* keys, int64_t n, int64_t leaf, int64_t merge_leaf, gtap_run_stats* stats);
 

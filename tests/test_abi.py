@@ -82,3 +82,12 @@ def test_initialize_rejects_bad_layout_without_gpu():
             assert gtap_amd.gtap_get_runtime_error_code() == 11, bad  # GTAP_ERROR_INVALID_ARGUMENT
     finally:
         assert gtap_amd.gtap_set_config() == 0
+
+
+def test_noalloc_entry_refuses_an_uninitialized_runtime_without_gpu():
+    """gtap_cilksort_noalloc never initializes (initialization allocates): before
+    gtap_initialize it returns GTAP_ERROR_NOT_INITIALIZED without any device call."""
+    if not os.path.exists(gtap_amd.LIB_PATH):
+        pytest.skip("libgtap_amd.so not built")
+    assert gtap_amd.lib().gtap_cilksort_noalloc(None, 16, 0, 0, None) != 0
+    assert gtap_amd.gtap_get_runtime_error_code() == 12  # GTAP_ERROR_NOT_INITIALIZED
