@@ -296,13 +296,94 @@ def bench_reference_settings(gtap_amd, reps: int = 5) -> dict:
     return out
 
 
-def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024,
-                 serve_rates=(500.0, 2000.0, 8000.0, 32000.0), serve_jobs: int = 1024) -> dict:
+def _rate_at_p99(points, form: str, p99_ms: float):
+    """Largest arrival rate of the sweep at which `form` kept its p99 within p99_ms, with
+    log-linear interpolation to the crossing between the last point inside and the first
+    point outside; None if even the lowest rate exceeds it. ('lower_bound' when the form is
+    still inside at the highest rate swept.)"""
+    import math
+    inside = None
+    for i, p in enumerate(points):
+        if form not in p:
+            break
+        if p[form]["p99_ms"] <= p99_ms:
+            inside = i
+            continue
+        if inside is None:
+            return None, False
+        a, b = points[inside], p
+        la, lb = math.log(a[form]["p99_ms"]), math.log(b[form]["p99_ms"])
+        t = (math.log(p99_ms) - la) / (lb - la) if lb > la else 0.0
+        ra, rb = math.log(a["rate_jobs_per_s"]), math.log(b["rate_jobs_per_s"])
+        return math.exp(ra + t * (rb - ra)), False
+    if inside is None:
+        return None, False
+    return points[inside]["rate_jobs_per_s"], True
+
+
+def served_equal_p99(gtap_amd, seed: int, njobs: int, expected, lo: float = 1000.0, hi: float = 80000.0,
+                     step: float = 1.25) -> dict:
+    """Poisson arrivals of the first `njobs` jobs of the stream at rates lo, lo*step, ... hi,
+    served by one resident launch and by one launch per job; latency = arrival to the result
+    seen by the host, for both. The per-job form is swept until it saturates (it completes
+    under 80 % of the offered rate twice in a row). The target metric: at the p99 the per-job
+    form keeps at its highest unsaturated rate, the rate the resident form sustains within the
+    same p99, over the per-job form's rate."""
+    import numpy as np
+    points = []
+    sv = gtap_amd.Stream(seed=seed, njobs=njobs)
+    saturated = 0
+    try:
+        rate = lo
+        while rate <= hi * 1.0001:
+            row = {"rate_jobs_per_s": rate}
+            forms = (("resident", False),) if saturated >= 2 else (("resident", False), ("per_job_launch", True))
+            for form, per_job in forms:
+                r, lat, tot, st = sv.serve(rate=rate, arrival_seed=5, per_job_launch=per_job)
+                if not np.array_equal(r, expected):
+                    raise RuntimeError(f"stream: served ({form}) results differ from the batched run")
+                ex = sv.exec_ms()
+                row[form] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                             "exec_p50_ms": float(np.percentile(ex, 50)), "jobs_per_s": njobs / (tot / 1e3)}
+                if not per_job:
+                    row[form]["admission_holds"] = int(st.get("admission_holds", 0))
+            if "per_job_launch" in row:
+                saturated = saturated + 1 if row["per_job_launch"]["jobs_per_s"] < 0.8 * rate else 0
+            points.append(row)
+            pj = row.get("per_job_launch")
+            log(f"stream served at {rate:.0f} jobs/s: resident p99 {row['resident']['p99_ms']:.3f} ms"
+                + (f", per-job launch p99 {pj['p99_ms']:.3f} ms ({pj['jobs_per_s']:.0f} jobs/s)" if pj else ""))
+            rate *= step
+    finally:
+        sv.close()
+    # the per-job form's best operating point: its highest rate completed at >= 95 % of the offer
+    ok = [p for p in points if "per_job_launch" in p and p["per_job_launch"]["jobs_per_s"] >= 0.95 * p["rate_jobs_per_s"]]
+    out = {"jobs": njobs, "arrival_seed": 5, "rate_step": step, "latency": "arrival to result seen by the host",
+           "points": points}
+    if ok:
+        ref = ok[-1]
+        target = ref["per_job_launch"]["p99_ms"]
+        r_res, bound = _rate_at_p99(points, "resident", target)
+        r_pj, _ = _rate_at_p99(points, "per_job_launch", target)
+        out["equal_p99"] = {
+            "p99_ms": target,
+            "per_job_launch_jobs_per_s": r_pj,
+            "resident_jobs_per_s": r_res,
+            "resident_is_lower_bound": bound,  # still within the p99 at the highest rate swept
+            "resident_over_per_job": (r_res / r_pj) if (r_res and r_pj) else None,
+        }
+        log(f"stream at equal p99 {target:.2f} ms: resident {r_res:.0f} jobs/s vs one launch per job {r_pj:.0f} "
+            f"jobs/s = {r_res / r_pj:.1f}x" + (" (lower bound)" if bound else ""))
+    return out
+
+
+def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024, serve_jobs: int = 1024) -> dict:
     """The 4096-job mixed stream (fib / n-queens / sort jobs, fixed seed): one resident batched
-    launch against one launch per job; then Poisson arrivals swept over `serve_rates` (the first
-    `serve_jobs` jobs of the same stream), served by one resident launch and by one launch per
-    job, with p50/p99 latency from arrival to the result seen by the host. Results of every
-    form must agree job for job."""
+    launch against one launch per job (the forms must agree job for job; the GPU tests check
+    every result, sort jobs by the checksum of their sorted keys, against the oracle); then
+    Poisson arrivals (the first `serve_jobs` jobs) swept in x1.25 steps, served by one resident
+    launch and by one launch per job, with p50/p99 latency from arrival to the result seen by
+    the host, and the rate each sustains at equal p99."""
     import numpy as np
     st = gtap_amd.Stream(seed=seed, njobs=njobs)
     try:
@@ -321,33 +402,12 @@ def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024,
         "jobs": njobs,
         "batched": {"kernel_ms_median": kms, "jobs_per_s": njobs / (kms / 1e3),
                     "p50_ms": float(np.percentile(lat_b, 50)), "p99_ms": float(np.percentile(lat_b, 99))},
+        # every job arrives at the start in both forms: latency = its completion after the start
         "per_job_launch": {"total_ms": tot_p, "jobs_per_s": njobs / (tot_p / 1e3),
-                           "p50_ms": float(np.percentile(np.diff(lat_p, prepend=0.0), 50)),
-                           "p99_ms": float(np.percentile(np.diff(lat_p, prepend=0.0), 99))},
+                           "p50_ms": float(np.percentile(lat_p, 50)), "p99_ms": float(np.percentile(lat_p, 99))},
     }
     out["batched_over_per_job"] = out["batched"]["jobs_per_s"] / out["per_job_launch"]["jobs_per_s"]
-    sweep = []
-    sv = gtap_amd.Stream(seed=seed, njobs=serve_jobs)
-    try:
-        for rate in serve_rates:
-            row = {"rate_jobs_per_s": rate}
-            for form, per_job in (("resident", False), ("per_job_launch", True)):
-                r, lat, tot, _ = sv.serve(rate=rate, arrival_seed=5, per_job_launch=per_job)
-                if not np.array_equal(r, res_b[:serve_jobs]):
-                    raise RuntimeError(f"stream: served ({form}) and batched results differ")
-                ex = sv.exec_ms()
-                row[form] = {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                             "exec_p50_ms": float(np.percentile(ex, 50)), "jobs_per_s": serve_jobs / (tot / 1e3)}
-            sweep.append(row)
-            log(f"stream served at {rate:.0f} jobs/s: resident p50 {row['resident']['p50_ms']:.3f} p99 "
-                f"{row['resident']['p99_ms']:.3f} ms; per-job launch p50 {row['per_job_launch']['p50_ms']:.3f} "
-                f"p99 {row['per_job_launch']['p99_ms']:.3f} ms")
-    finally:
-        sv.close()
-    out["served_sweep"] = {"jobs": serve_jobs, "arrival_seed": 5, "points": sweep,
-                           # p99 latency of the resident form over the per-job form, per rate
-                           "p99_resident_over_per_job": [p["resident"]["p99_ms"] / p["per_job_launch"]["p99_ms"]
-                                                         for p in sweep]}
+    out["served"] = served_equal_p99(gtap_amd, seed, serve_jobs, res_b[:serve_jobs])
     log(f"stream: batched {out['batched']['jobs_per_s']:.0f} jobs/s, per-job {out['per_job_launch']['jobs_per_s']:.0f}"
         f" jobs/s ({out['batched_over_per_job']:.1f}x)")
     return out

@@ -62,11 +62,11 @@ class RunStats(ctypes.Structure):
         ("workers", ctypes.c_uint32),
         ("blocks", ctypes.c_uint32),
         ("error_code", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("admission_holds", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class Config(ctypes.Structure):
@@ -98,7 +98,7 @@ class Job(ctypes.Structure):
         ("t_start", ctypes.c_uint64),
         ("t_end", ctypes.c_uint64),
         ("t_done_host", ctypes.c_uint64),
-        ("reserved", ctypes.c_uint64),
+        ("reserve_ids", ctypes.c_uint64),
         ("args", ctypes.c_uint8 * 64),
     ]
 
@@ -123,6 +123,7 @@ class GtapError(RuntimeError):
 _i, _u32, _u64, _i64, _vp = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_void_p
 SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_initialize": ([], _i),
+    "gtap_register_entry_module": ([_vp], _i),
     "gtap_finalize": ([], _i),
     "gtap_reset": ([], _i),
     "gtap_set_config": ([ctypes.POINTER(Config)], _i),
@@ -297,6 +298,130 @@ def lowered(name: str) -> ctypes.CDLL:
         rb.restype = ctypes.c_uint32
         _LOWERED[name] = L
     return _LOWERED[name]
+
+
+_HIP: Optional[ctypes.CDLL] = None
+
+
+def hip_runtime() -> ctypes.CDLL:
+    """The HIP runtime library libgtap_amd.so is linked against (the instance already loaded
+    in this process)."""
+    global _HIP
+    if _HIP is None:
+        lib()
+        path = "libamdhip64.so"
+        try:
+            with open("/proc/self/maps") as f:
+                for line in f:
+                    if "libamdhip64.so" in line:
+                        path = line.split()[-1]
+                        break
+        except OSError:
+            pass
+        H = ctypes.CDLL(path)
+        H.hipMemcpyToSymbol.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]
+        H.hipMemcpyToSymbol.restype = ctypes.c_int
+        H.hipMemcpyFromSymbol.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t,
+                                          ctypes.c_int]
+        H.hipMemcpyFromSymbol.restype = ctypes.c_int
+        H.hipDeviceSynchronize.argtypes = []
+        H.hipDeviceSynchronize.restype = ctypes.c_int
+        H.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        H.hipEventCreate.restype = ctypes.c_int
+        H.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        H.hipEventRecord.restype = ctypes.c_int
+        H.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        H.hipEventSynchronize.restype = ctypes.c_int
+        H.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        H.hipEventElapsedTime.restype = ctypes.c_int
+        H.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        H.hipEventDestroy.restype = ctypes.c_int
+        _HIP = H
+    return _HIP
+
+
+class EntryModule:
+    """A module written by gtap_amd.lower, driven the way the reference's host programs drive
+    theirs (examples/fib/fib.cu:26-57): device variables set and read with hipMemcpyToSymbol /
+    hipMemcpyFromSymbol, an entry kernel launched with <<<grid, block>>> (its generated
+    `gtap_launch_<kernel>`), then hipDeviceSynchronize. One launch may follow each
+    gtap_initialize / gtap_reset; launch() calls gtap_reset first unless told not to."""
+
+    def __init__(self, path: str, name: str):
+        lib()  # the runtime first: the module links against it and registers with it on load
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing")
+        self.L = ctypes.CDLL(path)
+        self.name = name
+        f = getattr(self.L, f"gtap_lowered_{name}_symbol")
+        f.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64)]
+        f.restype = ctypes.c_void_p
+        self._symbol = f
+        d = getattr(self.L, f"gtap_lowered_{name}_launch_dims")
+        d.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
+        d.restype = ctypes.c_int
+        self._dims = d
+        rb = getattr(self.L, f"gtap_lowered_{name}_record_bytes")
+        rb.argtypes = []
+        rb.restype = ctypes.c_uint32
+        self.record_bytes = rb()
+
+    def symbol(self, sym: str):
+        n = ctypes.c_uint64(0)
+        addr = self._symbol(sym.encode(), ctypes.byref(n))
+        if not addr:
+            raise KeyError(f"{self.name}: no device variable {sym}")
+        return addr, n.value
+
+    def set_symbol(self, sym: str, value, ctype=ctypes.c_int) -> None:
+        addr, n = self.symbol(sym)
+        v = ctype(value)
+        if ctypes.sizeof(v) != n:
+            raise ValueError(f"{sym} is {n} bytes, {ctype.__name__} is {ctypes.sizeof(v)}")
+        _check(hip_runtime().hipMemcpyToSymbol(addr, ctypes.byref(v), n, 0, 1), f"hipMemcpyToSymbol({sym})")
+
+    def get_symbol(self, sym: str, ctype=ctypes.c_int):
+        addr, n = self.symbol(sym)
+        v = ctype()
+        if ctypes.sizeof(v) != n:
+            raise ValueError(f"{sym} is {n} bytes, {ctype.__name__} is {ctypes.sizeof(v)}")
+        _check(hip_runtime().hipMemcpyFromSymbol(ctypes.byref(v), addr, n, 0, 2), f"hipMemcpyFromSymbol({sym})")
+        return v.value
+
+    def launch_dims(self) -> tuple:
+        g, b = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        _check(self._dims(ctypes.byref(g), ctypes.byref(b)), "launch_dims")
+        return g.value, b.value
+
+    def launch(self, kernel: str, argtypes: list, *args, grid: Optional[int] = None, block: Optional[int] = None,
+               reset: bool = True, timed: bool = False):
+        """Launches `kernel` (its parameters' ctypes in `argtypes`) and synchronizes; returns the
+        status of the synchronize, or (status, ms) with hipEvents around the launch."""
+        if reset:
+            _check(lib().gtap_reset(), "gtap_reset")
+        g0, b0 = self.launch_dims()
+        f = getattr(self.L, f"gtap_launch_{kernel}")
+        f.argtypes = [ctypes.c_uint32, ctypes.c_uint32] + list(argtypes)
+        f.restype = ctypes.c_int
+        H = hip_runtime()
+        ev = [ctypes.c_void_p(), ctypes.c_void_p()]
+        if timed:
+            for e in ev:
+                _check(H.hipEventCreate(ctypes.byref(e)), "hipEventCreate")
+            H.hipEventRecord(ev[0], None)
+        st = f(grid or g0, block or b0, *args)
+        if timed:
+            H.hipEventRecord(ev[1], None)
+        sync = H.hipDeviceSynchronize()
+        st = st or sync
+        if not timed:
+            return st
+        ms = ctypes.c_float(0)
+        H.hipEventSynchronize(ev[1])
+        H.hipEventElapsedTime(ctypes.byref(ms), ev[0], ev[1])
+        for e in ev:
+            H.hipEventDestroy(e)
+        return st, ms.value
 
 
 def gtap_set_profiling(enable: bool, max_samples: int = 4096) -> int:
@@ -609,6 +734,7 @@ class Stream:
         h = ctypes.c_void_p(0)
         _check(lib().gtap_stream_create(seed, njobs, ctypes.byref(h)), "gtap_stream_create")
         self.h = h.value
+        self.seed = seed
         self.njobs = njobs
         self.kind, self.param, self.key_seed = jobs_generate(seed, njobs)
 

@@ -23,6 +23,7 @@
 // tasks; bounds in gtap_dev.hpp).
 #pragma once
 
+#include "gtap_block.hpp"
 #include "gtap_thread.hpp"
 
 template <class L>
@@ -65,4 +66,29 @@ __device__ __forceinline__ void __gtap_return(L& lane, T value) {
 template <class L>
 __device__ __forceinline__ uint8_t* __gtap_get_task_data(L& lane) {
     return lane.rec + ::gtap::kHdrBytes;
+}
+
+// Block mode (gtap_runtime_block.cuh:441-576): the same calls on a workgroup task. A spawn may
+// come from any subset of the threads; the queue is ignored (one deque per workgroup, as in
+// the reference's block mode); taskwait, return and finish are workgroup-collective.
+template <class P, class A>
+__device__ __forceinline__ uint32_t __gtap_spawn_task(::gtap::BlockTask<P>& blk, uint32_t func, uint32_t /*queue*/,
+                                                      int res_off, const A& args) {
+    return blk.spawn(func, res_off, args);
+}
+template <class P, class A>
+__device__ __forceinline__ uint32_t __gtap_spawn_task_detached(::gtap::BlockTask<P>& blk, uint32_t func,
+                                                               uint32_t /*queue*/, const A& args) {
+    return blk.spawn_detached(func, args);
+}
+template <class P>
+__device__ __forceinline__ bool __gtap_set_state_for_join(::gtap::BlockTask<P>& blk, uint32_t next_state,
+                                                          uint32_t /*queue_after*/) {
+    return blk.taskwait(next_state);
+}
+// (reference: __gtap_set_state_for_join_block(tid, ctx, next_state, unused), the child count
+// taken from the workgroup's context)
+template <class P>
+__device__ __forceinline__ bool __gtap_set_state_for_join_block(::gtap::BlockTask<P>& blk, uint32_t next_state) {
+    return blk.taskwait(next_state);
 }

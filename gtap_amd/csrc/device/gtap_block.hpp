@@ -148,7 +148,7 @@ __device__ __forceinline__ void block_spill_chunk(const GtapDev& dev, BlockCtx* 
     const uint32_t t = blk_thread();
     const uint32_t top = c->free_top;
     const uint32_t ptop = c->priv_top;
-    if (ptop + kChunk <= kPrivCap) {
+    if (ptop + kChunk <= dev.priv_cap) {
         uint32_t* dst = dev.priv_ids + (size_t)me * kPrivCap + ptop;
         if (t < kChunk) dst[t] = c->free_ids[top - kChunk + t];
         __syncthreads();
@@ -473,7 +473,7 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
     const bool host_poller = dev.term_mode == GTAP_TERM_SERVE && me == 0 && t == 0;
 
     while (true) {
-        if (host_poller) host_poll(dev.ctl, dev.host_ctl, lp.host_poll_last);
+        if (host_poller) host_poll(dev, lp.host_poll_last);
         // keep enough ids in LDS for one invocation's spawns (collective)
         while (c->free_top < kBlockMaxSpawn)
             block_refill_chunk(dev, c, me, (kBlockMaxSpawn - c->free_top + kChunk - 1) / kChunk);

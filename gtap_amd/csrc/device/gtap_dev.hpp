@@ -41,8 +41,14 @@ enum CtlIndex : int {
                           // has not made resident yet and that owns no job holds no work and
                           // must not be waited for; one that owns a job must)
     CTL_PROGRAM = 10,     // a counter a program may use (cleared with the others per launch)
-    CTL_COUNT = 12
+    CTL_ARMED = 11,       // 1 after gtap_initialize / gtap_reset: one host-launched entry kernel
+                          // may run on this state (it clears the word when it is done)
+    CTL_ADMIT_HOLDS = 12, // served: admission attempts the reservation ledger held back
+    CTL_COUNT = 13
 };
+// Served admission ledger: the 8-byte word at CTL_JOB_NEXT holds the admission counter in its
+// low half and, in its high half, the ids (in 64-id units) reserved by the admitted jobs that
+// have not completed. One CAS admits jobs and reserves their ids together.
 
 // Host-visible control (pinned host memory, read with system-scope loads).
 enum HostCtlIndex : int {
@@ -91,7 +97,9 @@ struct GtapDev {
     uint32_t term_mode;        // gtap_term_mode
     uint32_t njobs;            // jobs of this launch (JOBS / QUIESCE modes)
     uint32_t job_ring;         // job-table entries (anchors are job index % job_ring)
-    uint32_t admit_reserve;    // free ids required per admitted job
+    uint32_t admit_budget;     // served: 64-id units the reservation ledger may hand out (0: no ledger)
+    uint32_t priv_cap;         // ids a wave may park in its private HBM stack (<= kPrivCap)
+    uint32_t* job_units;       // served: reserved 64-id units per job-table slot (device copy)
     uint32_t max_live_jobs;    // admission cap on concurrently running jobs
     uint32_t static_jobs;      // jobs [0, static_jobs) start on workers [0, static_jobs), by index
     uint32_t static_ids;       // ids [64 + w * static_ids, 64 + (w + 1) * static_ids) are worker w's
@@ -380,17 +388,32 @@ __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
 // A shared "last poll" word claimed by CAS would be read (and CAS'd) by every idle wave on
 // every iteration: thousands of waves on one line slow every access mapped to its memory
 // channel, including the running jobs' own.
+// With a reservation ledger (dev.job_units), the poller also copies each newly available job's
+// reservation into device memory before it publishes the count, at most kPollJobs per poll,
+// so admission never reads the host's job table.
 constexpr uint64_t kHostPollTicks = 100;  // 1 us of the 100 MHz wall clock
-__device__ __forceinline__ void host_poll(int32_t* ctl, const int32_t* host_ctl, uint64_t& last) {
+constexpr int32_t kPollJobs = 64;
+__device__ __forceinline__ void host_poll(const GtapDev& dev, uint64_t& last) {
     const uint64_t now = now_ticks();
     if (now - last < kHostPollTicks) return;
     last = now;
-    const int32_t stop = ld32_sys(host_ctl + HCTL_STOP * 32);
+    int32_t* ctl = dev.ctl;
+    const int32_t stop = ld32_sys(dev.host_ctl + HCTL_STOP * 32);
     complete_stores();  // the stop flag is read before the job count (the host writes them in the other order)
-    const int32_t avail = ld32_sys(host_ctl + HCTL_JOB_AVAIL * 32);
+    int32_t avail = ld32_sys(dev.host_ctl + HCTL_JOB_AVAIL * 32);
+    if (dev.job_units != nullptr) {
+        const int32_t seen = ld32(ctl + CTL_HOST_AVAIL * 32);
+        if (avail - seen > kPollJobs) avail = seen + kPollJobs;
+        for (int32_t j = seen; j < avail; ++j) {
+            const uint32_t slot = (uint32_t)j % dev.job_ring;
+            const uint64_t ids = ld64_sys(&dev.jobs[slot].reserve_ids);
+            st32(dev.job_units + slot, (uint32_t)((ids + 63u) / 64u));
+        }
+        complete_stores();  // the reservations are in place before the jobs become visible
+    }
     // one writer and a non-decreasing host count: a plain store keeps the mirror monotonic
     st32(ctl + CTL_HOST_AVAIL * 32, avail);
-    if (stop) {
+    if (stop && avail == ld32_sys(dev.host_ctl + HCTL_JOB_AVAIL * 32)) {
         complete_stores();  // a wave that sees the stop flag sees the final job count
         st32(ctl + CTL_HOST_STOP * 32, 1);
     }

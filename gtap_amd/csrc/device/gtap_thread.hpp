@@ -295,7 +295,7 @@ __device__ __forceinline__ void spill_chunk(const GtapDev& dev, W* w, uint32_t m
     const int leader = first_lane(m);
     const uint32_t top = uniform(vld(&w->free_top));
     const uint32_t ptop = uniform(vld(&w->priv_top));
-    if (ptop + kChunk <= kPrivCap) {
+    if (ptop + kChunk <= dev.priv_cap) {
         uint32_t* dst = dev.priv_ids + (size_t)me * kPrivCap + ptop;
         for (uint32_t i = r; i < kChunk; i += cnt) dst[i] = vld(&w->free_ids[top - kChunk + i]);
         vst(&w->priv_top, ptop + kChunk);
@@ -955,27 +955,37 @@ __device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtxOf<P>*
     // and the shared counter's line is not read again
     if (seen_next >= avail) return 0;
     int32_t* nextp = dev.ctl + CTL_JOB_NEXT * 32;
-    uint32_t next = uniform((uint32_t)ld32(nextp));
+    uint64_t* wordp = reinterpret_cast<uint64_t*>(nextp);  // {admission counter, reserved units}
+    const uint64_t word = uniform(ld64(wordp));
+    const uint32_t next = (uint32_t)word;
     seen_next = next;
     if (next >= avail) return 0;
     uint32_t k = avail - next;
     if (k > kWave) k = kWave;
-    // admission: cap on live jobs and free task ids
+    // admission: cap on live jobs, and (served) the reservation ledger
     const uint32_t live = uniform((uint32_t)ld32(dev.ctl + CTL_LIVE_JOBS * 32));
     if (live >= dev.max_live_jobs) return 0;
     if (k > dev.max_live_jobs - live) k = dev.max_live_jobs - live;
-    if (dev.admit_reserve > 0) {
-        // ids never handed out plus ids parked in the shared rings
-        const uint32_t fresh = uniform(ld32(pool_fresh_ptr(dev)));
-        uint64_t free_ids = (uint64_t)(dev.nrecs > fresh ? dev.nrecs - fresh : 0);
-        for (uint32_t sh = 0; sh < dev.nshards; ++sh) {
-            const uint32_t enq = uniform(ld32(pool_enq_ptr(dev, sh)));
-            const uint32_t deq = uniform(ld32(pool_deq_ptr(dev, sh)));
-            free_ids += (uint64_t)(enq - deq) * kChunk;
+    uint32_t units = 0;
+    if (dev.admit_budget > 0) {
+        // the longest run of the next jobs whose reservations fit beside those of the running jobs
+        const uint32_t ledger = (uint32_t)(word >> 32);
+        const uint32_t room = dev.admit_budget > ledger ? dev.admit_budget - ledger : 0u;
+        uint32_t u = (uint32_t)lane < k ? ld32(dev.job_units + (next + (uint32_t)lane) % dev.job_ring) : 0u;
+        uint32_t pre = u;  // inclusive prefix sum over the lanes
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t t = (uint32_t)__shfl_up((int)pre, (unsigned)d, kWave);
+            if (lane >= d) pre += t;
         }
-        uint64_t fit = free_ids / dev.admit_reserve;
-        if (fit == 0) return 0;
-        if (k > fit) k = (uint32_t)fit;
+        const uint32_t m = (uint32_t)popc(ballot((uint32_t)lane < k && pre <= room));
+        if (m == 0) {
+            // held back: the job at the head waits for running jobs to release their ids
+            if (lane == 0 && me == next % dev.nworkers) atomic_add_noret(dev.ctl + CTL_ADMIT_HOLDS * 32, 1);
+            return 0;
+        }
+        k = m;
+        units = shfl(pre, (int)m - 1);
     }
     if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
         if (lane == 0) atomic_add(dev.ctl + CTL_IDLE * 32, -1);
@@ -988,8 +998,8 @@ __device__ __forceinline__ uint32_t admit_jobs(const GtapDev& dev, WaveCtxOf<P>*
         // and must not see the jobs admitted but not yet live
         atomic_add(dev.ctl + CTL_LIVE_JOBS * 32, (int32_t)k);
         complete_stores();
-        int32_t expected = (int32_t)next;
-        ok = atomic_cas(nextp, expected, (int32_t)(next + k)) ? 1 : 0;
+        uint64_t expected = word;
+        ok = atomic_cas(wordp, expected, word + (uint64_t)k + ((uint64_t)units << 32)) ? 1 : 0;
         if (!ok) atomic_add_noret(dev.ctl + CTL_LIVE_JOBS * 32, -(int32_t)k);
     }
     ok = shfl((int32_t)ok, 0);
@@ -1015,6 +1025,11 @@ template <class P>
 __device__ __forceinline__ void complete_job(const GtapDev& dev, uint32_t anchor) {
     const uint32_t slot = anchor - dev.nrecs;
     gtap_job* job = dev.jobs + slot;
+    if (dev.admit_budget > 0) {
+        // release the job's reservation (read before the slot is handed back to the host)
+        const uint64_t units = ld32(dev.job_units + slot);
+        atomic_add_noret(reinterpret_cast<uint64_t*>(dev.ctl + CTL_JOB_NEXT * 32), (uint64_t)0 - (units << 32));
+    }
     const uint64_t res = ld64(reinterpret_cast<const uint64_t*>(rec_ptr(dev, anchor) + kHdrBytes));
     st64_sys(reinterpret_cast<uint64_t*>(&job->result), res);
     st64_sys(&job->t_end, now_ticks());
@@ -1316,7 +1331,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     }
 
     while (true) {
-        if (host_poller && lane == 0) host_poll(dev.ctl, dev.host_ctl, host_poll_last);
+        if (host_poller && lane == 0) host_poll(dev, host_poll_last);
         if (n == 0) {
             // ---- acquire work: own deque, then the job table, then steal (records in HBM)
             my_lds = false;

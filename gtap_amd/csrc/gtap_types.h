@@ -57,7 +57,7 @@ typedef struct gtap_job {
     uint64_t t_start;     /* device: admission time (wall clock ticks) */
     uint64_t t_end;       /* device: completion time (wall clock ticks) */
     uint64_t t_done_host; /* host: time the completion was seen (host clock, ns) */
-    uint64_t reserved;
+    uint64_t reserve_ids; /* served: task ids the job reserves while it runs (0 = none) */
     uint8_t args[64];     /* root frame arguments */
 } gtap_job;
 
@@ -89,7 +89,8 @@ typedef struct gtap_run_stats {
     uint32_t workers;         /* wavefront workers in the launch */
     uint32_t blocks;          /* workgroups in the launch */
     int32_t error_code;
-    int32_t reserved;
+    int32_t admission_holds;  /* served: admission attempts the id reservation held back
+                                 (counted by one wave per pending job) */
 } gtap_run_stats;
 
 /* Schedulers of gtap_config.scheduler. */
@@ -120,6 +121,11 @@ typedef struct gtap_config {
                                  steals (programs built with the alternatives: fib, n-queens) */
     uint32_t reserved[4];
 } gtap_config;
+
+/* Called by gtap_initialize and gtap_reset (and at registration when the runtime is already
+ * initialized) with the runtime descriptor (`bytes` = its size): a lowered module copies it to
+ * its device symbol, so its host-launched entry kernels reach the runtime's pools. */
+typedef int (*gtap_entry_setup_fn)(const void* desc, uint64_t bytes);
 
 #ifdef __cplusplus
 }

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "../gtap_types.h"
 #include "gtap_launch.hpp"
@@ -129,7 +130,9 @@ GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job
     d.term_mode = term_mode;
     d.njobs = njobs;
     d.job_ring = job_ring;
-    d.admit_reserve = 0;
+    d.admit_budget = 0;
+    d.priv_cap = kPrivCap;
+    d.job_units = nullptr;
     d.max_live_jobs = R.nanchors;
     // A launch's first jobs go to its first workers by index, and every worker starts with a
     // fixed range of ids: starting touches neither the admission counter nor the id pool's
@@ -150,7 +153,7 @@ GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job
 }
 
 // Clears the per-launch state: control words, deque words, id-pool ring, statistics.
-__global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0) {
+__global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0, int32_t arm) {
     const uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const uint64_t gs = (uint64_t)gridDim.x * blockDim.x;
     if (write_job0 && gid < sizeof(gtap_job) / 4)
@@ -159,10 +162,9 @@ __global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0) {
     // compares the idle count with the started count): a worker that owns a job but is not
     // resident yet must keep every other worker from taking the launch as finished.
     for (uint64_t i = gid; i < (uint64_t)CTL_COUNT * 32; i += gs)
-        dev.ctl[i] = (i == CTL_JOB_NEXT * 32 || i == CTL_LIVE_JOBS * 32 ||
-                      (i == CTL_STARTED * 32 && dev.term_mode == GTAP_TERM_QUIESCE))
+        dev.ctl[i] = (i == CTL_JOB_NEXT * 32 || i == CTL_LIVE_JOBS * 32 || i == CTL_STARTED * 32)
                          ? (int32_t)dev.static_jobs
-                         : 0;
+                         : (i == CTL_ARMED * 32 ? arm : 0);
     for (uint64_t i = gid; i < (uint64_t)dev.nqueues * dev.nworkers; i += gs) dev.dq_word[i * 16] = 0;
     for (uint64_t i = gid; i < (uint64_t)dev.nshards * dev.ncells; i += gs)
         dev.pool_seq[i] = (uint32_t)(i & (dev.ncells - 1));
@@ -182,17 +184,17 @@ __global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0) {
         dev.pool_ctl[i] = (i == (uint64_t)dev.nshards * 32) ? 64ull + (uint64_t)dev.nworkers * dev.static_ids : 0ull;
 }
 
-int enqueue_reset(const GtapDev& dev, hipStream_t stream, const gtap_job* job0) {
+int enqueue_reset(const GtapDev& dev, hipStream_t stream, const gtap_job* job0, bool arm) {
     gtap_job j;
     if (job0) j = *job0;
     else memset(&j, 0, sizeof(j));
-    hipLaunchKernelGGL(reset_kernel, dim3(1024), dim3(256), 0, stream, dev, j, job0 ? 1 : 0);
+    hipLaunchKernelGGL(reset_kernel, dim3(1024), dim3(256), 0, stream, dev, j, job0 ? 1 : 0, arm ? 1 : 0);
     GTAP_HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int reset_launch_state(const GtapDev& dev, hipStream_t stream, const gtap_job* job0) {
-    if (int st = enqueue_reset(dev, stream, job0)) return st;
+int reset_launch_state(const GtapDev& dev, hipStream_t stream, const gtap_job* job0, bool arm) {
+    if (int st = enqueue_reset(dev, stream, job0, arm)) return st;
     Runtime& R = rt();
     if (R.report) memset(R.report, 0, sizeof(gtap_error_report));
     R.stats_host_valid = false;
@@ -225,6 +227,7 @@ __global__ void __launch_bounds__(kSummaryThreads) summary_kernel(GtapDev dev, L
     if (t < ST_COUNT) out->sums[t] = part[t][0];
     if (t == 0) {
         out->error_code = dev.ctl[CTL_ERROR * 32];
+        out->admission_holds = dev.ctl[CTL_ADMIT_HOLDS * 32];
         out->has_job0 = copy_job0;
     }
     if (copy_job0 && t < sizeof(gtap_job) / 4)
@@ -296,6 +299,7 @@ int finish_launch(const GtapDev& dev, uint32_t blocks, gtap_run_stats* stats) {
     s.blocks = blocks;
     R.last_nworkers = dev.nworkers;
     s.error_code = err;
+    s.admission_holds = sm->admission_holds;
     R.last = s;
     R.last_error_code = err;
     if (stats) *stats = s;
@@ -332,6 +336,7 @@ static void free_all() {
     dfree(R.prof_counts);
     dfree(R.gq_slots);
     dfree(R.gq_ctl);
+    dfree(R.job_units);
     if (R.host_ctl) (void)hipHostFree(R.host_ctl);
     if (R.report) (void)hipHostFree(R.report);
     if (R.summary) (void)hipHostFree(R.summary);
@@ -342,6 +347,27 @@ static void free_all() {
     gtap_config keep = R.cfg;
     R = Runtime();
     R.cfg = keep;
+}
+
+// Lowered modules whose entry kernels the host launches (gtap_register_entry_module).
+static std::vector<gtap_entry_setup_fn>& entry_modules() {
+    static std::vector<gtap_entry_setup_fn> mods;
+    return mods;
+}
+
+GtapDev entry_dev() {
+    Runtime& R = rt();
+    return make_dev(R.max_workers, GTAP_TERM_JOBS, 1, R.jobs, 1);
+}
+
+int arm_entry_modules() {
+    Runtime& R = rt();
+    const GtapDev d = entry_dev();
+    if (int st = reset_launch_state(d, R.stream, nullptr, true)) return st;
+    GTAP_HIP_TRY(hipStreamSynchronize(R.stream));
+    for (gtap_entry_setup_fn f : entry_modules())
+        if (int st = f(&d, sizeof(d))) return hip_status((hipError_t)st, "lowered module setup");
+    return 0;
 }
 
 // Largest frame over the programs compiled into this library, header included
@@ -437,6 +463,7 @@ int gtap_initialize(void) {
     GTAP_HIP_TRY(hipMemset(R.ctl, 0, (size_t)CTL_COUNT * 32 * sizeof(int32_t)));
     GTAP_HIP_TRY(hipMalloc(&R.jobs, (size_t)R.nanchors * sizeof(gtap_job)));
     GTAP_HIP_TRY(hipMemset(R.jobs, 0, (size_t)R.nanchors * sizeof(gtap_job)));
+    GTAP_HIP_TRY(hipMalloc(&R.job_units, (size_t)R.nanchors * sizeof(uint32_t)));
     GTAP_HIP_TRY(hipMalloc(&R.stats, (size_t)R.max_workers * ST_COUNT * sizeof(uint64_t)));
 
     GTAP_HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&R.host_ctl), HCTL_COUNT * 32 * sizeof(int32_t),
@@ -453,7 +480,8 @@ int gtap_initialize(void) {
     GTAP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&R.summary_dev), R.summary, 0));
     GTAP_HIP_TRY(hipDeviceSynchronize());
     R.initialized = true;
-    return 0;
+    // ready for one host-launched entry kernel, as the reference's state is after initialization
+    return arm_entry_modules();
 }
 
 int gtap_finalize(void) {
@@ -464,16 +492,24 @@ int gtap_finalize(void) {
     return 0;
 }
 
-// Clears the runtime state without reallocating (every launch also does this itself).
+// Clears the runtime state without reallocating (every launch also does this itself) and
+// arms it for one host-launched entry kernel of a lowered module.
 int gtap_reset(void) {
     Runtime& R = rt();
     if (!R.initialized) return fail(GTAP_ERROR_NOT_INITIALIZED, "gtap_reset before gtap_initialize");
-    GtapDev d = make_dev(R.max_workers, GTAP_TERM_JOBS, 0, R.jobs, 1);
-    int st = reset_launch_state(d, R.stream);
-    if (st) return st;
-    GTAP_HIP_TRY(hipStreamSynchronize(R.stream));
+    if (int st = arm_entry_modules()) return st;
     R.last_error_code = 0;
     return 0;
+}
+
+int gtap_register_entry_module(gtap_entry_setup_fn setup) {
+    if (setup == nullptr) return invalid_argument("gtap_register_entry_module: null setup function");
+    std::vector<gtap_entry_setup_fn>& mods = entry_modules();
+    if (std::find(mods.begin(), mods.end(), setup) == mods.end()) mods.push_back(setup);
+    Runtime& R = rt();
+    if (!R.initialized) return 0;  // gtap_initialize pushes the descriptor
+    const GtapDev d = entry_dev();
+    return setup(&d, sizeof(d));
 }
 
 int gtap_get_runtime_error_code(int* code) {

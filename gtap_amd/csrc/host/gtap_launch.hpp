@@ -18,6 +18,7 @@
 #include "../gtap_types.h"
 
 extern "C" int gtap_initialize(void);
+extern "C" int gtap_register_entry_module(gtap_entry_setup_fn setup);
 
 namespace gtap {
 namespace host {
@@ -33,6 +34,8 @@ struct LaunchSummary {
     uint64_t sums[ST_COUNT];
     int32_t error_code;
     int32_t has_job0;
+    int32_t admission_holds;
+    int32_t pad;
     gtap_job job0;
 };
 
@@ -71,6 +74,7 @@ struct Runtime {
     gtap_job* jobs = nullptr;          // device job table
     gtap_job* jobs_host = nullptr;     // pinned job table (served mode)
     gtap_job* jobs_host_dev = nullptr;
+    uint32_t* job_units = nullptr;     // served: device copy of each job slot's reservation
     uint64_t* stats = nullptr;
     std::vector<uint64_t> stats_host;  // per-worker statistics of the last launch, fetched on demand
     bool stats_host_valid = false;
@@ -104,9 +108,16 @@ gtap_job make_job(uint32_t func, uint32_t queue, const void* args, size_t nbytes
 GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job* jobs, uint32_t job_ring);
 // Resets the per-launch device state; a non-null job0 is written into job-table slot 0 by
 // the reset kernel (a single-job launch needs no host-to-device copy).
-int reset_launch_state(const GtapDev& dev, hipStream_t stream, const gtap_job* job0 = nullptr);
+// `arm` = 1 lets one host-launched entry kernel run on the state (CTL_ARMED).
+int reset_launch_state(const GtapDev& dev, hipStream_t stream, const gtap_job* job0 = nullptr, bool arm = false);
 // Enqueues the per-launch state reset kernel only (usable under stream capture).
-int enqueue_reset(const GtapDev& dev, hipStream_t stream, const gtap_job* job0 = nullptr);
+int enqueue_reset(const GtapDev& dev, hipStream_t stream, const gtap_job* job0 = nullptr, bool arm = false);
+// The descriptor of a host-launched entry kernel: every provisioned worker, one job (slot 0,
+// started by worker 0), first-task termination (a module sets its own mode).
+GtapDev entry_dev();
+// Resets and arms the state for one host-launched entry kernel and pushes the descriptor to
+// every registered lowered module (gtap_initialize, gtap_reset).
+int arm_entry_modules();
 // Enqueues the summary kernel (after the launch's end event): it fills R.summary.
 int enqueue_summary(const GtapDev& dev, hipStream_t stream, bool copy_job0);
 // After the stream has synchronised: kernel time, summed statistics and the error check.

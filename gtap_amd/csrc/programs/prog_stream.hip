@@ -7,12 +7,20 @@
 //                                       children summed by the additive join)
 //   nq(row, cols, diag, anti, N) lane  (evaluation/2-comparison/nq/gtap_nq.cu:25-55 shape,
 //                                       taskwait + int64 sum by the additive join, N per job)
-//   sort_root(arr, tmp, n)       lane  spawns msort on queue 1, taskwait, returns n
+//   sort_root(arr, tmp, n)       lane  spawns msort on queue 1, taskwait; spawns csum over
+//                                      chunks of the sorted keys, taskwait; returns their sum
 //   msort(a, t, n, into_t)       wave  2-way mergesort of examples/mergesort/mergesort.cu:34-50:
 //                                      n <= 1024 -> the wave sorts in LDS; else two msort
 //                                      children (ping-pong buffers), taskwait, then merge
 //   merge(a, na, b, nb, dst)     wave  cilkmerge split of gtap_cilksort.cu:89-127 down to
 //                                      4096 keys, merged by the wave with merge-path tiles
+//   csum(keys, n, pos0, acc)     wave  adds sum_i mix(pos0 + i, keys[i]) to the sort root's
+//                                      64-bit accumulator with one atomic
+//
+// A sort job's result is the checksum of its sorted keys, sum over positions i of
+// mix((i << 32) ^ key[i]) mod 2^64 with mix the SplitMix64 finalizer (Steele, Lea, Flood,
+// "Fast splittable pseudorandom number generators", OOPSLA 2014): every key is checked at
+// its position, and the sum is computed in parallel.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -26,7 +34,7 @@
 namespace gtap {
 namespace stream {
 
-enum : uint32_t { F_FIB = 1, F_NQ = 2, F_SROOT = 3, F_MSORT = 4, F_MERGE = 5 };
+enum : uint32_t { F_FIB = 1, F_NQ = 2, F_SROOT = 3, F_MSORT = 4, F_MERGE = 5, F_CSUM = 6 };
 enum : uint32_t { Q_LANE = 0, Q_WAVE = 1 };
 constexpr int kWaveLeaf = 1024;       // keys sorted by one wave in LDS
 constexpr int kWaveMergeLeaf = 4096;  // keys merged by one wave
@@ -62,7 +70,23 @@ struct MergeArg {
     int64_t na;
     int64_t nb;
 };
+struct CsumArg {
+    const uint32_t* keys;
+    int64_t n;
+    int64_t pos0;
+    uint64_t* acc;
+};
 constexpr int kDummySlot = 40;  // result slot of void children (never read)
+constexpr int kAccOff = 24;     // the sort root's checksum accumulator (after its arguments)
+constexpr int64_t kCsumChunk = 16384;  // keys per checksum task
+
+// The checksum term of key `k` at position `pos`.
+__host__ __device__ __forceinline__ uint64_t sort_mix(uint64_t pos, uint32_t k) {
+    uint64_t z = (pos << 32) ^ (uint64_t)k;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
 
 struct StreamArgs {
     int32_t unused;
@@ -72,7 +96,7 @@ struct Program {
     static constexpr int kQueues = 2;
     static constexpr int kWaveQueue = Q_WAVE;
     static constexpr int kFrameWords = 10;
-    static constexpr uint32_t kMaxFunc = F_MERGE;
+    static constexpr uint32_t kMaxFunc = F_CSUM;
     static constexpr uint32_t kFrameBytes = 48;  // the largest arguments (merge: 40 bytes) and the dummy slot
     using Args = StreamArgs;
     struct WaveShared {
@@ -153,11 +177,28 @@ __device__ __forceinline__ void sort_root_task(L& lane) {
     uint32_t* arr = lane.template arg<uint32_t*>(0);
     uint32_t* tmp = lane.template arg<uint32_t*>(8);
     const int64_t n = lane.template arg<int64_t>(16);
-    if (lane.get_state() == 0 && n > 1) {
-        lane.spawn(F_MSORT, Q_WAVE, kDummySlot, MsortArg{arr, tmp, n, 0, 0});
-        if (lane.taskwait(1, Q_LANE)) return;
+    switch (lane.get_state()) {
+        case 0:
+            if (n <= 1) {  // nothing to sort; the checksum of at most one key
+                lane.ret(n == 1 ? (int64_t)sort_mix(0, ld32(arr)) : (int64_t)0);
+                return;
+            }
+            lane.spawn(F_MSORT, Q_WAVE, kDummySlot, MsortArg{arr, tmp, n, 0, 0});
+            if (lane.taskwait(1, Q_LANE)) return;
+            [[fallthrough]];
+        case 1: {
+            // the sorted keys' checksum: wave tasks over chunks add into this record's accumulator
+            lane.template frame_st<uint64_t>(kAccOff, 0ull);
+            uint64_t* acc = reinterpret_cast<uint64_t*>(lane.rec + kHdrBytes + kAccOff);
+            for (int64_t p = 0; p < n; p += kCsumChunk)
+                lane.spawn(F_CSUM, Q_WAVE, kDummySlot, CsumArg{arr + p, n - p < kCsumChunk ? n - p : kCsumChunk, p, acc});
+            if (lane.taskwait(2, Q_LANE)) return;
+        }
+            [[fallthrough]];
+        default:
+            lane.ret((int64_t)lane.template frame_ld<uint64_t>(kAccOff));
+            return;
     }
-    lane.ret(n);
 }
 
 template <class L>
@@ -367,12 +408,28 @@ __device__ void merge_task(W& wt) {
     wt.finish();
 }
 
+// One chunk of a sort root's checksum: lane-strided terms, a wave sum, one atomic add.
+template <class W>
+__device__ void csum_task(W& wt) {
+    const uint32_t* keys = wt.template arg<const uint32_t*>(0);
+    const int64_t n = wt.template arg<int64_t>(8);
+    const int64_t pos0 = wt.template arg<int64_t>(16);
+    uint64_t* acc = wt.template arg<uint64_t*>(24);
+    uint64_t part = 0;
+    for (int64_t i = wt.lane; i < n; i += kWave) part += sort_mix((uint64_t)(pos0 + i), ldk(keys + i));
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) part += shfl(part, wt.lane ^ d);
+    if (wt.lane == 0) atomic_add_noret(acc, part);
+    wt.finish();
+}
+
 template <class W>
 __device__ __forceinline__ void Program::run_wave(W& wt) {
     if (wt.func == F_MSORT) msort_task(wt);
     else if (wt.func == F_MERGE) merge_task(wt);
+    else if (wt.func == F_CSUM) csum_task(wt);
     else {
-        if (wt.lane == 0) GTAP_DEVICE_ERROR(*wt.dev, GTAP_ERROR_INVALID_FUNCTION, wt.tid, wt.queue, wt.func, F_MERGE);
+        if (wt.lane == 0) GTAP_DEVICE_ERROR(*wt.dev, GTAP_ERROR_INVALID_FUNCTION, wt.tid, wt.queue, wt.func, F_CSUM);
         wt.finish();
     }
 }
@@ -394,6 +451,45 @@ uint64_t splitmix64_next(uint64_t& s) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
+}
+
+// Task ids a job may hold at once, from its kind and size: at most the job's whole task
+// count (a record is live from its spawn to its completion). fib(n) runs 2 F(n+1) - 1 tasks;
+// n-queens(N) one per partial placement (counted here once per N); a sort job's msort tree
+// (leaves of 1024 keys), its merge tasks (split down to 4096 keys, at most about four per
+// 4096 outputs per level), its checksum tasks and its root.
+uint64_t fib_tasks_host(int n) {
+    uint64_t a = 0, b = 1;  // F(0), F(1)
+    for (int i = 0; i < n + 1; ++i) {
+        const uint64_t c = a + b;
+        a = b;
+        b = c;
+    }
+    return 2 * a - 1;  // a = F(n + 1)
+}
+uint64_t nq_nodes_rec(int n, int row, uint32_t cols, uint32_t diag, uint32_t anti) {
+    if (row == n) return 1;
+    uint64_t c = 1;
+    uint32_t avail = ((1u << n) - 1u) & ~(cols | diag | anti);
+    while (avail) {
+        const uint32_t p = avail & (0u - avail);
+        avail -= p;
+        c += nq_nodes_rec(n, row + 1, cols | p, (diag | p) << 1, (anti | p) >> 1);
+    }
+    return c;
+}
+uint64_t nq_tasks_host(int n) {
+    static uint64_t memo[17] = {0};
+    if (n < 1 || n > 16) return 1;
+    if (!memo[n]) memo[n] = nq_nodes_rec(n, 0, 0u, 0u, 0u);
+    return memo[n];
+}
+uint64_t sort_tasks_host(int64_t n) {
+    const uint64_t leaves = (uint64_t)((n + stream::kWaveLeaf - 1) / stream::kWaveLeaf);
+    uint64_t levels = 1;
+    while ((1ull << levels) < leaves + 1) ++levels;
+    const uint64_t merge_per_level = 4 * (uint64_t)((n + stream::kWaveMergeLeaf - 1) / stream::kWaveMergeLeaf);
+    return 1 + 2 * leaves + levels * merge_per_level + (uint64_t)((n + stream::kCsumChunk - 1) / stream::kCsumChunk);
 }
 
 uint64_t host_now_ns() {
@@ -444,6 +540,7 @@ struct gtap_stream {
     uint32_t* d_work = nullptr;     // sorted in place
     uint32_t* d_tmp = nullptr;
     std::vector<gtap_job> table;    // job table template
+    std::vector<uint64_t> tasks;    // task ids each job may hold at once (its reservation before capping)
     gtap_job* d_table = nullptr;    // the same, device-resident (source of the batched graph's copy)
     host::ThreadGraph graph;        // batched launch, captured on first use
     std::vector<double> exec_ms;    // last run: device time from admission to completion per job
@@ -491,7 +588,10 @@ extern "C" int gtap_stream_create(uint64_t seed, int32_t njobs, gtap_stream** ou
         return host::hip_status(e, "gtap_stream_create upload");
     }
     S->table.resize(njobs);
+    S->tasks.resize(njobs);
     for (int32_t i = 0; i < njobs; ++i) {
+        S->tasks[i] = S->kind[i] == 0 ? fib_tasks_host(S->param[i])
+                                       : (S->kind[i] == 1 ? nq_tasks_host(S->param[i]) : sort_tasks_host(S->param[i]));
         gtap_job j;
         memset(&j, 0, sizeof(j));
         j.status = GTAP_JOB_READY;
@@ -558,7 +658,7 @@ static int stream_reset_keys(gtap_stream* S) {
 
 // All jobs in one persistent launch, replayed from a HIP graph captured on the first call
 // (job-table copy, state reset, persistent kernel; re-captured when the runtime's buffers or
-// configuration change). results[i] = root return value (sort jobs: n); latency_ms[i] =
+// configuration change). results[i] = root return value (sort jobs: the checksum of the sorted keys); latency_ms[i] =
 // completion time of job i after the launch start (all jobs arrive at the start).
 // stats->kernel_ms is the time of the whole graph.
 extern "C" int gtap_stream_run_batched(gtap_stream* S, int64_t* results, double* latency_ms, gtap_run_stats* stats) {
@@ -659,12 +759,13 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
                                        hipHostMallocMapped | hipHostMallocCoherent));
             GTAP_HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&R.jobs_host_dev), R.jobs_host, 0));
         }
+        // The job table is a ring of `ring` slots: job i uses slot i % ring, and the host writes
+        // job i only after job i - ring has completed and been collected, so a served stream
+        // is not bounded by the table.
+        const uint32_t live_cap = max_live > 0 ? (uint32_t)max_live : 1024u;
+        uint32_t ring = std::max<uint32_t>(2 * live_cap, 256u);
+        ring = std::min<uint32_t>(ring, std::min<uint32_t>((uint32_t)n, R.nanchors));
         volatile gtap_job* tab = R.jobs_host;
-        for (int32_t i = 0; i < n; ++i) {
-            gtap_job j = S->table[i];
-            j.status = GTAP_JOB_EMPTY;
-            memcpy((void*)&tab[i], &j, sizeof(j));
-        }
         volatile int32_t* hctl = R.host_ctl;
         hctl[HCTL_JOB_AVAIL * 32] = 0;
         hctl[HCTL_STOP * 32] = 0;
@@ -682,9 +783,27 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
         uint32_t blocks = max_blocks[prof];
         if (R.cfg.blocks && R.cfg.blocks < blocks) blocks = R.cfg.blocks;
         if (blocks * 4 > R.max_workers) blocks = R.max_workers / 4;
-        GtapDev dev = host::make_dev(blocks * 4, GTAP_TERM_SERVE, (uint32_t)n, R.jobs_host_dev, (uint32_t)n);
-        dev.max_live_jobs = max_live > 0 ? (uint32_t)max_live : 1024u;
-        dev.admit_reserve = 4096;  // free task ids required per admitted job
+        GtapDev dev = host::make_dev(blocks * 4, GTAP_TERM_SERVE, (uint32_t)n, R.jobs_host_dev, ring);
+        dev.max_live_jobs = live_cap;
+        // Admission by free task ids. A wave may park ids it freed in its LDS stack (fewer than
+        // kFreeCap) and in its private HBM stack (priv_cap, a quarter of the pool spread over the
+        // waves), where other waves cannot take them; the rest of the pool is the budget that
+        // the reservations of the running jobs share. A job reserves the task ids it may hold
+        // at once (its task count, at most half the budget) and releases them when it
+        // completes; a job whose reservation does not fit waits in the job table. With the
+        // default pool every job's task count fits under the cap, so the pool cannot run dry.
+        {
+            uint64_t pc = (uint64_t)R.nrecs / (4ull * dev.nworkers) / kChunk * kChunk;
+            dev.priv_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(pc, kChunk), kPrivCap);
+            const int64_t parked = (int64_t)dev.nworkers * (int64_t)(kFreeCap + dev.priv_cap);
+            const int64_t budget = (int64_t)R.nrecs - parked;
+            if (budget < (int64_t)(64 * kChunk))
+                return host::invalid_argument("gtap_stream_serve: the task-id pool is too small for the served workers");
+            dev.admit_budget = (uint32_t)std::min<int64_t>(budget / kChunk, 0x7FFFFFFF);
+            dev.job_units = R.job_units;
+        }
+        const uint64_t cap_ids = (uint64_t)dev.admit_budget * kChunk / 2;
+        auto reserve_of = [&](int32_t i) { return std::min<uint64_t>(S->tasks[i], cap_ids); };
         const double expected_s = arrival.back() * 1e-9;
         const uint64_t need = (uint64_t)((expected_s * 3.0 + 30.0) * 1e8);
         if (dev.timeout_ticks < need) dev.timeout_ticks = need;
@@ -707,21 +826,32 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
         const uint64_t deadline = t0 + (uint64_t)((expected_s * 3.0 + 60.0) * 1e9);
         while (done < n) {
             const uint64_t now = host_now_ns();
-            while (submitted < n && now - t0 >= arrival[submitted]) {
-                tab[submitted].t_submit = now;
-                tab[submitted].status = GTAP_JOB_READY;
+            // submit the arrived jobs whose ring slot is free (its previous job collected)
+            while (submitted < n && now - t0 >= arrival[submitted] &&
+                   (submitted < (int32_t)ring || seen[submitted - (int32_t)ring])) {
+                gtap_job j = S->table[submitted];
+                j.t_submit = now;
+                j.reserve_ids = reserve_of(submitted);
+                j.status = GTAP_JOB_EMPTY;
+                volatile gtap_job* slot = &tab[submitted % (int32_t)ring];
+                memcpy((void*)slot, &j, sizeof(j));
+                __atomic_thread_fence(__ATOMIC_SEQ_CST);
+                slot->status = GTAP_JOB_READY;
                 ++submitted;
                 __atomic_thread_fence(__ATOMIC_SEQ_CST);
                 hctl[HCTL_JOB_AVAIL * 32] = submitted;
             }
+            // completions: only the jobs in flight, oldest first
             while (first_pending < submitted && seen[first_pending]) ++first_pending;
             for (int32_t i = first_pending; i < submitted; ++i) {
-                if (!seen[i] && tab[i].status == GTAP_JOB_DONE) {
+                volatile gtap_job* slot = &tab[i % (int32_t)ring];
+                if (!seen[i] && slot->status == GTAP_JOB_DONE) {
                     seen[i] = 1;
                     ++done;
                     const uint64_t tn = host_now_ns();
                     lat[i] = (double)(tn - t0 - arrival[i]) * 1e-6;
-                    res[i] = tab[i].result;
+                    res[i] = slot->result;
+                    S->exec_ms[i] = (double)(slot->t_end - slot->t_start) * 1e-5;
                 }
             }
             if (hipStreamQuery(R.stream) == hipSuccess) break;  // the kernel ended (error or done)
@@ -734,14 +864,16 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
         st = host::finish_launch(dev, blocks, stats);
         if (st) return st;
         for (int32_t i = 0; i < n; ++i) {
-            if (!seen[i]) {
-                if (tab[i].status != GTAP_JOB_DONE) return host::fail(GTAP_ERROR_INVALID_TASKWAIT, "served job did not complete");
-                seen[i] = 1;
-                lat[i] = (double)(host_now_ns() - t0 - arrival[i]) * 1e-6;
-                res[i] = tab[i].result;
-            }
+            if (seen[i]) continue;
+            // not collected in the loop: still in its slot (job i + ring was never written)
+            if (i >= submitted) return host::fail(GTAP_ERROR_INVALID_TASKWAIT, "served job was never submitted");
+            volatile gtap_job* slot = &tab[i % (int32_t)ring];
+            if (slot->status != GTAP_JOB_DONE) return host::fail(GTAP_ERROR_INVALID_TASKWAIT, "served job did not complete");
+            seen[i] = 1;
+            lat[i] = (double)(host_now_ns() - t0 - arrival[i]) * 1e-6;
+            res[i] = slot->result;
+            S->exec_ms[i] = (double)(slot->t_end - slot->t_start) * 1e-5;
         }
-        for (int32_t i = 0; i < n; ++i) S->exec_ms[i] = (double)(tab[i].t_end - tab[i].t_start) * 1e-5;
     }
     if (results) memcpy(results, res.data(), n * sizeof(int64_t));
     if (latency_ms) memcpy(latency_ms, lat.data(), n * sizeof(double));

@@ -1,0 +1,33 @@
+// The same program as fib_epaq.gtap.cu on one queue: the comparison point of the EPAQ form.
+#include <cuda_runtime.h>
+#include "gtap_thread.cuh"
+
+__device__ long long d_result;
+__device__ __constant__ int d_cutoff;
+
+__device__ long long fib_serial(int n) {
+    long long a = 0, b = 1;
+    for (int i = 0; i < n; ++i) {
+        long long c = a + b;
+        a = b;
+        b = c;
+    }
+    return a;
+}
+
+#pragma gtap function
+__device__ long long fib(int n) {
+    if (n < d_cutoff) return fib_serial(n);
+    long long x, y;
+    #pragma gtap task
+    x = fib(n - 1);
+    #pragma gtap task
+    y = fib(n - 2);
+    #pragma gtap taskwait
+    return x + y;
+}
+
+__global__ void fib_kernel(int n) {
+    #pragma gtap entry
+    d_result = fib(n);
+}

@@ -28,8 +28,16 @@ extern "C" {
 int gtap_initialize(void);
 /* Replaces gtap_finalize (gtap_runtime_thread.cuh:342-344). */
 int gtap_finalize(void);
-/* Replaces gtap_reset (gtap_runtime_thread.cuh:439-441): clears state, keeps allocations. */
+/* Replaces gtap_reset (gtap_runtime_thread.cuh:439-441): clears state, keeps allocations, and
+ * arms the state for one host-launched entry kernel (as gtap_initialize does). */
 int gtap_reset(void);
+/* Registers a module written by gtap_amd.lower whose `__global__` entry kernels the host
+ * launches itself (the reference's `kernel<<<GTAP_GRID_SIZE, GTAP_BLOCK_SIZE>>>(...)` with
+ * `#pragma gtap entry` inside, examples/fib/fib.cu:21-46). gtap_initialize and gtap_reset
+ * reset the runtime state, arm it for one such launch, and call `setup` with the runtime
+ * descriptor; the generated code calls this from a static initializer. No reference
+ * counterpart: the reference runtime is compiled into the program's own translation unit. */
+int gtap_register_entry_module(gtap_entry_setup_fn setup);
 /* Runtime knobs that the reference fixes at compile time with GTAP_* macros
  * (runtime/detail/common/gtap_config.cuh:6-30; examples/README.md:97-105).
  * Must precede gtap_initialize. Zero fields take defaults. */

@@ -71,6 +71,8 @@ def lib() -> ctypes.CDLL:
         L.gtap_oracle_jobs.argtypes = [u64, i32, vp, vp, vp]
         L.gtap_oracle_hash_u32.argtypes = [vp, i64]
         L.gtap_oracle_hash_u32.restype = u64
+        L.gtap_oracle_sort_checksum.argtypes = [vp, i64]
+        L.gtap_oracle_sort_checksum.restype = u64
         L.gtap_oracle_spmv.argtypes = [i32, vp, vp, vp, vp, vp]
         L.gtap_oracle_tree.argtypes = [i32, i32, i32, vp, i64, vp]
         L.gtap_oracle_tree_node.argtypes = [i32, i32, i32, vp, i64]
@@ -189,15 +191,22 @@ def hash_u32(a: np.ndarray) -> int:
     return int(lib().gtap_oracle_hash_u32(_p(a), a.size))
 
 
+def sort_checksum(a: np.ndarray) -> int:
+    """A stream sort job's result for its sorted keys `a` (as an int64 bit pattern)."""
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    v = int(lib().gtap_oracle_sort_checksum(_p(a) if a.size else None, a.size))
+    return v - (1 << 64) if v >= 1 << 63 else v
+
+
 def job_result(kind: int, param: int, key_seed: int) -> int:
-    """Expected result of one stream job: fib value, n-queens count, or the hash of the
-    sorted keys."""
+    """Expected result of one stream job: fib value, n-queens count, or the checksum of the
+    sorted keys (as the int64 the device returns)."""
     if kind == 0:
         return fib(param)
     if kind == 1:
         return nq(param)[0]
     keys = gen_keys_u32(param, key_seed)
-    return hash_u32(mergesort_u32(keys))
+    return sort_checksum(mergesort_u32(keys))
 
 
 def ref_binary(name: str) -> Optional[str]:

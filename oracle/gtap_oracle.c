@@ -376,6 +376,17 @@ void gtap_oracle_jobs(uint64_t seed, int32_t njobs, int32_t* kind, int32_t* para
     }
 }
 
+uint64_t gtap_oracle_sort_checksum(const uint32_t* a, int64_t n) {
+    uint64_t sum = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t z = ((uint64_t)i << 32) ^ (uint64_t)a[i];
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        sum += z ^ (z >> 31);
+    }
+    return sum;
+}
+
 uint64_t gtap_oracle_hash_u32(const uint32_t* a, int64_t n) {
     uint64_t h = 0xcbf29ce484222325ULL;
     const unsigned char* p = (const unsigned char*)a;

@@ -99,6 +99,11 @@ void gtap_oracle_bfs(int32_t nv, const int32_t* row, const int32_t* col, int32_t
 void gtap_oracle_jobs(uint64_t seed, int32_t njobs, int32_t* kind, int32_t* param, uint64_t* key_seed);
 /* 64-bit FNV-1a over the bytes of a uint32 array (used to compare sorted outputs). */
 uint64_t gtap_oracle_hash_u32(const uint32_t* a, int64_t n);
+/* A stream sort job's result: sum over positions i of mix((i << 32) ^ a[i]) mod 2^64, with
+ * mix the SplitMix64 finalizer (Steele, Lea, Flood, OOPSLA 2014). Every key is weighed at its
+ * position, and the terms add in any order (the device sums them in parallel). Pinned: a
+ * numpy restatement, and changes under a transposition or one flipped bit. */
+uint64_t gtap_oracle_sort_checksum(const uint32_t* a, int64_t n);
 
 /* SpMV: examples/spmv/spmv_thread.cu:34-46 (one row = sum over the row's entries in column
  * order), with the product-add written as fma(val, x, sum) so the result is one bit pattern.

@@ -18,21 +18,38 @@ def runtime():
     assert gtap_amd.gtap_finalize() == 0
 
 
-def expected_scalar(kind, param):
-    if kind == 0:
-        return oracle.fib(int(param))
-    if kind == 1:
-        return oracle.nq(int(param))[0]
-    return int(param)
+_EXPECTED = {}
 
 
-def check_stream(stream, results, sort_jobs=None):
-    for i in range(stream.njobs):
-        assert results[i] == expected_scalar(stream.kind[i], stream.param[i]), (i, stream.kind[i], stream.param[i])
-    sort_idx = np.where(stream.kind == 2)[0]
-    if sort_jobs is not None:
-        sort_idx = sort_idx[:sort_jobs]
-    for i in sort_idx:
+def expected_results(stream):
+    """The oracle's result of every job: fib value, n-queens count, and for a sort job the
+    checksum of its sorted keys (every key at its position; oracle.job_result, with numpy's
+    sort for speed: the oracle's merge sort equals it, tests/test_oracle.py)."""
+    key = (stream.seed, stream.njobs)
+    if key not in _EXPECTED:
+        out = np.empty(stream.njobs, dtype=np.int64)
+        nq_cache = {}
+        for i in range(stream.njobs):
+            k, p = int(stream.kind[i]), int(stream.param[i])
+            if k == 0:
+                out[i] = oracle.fib(p)
+            elif k == 1:
+                if p not in nq_cache:
+                    nq_cache[p] = oracle.nq(p)[0]
+                out[i] = nq_cache[p]
+            else:
+                out[i] = oracle.sort_checksum(np.sort(oracle.gen_keys_u32(p, int(stream.key_seed[i]))))
+        _EXPECTED[key] = out
+    return _EXPECTED[key]
+
+
+def check_stream(stream, results, sort_jobs=0):
+    """Every job's result against the oracle (for a sort job that checks every sorted key);
+    the first `sort_jobs` sort jobs' keys are also read back and compared directly."""
+    exp = expected_results(stream)
+    bad = np.nonzero(np.asarray(results, dtype=np.int64) != exp)[0]
+    assert bad.size == 0, [(int(i), int(stream.kind[i]), int(stream.param[i])) for i in bad[:8]]
+    for i in np.where(stream.kind == 2)[0][:sort_jobs]:
         keys = oracle.gen_keys_u32(int(stream.param[i]), int(stream.key_seed[i]))
         assert np.array_equal(stream.sorted_keys(int(i)), np.sort(keys)), i
 
@@ -48,7 +65,7 @@ def test_stream_small_all_forms():
     try:
         print("case stream small batched", flush=True)
         res_b, lat_b, stats = st.run_batched()
-        check_stream(st, res_b)
+        check_stream(st, res_b, sort_jobs=48)
         assert stats["jobs_taken"] == 48
         ex_b = st.exec_ms()
         # device time from admission to completion: positive, and within the launch-relative
@@ -76,11 +93,11 @@ def test_stream_full_batched():
     try:
         print("case stream 4096 batched", flush=True)
         res, lat, stats = st.run_batched()
-        check_stream(st, res, sort_jobs=64)
+        check_stream(st, res, sort_jobs=16)  # all 4096 results; every sort job by its checksum
         assert stats["jobs_taken"] == 4096
-        # the same jobs alone (one launch each) for a sample give the same results
         print("case stream 4096 served", flush=True)
         res_s, lat_s, tot_s, _ = st.serve(rate=40000.0, arrival_seed=11)
+        check_stream(st, res_s, sort_jobs=16)
         assert np.array_equal(res, res_s)
     finally:
         st.close()
@@ -94,7 +111,7 @@ def test_stream_graph_replays():
         for rep in range(4):
             print(f"case stream graph replay {rep}", flush=True)
             res, lat, stats = st.run_batched()
-            check_stream(st, res)
+            check_stream(st, res, sort_jobs=96)
             assert stats["jobs_taken"] == 96
             v, _ = gtap_amd.fib(15 + rep)  # eager work on the same stream between replays
             assert v == oracle.fib(15 + rep)

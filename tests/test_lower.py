@@ -113,3 +113,152 @@ def test_built_libraries_export_entries():
         for e in entries + [f"gtap_lowered_{name}_record_bytes"]:
             assert hasattr(L2, e), (path, e)
         del L, L2
+
+
+PLAIN_MSORT = """
+#include "gtap_thread.cuh"
+__device__ int* g_data;
+__device__ int* g_buf;
+__device__ void merge(int* a, int* buf, int l, int m, int r) {
+    int i = l, j = m + 1, k = l;
+    while (i <= m && j <= r) {
+        if (a[i] <= a[j]) buf[k++] = a[i++];
+        else buf[k++] = a[j++];
+    }
+    while (i <= m) buf[k++] = a[i++];
+    for (int t = l; t <= r; ++t) a[t] = buf[t];
+}
+#pragma gtap function
+__device__ void msort(int l, int r) {
+    if (l >= r) return;
+    int m = l + (r - l) / 2;
+    #pragma gtap task
+    msort(l, m);
+    #pragma gtap task
+    msort(m + 1, r);
+    #pragma gtap taskwait
+    merge(g_data, g_buf, l, m, r);
+}
+__global__ void k(int n) {
+    g_buf[0] = load_L2(&g_data[0]);
+    #pragma gtap entry
+    msort(0, n - 1);
+}
+"""
+
+
+def test_plain_accesses_become_coherent():
+    """Task code reads and writes shared data with plain accesses (the reference's idiom,
+    examples/mergesort/mergesort.cu:15-24): every one becomes a coherent load or store, the
+    address of an element stays an address, and the kernel's own statements are rewritten too."""
+    out = lower.lower(PLAIN_MSORT, "m.cu")
+    merge = out[out.index("__device__ void merge("):out.index("__device__ void msort__seq(int l, int r) {")]
+    assert "::gtap::cld(&(a[i])) <= ::gtap::cld(&(a[j]))" in merge
+    assert "::gtap::coh(buf[k++]) = ::gtap::cld(&(a[i++]))" in merge
+    assert "::gtap::coh(a[t]) = ::gtap::cld(&(buf[t]))" in merge
+    kern = out[out.index("void k(int n)"):]
+    assert "::gtap::coh(g_buf[0]) = load_L2(&g_data[0]);" in kern
+    assert "a[i]" not in merge.replace("&(a[i])", "").replace("(a[i++])", "")
+
+
+def test_coherent_rewrite_can_be_turned_off():
+    out = lower.lower(PLAIN_MSORT, "m.cu", coherent=False)
+    assert "::gtap::cld(&(" not in out and "::gtap::coh(" not in out
+    assert "if (a[i] <= a[j]) buf[k++] = a[i++];" in out
+
+
+def test_device_variables_entry_kernel_and_symbols():
+    out = lower.lower(read("fib_epaq.gtap.cu"), "fib_epaq.gtap.cu")
+    assert "__device__ long long d_result;" in out and "__device__ __constant__ int d_cutoff;" in out
+    assert "static constexpr int kQueues = 3;" in out  # #define GTAP_NUM_QUEUES 3 in the source
+    assert "__global__ __launch_bounds__(GTAP_BLOCK_SIZE) void fib_kernel(int n)" in out
+    assert "d_result = ::gtap::result_as<long long>(__gtap_bits);" in out
+    assert "(uint32_t)((fr.n - 1) < d_cutoff ? 1 : 0)" in out  # queue(expr) over a parameter
+    assert "__gtap_set_state_for_join(lane, 1u, (uint32_t)(2))" in out
+    assert 'if (strcmp(name, "d_cutoff") == 0)' in out
+    assert 'extern "C" int gtap_launch_fib_kernel(uint32_t grid, uint32_t block, int n)' in out
+    assert "constexpr uint32_t kTermMode = GTAP_TERM_QUIESCE;" in out
+
+
+def test_switches_from_the_command_line():
+    src = read("fib.gtap.cu")
+    out = lower.lower(src, "fib.gtap.cu", defines={"GTAP_NUM_QUEUES": "2", "GTAP_TERMINATE_ON_FIRST_TASK_FINISH": ""})
+    assert "#define GTAP_NUM_QUEUES 2" in out and "static constexpr int kQueues = 2;" in out
+    assert "constexpr uint32_t kTermMode = GTAP_TERM_JOBS;" in out
+    with pytest.raises(lower.LoweringError):
+        lower.lower(src, "fib.gtap.cu", defines={"GTAP_BLOCK_SIZE": "32"})  # 32-lane warps do not exist here
+
+
+def test_detached_spawns_without_taskwait():
+    """A spawn that no taskwait follows is detached (examples/nq/nq.cu:45-49): the lowering
+    uses the detached spawn, and a program without any taskwait is built without the
+    suspension staging."""
+    out = lower.lower(lower_src_nq(), "nq.cu")
+    assert "__gtap_spawn_task_detached(lane, F_nq" in out and "__gtap_spawn_task(lane" not in out
+    assert "static constexpr bool kNoTaskwait = true;" in out
+
+
+def lower_src_nq():
+    return """
+__device__ int d_answer;
+#pragma gtap function
+__device__ void nq(int n, int row, unsigned cols) {
+    if (row == n) { atomicAdd(&d_answer, 1); return; }
+    for (int c = 0; c < n; ++c) {
+        if (cols & (1u << c)) continue;
+        #pragma gtap task
+        nq(n, row + 1, cols | (1u << c));
+    }
+}
+__global__ void k(int n) {
+    #pragma gtap entry
+    nq(n, 0, 0u);
+}
+"""
+
+
+def test_block_mode_from_the_include():
+    out = lower.lower(read("bsort.gtap.cu"), "bsort.gtap.cu")
+    assert "(block mode)" in out and "::gtap::entry_block<" in out
+    assert "__shared__ unsigned tile[256];" in out  # workgroup memory keeps its declaration
+    save = out[out.index("void bsort__save"):out.index("void bsort__load")]
+    assert "if (threadIdx.x == 0)" in save and "complete_stores" in save
+    assert "struct Shared" in out and "kQueues" not in out[out.index("struct Program"):out.index("struct Program") + 300]
+
+
+def test_utf8_comments_keep_offsets():
+    src = "// 乱数の説明 — コメント\n" + read("fib.gtap.cu").replace("if (n < 2)", "// 葉\n    if (n < 2)")
+    out = lower.lower(src, "fib_utf8.cu")
+    assert "__gtap_spawn_task(lane, F_fib" in out and "葉" in out
+
+
+BAD_MORE = {
+    "memcpy in task code": """
+#pragma gtap function
+__device__ void f(int* a, int* b, int n) {
+    memcpy(a, b, n * 4);
+}
+""",
+    "inline asm in task code": """
+#pragma gtap function
+__device__ void f(int* a) {
+    asm volatile("s_nop 0");
+}
+""",
+    "result without a later taskwait": """
+#pragma gtap function
+__device__ int f(int n) {
+    if (n < 2) return n;
+    int a;
+    #pragma gtap task
+    a = f(n - 1);
+    return a;
+}
+""",
+}
+
+
+@pytest.mark.parametrize("case", sorted(BAD_MORE))
+def test_rejected_unsafe_forms(case):
+    with pytest.raises(lower.LoweringError):
+        lower.lower(BAD_MORE[case], "bad.cu")

@@ -279,7 +279,43 @@ def test_job_results():
     assert oracle.job_result(0, 20, 0) == 6765
     assert oracle.job_result(1, 8, 0) == 92
     keys = oracle.gen_keys_u32(1500, 99)
-    assert oracle.job_result(2, 1500, 99) == oracle.hash_u32(np.sort(keys))
+    assert oracle.job_result(2, 1500, 99) == oracle.sort_checksum(np.sort(keys))
+
+
+def _checksum_np(a):
+    """The sort checksum restated with numpy's wrapping uint64 arithmetic."""
+    a = np.asarray(a, dtype=np.uint64)
+    z = (np.arange(a.size, dtype=np.uint64) << np.uint64(32)) ^ a
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    v = int(z.sum(dtype=np.uint64))
+    return v - (1 << 64) if v >= 1 << 63 else v
+
+
+def test_sort_checksum_pins():
+    # the finalizer maps 0 to 0: the empty array and a single zero key sum to 0
+    assert oracle.sort_checksum(np.array([], dtype=np.uint32)) == 0
+    assert oracle.sort_checksum(np.array([0], dtype=np.uint32)) == 0
+    for n in (1, 2, 7, 1000, 65537):
+        a = np.sort(oracle.gen_keys_u32(n, 3 + n))
+        assert oracle.sort_checksum(a) == _checksum_np(a)
+    a = np.sort(oracle.gen_keys_u32(4096, 11))
+    c = oracle.sort_checksum(a)
+    b = a.copy()
+    b[[10, 11]] = b[[11, 10]]  # a transposition of two different keys
+    assert a[10] != a[11] and oracle.sort_checksum(b) != c
+    b = a.copy()
+    b[2000] ^= 1 << 17  # one flipped bit
+    assert oracle.sort_checksum(b) != c
+    # the terms add in any order: the sum over two halves with their positions is the total
+    h = a.size // 2
+    lo = _checksum_np(a[:h])
+    z = (np.arange(h, a.size, dtype=np.uint64) << np.uint64(32)) ^ a[h:].astype(np.uint64)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    hi = int((z ^ (z >> np.uint64(31))).sum(dtype=np.uint64))
+    assert (lo + hi) % (1 << 64) == c % (1 << 64)
 
 
 def test_hash_is_fnv1a():
