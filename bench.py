@@ -38,6 +38,7 @@ NQ_N = 14
 NQ14_SOLUTIONS = 365_596
 NQ14_TASKS = 27_358_553
 FIB30 = 832_040
+FIB40 = 102_334_155
 FIB30_TASKS = 2_692_537
 
 
@@ -70,6 +71,8 @@ def cpu_baseline(nq_tasks: int, headline_only: bool = False) -> dict:
     else the oracle's restatement. `programs`: the reference's own sequential and OpenMP
     programs on the other configs (fib(30), cilksort of 2^26 keys) and the mixed job stream
     on a bounded sample, with the thread count used."""
+    import numpy as np
+
     import oracle
     ref = oracle.ref_binary("seq_nq")
     times = []
@@ -92,7 +95,7 @@ def cpu_baseline(nq_tasks: int, headline_only: bool = False) -> dict:
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="64M")
     progs = {"threads_openmp": threads, "cpu_model": _cpu_model()}
-    runs = [("seq_fib", [30], 5, None), ("omp_fib", [30], 1, env), ("seq_nq", [NQ_N], 3, None),
+    runs = [("seq_fib", [30], 5, None), ("omp_fib", [30], 3, env), ("seq_nq", [NQ_N], 3, None),
             ("omp_nq", [NQ_N, 7], 3, env)]
     for name, args, n, e in runs:
         b = oracle.ref_binary(name)
@@ -113,14 +116,27 @@ def cpu_baseline(nq_tasks: int, headline_only: bool = False) -> dict:
         finally:
             if os.path.exists(path):
                 os.remove(path)
-    # mixed job stream: the oracle's sequential restatement over the first 256 jobs
-    kinds, params, kseeds = oracle.jobs(2024, 4096)
-    t0 = time.perf_counter()
-    nj = 256
-    for i in range(nj):
-        oracle.job_result(int(kinds[i]), int(params[i]), int(kseeds[i]))
-    dt = time.perf_counter() - t0
-    progs["stream sequential (oracle), first 256 of 4096 jobs"] = {"seconds": dt, "jobs_per_s": nj / dt}
+    # the mixed job stream: all 4096 jobs with an OpenMP parallel-for over the jobs, and the
+    # sequential loop over the first 512 (the same programs as the GPU's jobs, in C)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    dt, _ = oracle.stream_cpu(2024, 4096, True)
+    progs["stream 4096 jobs, OpenMP parallel for schedule(dynamic) over jobs"] = {
+        "seconds": dt, "jobs_per_s": 4096 / dt, "threads": threads}
+    dt, _ = oracle.stream_cpu(2024, 512, False)
+    progs["stream sequential, first 512 of 4096 jobs"] = {"seconds": dt, "jobs_per_s": 512 / dt, "threads": 1}
+    # sequential queue BFS (examples/asynchronous_bfs/asynchronous_bfs_thread.cu:151-174) on
+    # the benchmark's R-MAT 22 graph
+    import gtap_amd
+    row, col = gtap_amd.rmat_csr(22, 16, 1)
+    bfs_s = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        d = oracle.bfs(row, col, 0)
+        bfs_s.append(time.perf_counter() - t0)
+    reached = d < oracle.INF_DEPTH
+    edges = int(np.diff(row)[reached].sum())
+    progs["bfs sequential R-MAT 22"] = {"median_ms": statistics.median(bfs_s) * 1e3, "reached_edges": edges,
+                                        "edges_per_s": edges / statistics.median(bfs_s), "threads": 1}
     out["programs"] = progs
     return out
 
@@ -138,9 +154,12 @@ def _cpu_model() -> str:
 
 def bench_cilksort(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
     """cilksort of 2^26 gen_vector keys (block mode), inputs resident on the GPU; the
-    unsorted keys are restored with a device copy before each run (outside the kernel time)."""
+    unsorted keys are restored with a device copy before each run (outside the kernel time).
+    Runs at the autotuner's committed settings for this workload."""
     import ctypes
     import numpy as np
+    tuned = use_tuned("cilksort")
+    leaf, merge_leaf = int(tuned.get("leaf", 4096)), int(tuned.get("merge_leaf", 0))
     n = 1 << 26
     keys = gtap_amd.gen_keys(n)
     src = gtap_amd.DeviceBuffer.from_array(keys)
@@ -151,7 +170,7 @@ def bench_cilksort(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
     for _ in range(reps + 1):
         L.gtap_memcpy_dtod(work.ptr, src.ptr, keys.nbytes)
         s = gtap_amd.RunStats()
-        st = L.gtap_cilksort(work.ptr, tmp.ptr, n, 4096, 0, ctypes.byref(s))
+        st = L.gtap_cilksort(work.ptr, tmp.ptr, n, leaf, merge_leaf, ctypes.byref(s))
         if st != 0:
             raise RuntimeError(f"gtap_cilksort failed: {st}")
         ms.append(s.kernel_ms)
@@ -163,17 +182,19 @@ def bench_cilksort(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
     gbps = roofline.cilksort_bytes(n) / (med / 1e3) / 1e9
     log(f"cilksort 2^26: {med:.3f} ms, {gbps:.0f} GB/s algorithmic, sorted={ok}")
     return {"kernel_ms_median": med, "keys_per_s": n / (med / 1e3), "bit_exact": ok,
-            "tasks_per_s": s.tasks_executed / (med / 1e3),
+            "tasks_per_s": s.tasks_executed / (med / 1e3), "tuned": tuned,
             "roofline": {"bound": "hbm", "achieved": gbps, "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
                          "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas,
                          "traffic": load_traffic("cilksort_2e26")}}
 
 
-def bench_bfs(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
+def bench_bfs(gtap_amd, roofline, hbm_meas, rand_meas=None, reps: int = 3) -> dict:
     """Asynchronous BFS on the R-MAT scale-22 graph (edge factor 16, seed 1), EPAQ with 4
     degree-bucket queues vs one queue; CSR resident on the GPU."""
     import ctypes
     import numpy as np
+    tuned = use_tuned("bfs")
+    split = int(tuned.get("split", 64))
     row, col = gtap_amd.rmat_csr(22, 16, 1)
     nv = row.size - 1
     drow = gtap_amd.DeviceBuffer.from_array(row)
@@ -186,7 +207,7 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
         ms = []
         for _ in range(reps + 1):
             s = gtap_amd.RunStats()
-            st = L.gtap_bfs(drow.ptr, dcol.ptr, nv, col.size, 0, ddep.ptr, q, 64, ctypes.byref(s))
+            st = L.gtap_bfs(drow.ptr, dcol.ptr, nv, col.size, 0, ddep.ptr, q, split, ctypes.byref(s))
             if st != 0:
                 raise RuntimeError(f"gtap_bfs failed: {st}")
             ms.append(s.kernel_ms)
@@ -200,9 +221,23 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
     med4, tasks4 = res[4]
     gbps = roofline.bfs_bytes(vr, er) / (med4 / 1e3) / 1e9
     log(f"bfs rmat22: epaq4 {med4:.3f} ms, 1 queue {res[1][0]:.3f} ms, {gbps:.0f} GB/s algorithmic")
+    # request-rate roofline: the expansion's random accesses (a depth probe per reached edge,
+    # the stale check and the row-offset pair per reached vertex) against the measured random
+    # 4-byte load rate over a table the size of depth[]
+    reqs = er + 2 * vr
+    req_rate = reqs / (med4 / 1e3) / 1e9
+    request_roofline = None
+    if rand_meas:
+        request_roofline = {"random_requests": reqs, "achieved_greq_s": req_rate,
+                            "measured_random_load_greq_s": rand_meas["load_gps"],
+                            "measured_random_fetch_min_gops": rand_meas["fetch_min_gps"],
+                            "frac": req_rate / rand_meas["load_gps"]}
+        log(f"bfs rmat22 requests: {req_rate:.1f} G random accesses/s of {rand_meas['load_gps']:.1f} measured "
+            f"({req_rate / rand_meas['load_gps']:.2f})")
     return {"kernel_ms_median_epaq4": med4, "kernel_ms_median_1queue": res[1][0],
+            "request_roofline": request_roofline,
             "epaq_speedup": res[1][0] / med4, "reached_vertices": vr, "reached_edges": er,
-            "edges_per_s": er / (med4 / 1e3), "tasks_executed_epaq4": tasks4,
+            "edges_per_s": er / (med4 / 1e3), "tasks_executed_epaq4": tasks4, "tuned": tuned,
             "roofline": {"bound": "hbm", "achieved": gbps, "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
                          "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas,
                          "algorithmic_bytes": roofline.bfs_bytes(vr, er), "traffic": load_traffic("bfs_rmat22")}}
@@ -266,6 +301,104 @@ REFERENCE_GH200_MS = {"nq14_cutoff7": 3.0135, "nq15_cutoff7": 17.175, "nq16_cuto
                       "fib30": 0.958, "fib35": 7.778}
 
 
+def bench_epaq_fib(gtap_amd, n: int = 40, cutoffs=(13, 16), reps: int = 5) -> dict:
+    """EPAQ where the reference shows it paying off (evaluation/3-epaq/fib/fib_queue_3.cu:16-26,
+    epaq_performance_results_fib.csv:13): fib(n) with a host-set cutoff below which the call
+    recurses sequentially, lowered by gtap_amd.lower from gtap_amd/lower/examples/fib_epaq.gtap.cu
+    (children bound for the sequential path on queue 1, the others on queue 0, parents resuming
+    on queue 2) and fib_cut.gtap.cu (the same on one queue), launched by the host the way the
+    reference's main() launches it; hipEvent time of the launch, median of `reps`."""
+    import ctypes
+    from gtap_amd import build as gbuild
+    mods = {name: gtap_amd.EntryModule(gbuild.lowered_lib(name), name) for name in ("fib_cut", "fib_epaq")}
+    ll = ctypes.c_longlong
+    out = {"n": n, "reference_gh200_ms": {"1 queue": 6.036, "3 queues": 3.3495, "cutoff": 13, "n": 40}}
+    expect = None
+    for cut in cutoffs:
+        row = {}
+        for name, m in mods.items():
+            m.set_symbol("d_cutoff", cut)
+            ms = []
+            for _ in range(reps):
+                m.set_symbol("d_result", -1, ll)
+                st, t = m.launch("fib_kernel", [ctypes.c_int], n, timed=True)
+                if st != 0 or gtap_amd.gtap_get_runtime_error_code() != 0:
+                    raise RuntimeError(f"{name}: launch failed")
+                v = m.get_symbol("d_result", ll)
+                expect = v if expect is None else expect
+                if v != expect:
+                    raise RuntimeError(f"{name}: fib({n}) = {v}, other runs gave {expect}")
+                ms.append(t)
+            row["3_queues_ms" if name == "fib_epaq" else "1_queue_ms"] = statistics.median(ms)
+        row["epaq_speedup"] = row["1_queue_ms"] / row["3_queues_ms"]
+        out[f"cutoff_{cut}"] = row
+        log(f"EPAQ fib({n}) cutoff {cut}: 1 queue {row['1_queue_ms']:.3f} ms, 3 queues {row['3_queues_ms']:.3f} ms "
+            f"({row['epaq_speedup']:.2f}x)")
+    if expect != FIB40:
+        raise RuntimeError(f"fib({n}) = {expect}, expected {FIB40}")
+    out["result"] = expect
+    return out
+
+
+def bench_lowered_fib(gtap_amd, n: int = 30, reps: int = 5) -> dict:
+    """The lowering's own output against the hand-lowered program: fib(n) from
+    gtap_amd/lower/examples/fib.gtap.cu (written in the reference's dialect, lowered by
+    gtap_amd.lower, launched by the host as fib_kernel<<<grid, 256>>>) and gtap_fib(n), both
+    with no cutoff; hipEvent time of the launch, median of `reps`."""
+    import ctypes
+    from gtap_amd import build as gbuild
+    m = gtap_amd.EntryModule(gbuild.lowered_lib("fib"), "fib")
+    ll = ctypes.c_longlong
+    lowered = []
+    for _ in range(reps + 1):
+        m.set_symbol("d_fib_result", -1, ll)
+        st, t = m.launch("fib_kernel", [ctypes.c_int], n, timed=True)
+        if st != 0 or m.get_symbol("d_fib_result", ll) != FIB30:
+            raise RuntimeError("lowered fib: wrong result")
+        lowered.append(t)
+    builtin = []
+    for _ in range(reps + 1):
+        v, s = gtap_amd.fib(n)
+        if v != FIB30:
+            raise RuntimeError("gtap_fib: wrong result")
+        builtin.append(s["kernel_ms"])
+    out = {"n": n, "lowered_host_launch_ms": statistics.median(lowered[1:]),
+           "hand_lowered_ms": statistics.median(builtin[1:])}
+    out["lowered_over_hand"] = out["lowered_host_launch_ms"] / out["hand_lowered_ms"]
+    log(f"fib({n}): lowered source, host launch {out['lowered_host_launch_ms']:.3f} ms; hand-lowered "
+        f"{out['hand_lowered_ms']:.3f} ms")
+    return out
+
+
+def bench_round_phases(gtap_amd) -> dict:
+    """Per-round breakdown of the thread-mode loop for n-queens(14) and fib(30): the profiling
+    build of the kernel stamps the shader clock at each phase boundary of every round (acquire,
+    record loads, task bodies, record writes issued, their completion, join atomics, publish)
+    and sums per worker. This build waits for the record loads and the join atomics before
+    stamping, so its rounds are slower than the plain kernel's; the shares show where a round
+    goes. The clock rate is estimated from the total stamped cycles and the kernel time."""
+    out = {}
+    gtap_amd.gtap_set_profiling(True, 2)
+    try:
+        for name, fn, arg in (("nq14", gtap_amd.nqueens, NQ_N), ("fib30", gtap_amd.fib, 30)):
+            fn(arg)  # warm-up
+            v, st = fn(arg)
+            ph = gtap_amd.round_phases()
+            total = sum(ph["cycles"].values())
+            ghz = total / max(1, st["workers"]) / (st["kernel_ms"] * 1e6)
+            busy = total - ph["cycles"]["idle"]
+            share = {k: ph["cycles"][k] / busy for k in gtap_amd.ROUND_PHASES[:7]} if busy else {}
+            per_round_us = {k: c / ghz / 1e3 for k, c in ph["cycles_per_round"].items()} if ghz else {}
+            out[name] = {"kernel_ms_profiling_build": st["kernel_ms"], "rounds": ph["rounds"],
+                         "clock_ghz_estimate": ghz, "cycles_per_round": ph["cycles_per_round"],
+                         "us_per_round": per_round_us, "share_of_busy_time": share,
+                         "idle_share": ph["cycles"]["idle"] / total if total else None}
+            log(f"round phases {name}: " + ", ".join(f"{k} {v * 100:.0f}%" for k, v in share.items()))
+    finally:
+        gtap_amd.gtap_set_profiling(False, 0)
+    return out
+
+
 def bench_reference_settings(gtap_amd, reps: int = 5) -> dict:
     """The reference's own benchmark programs at its own settings, timed here, beside the
     medians it publishes for 1x GH200: n-queens with tasks to row 7 and a sequential search
@@ -322,7 +455,7 @@ def _rate_at_p99(points, form: str, p99_ms: float):
 
 
 def served_equal_p99(gtap_amd, seed: int, njobs: int, expected, lo: float = 1000.0, hi: float = 80000.0,
-                     step: float = 1.25) -> dict:
+                     step: float = 1.25, merge_leaf: int = 4096) -> dict:
     """Poisson arrivals of the first `njobs` jobs of the stream at rates lo, lo*step, ... hi,
     served by one resident launch and by one launch per job; latency = arrival to the result
     seen by the host, for both. The per-job form is swept until it saturates (it completes
@@ -332,6 +465,7 @@ def served_equal_p99(gtap_amd, seed: int, njobs: int, expected, lo: float = 1000
     import numpy as np
     points = []
     sv = gtap_amd.Stream(seed=seed, njobs=njobs)
+    sv.set_merge_leaf(merge_leaf)
     saturated = 0
     try:
         rate = lo
@@ -385,7 +519,10 @@ def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024, serve_jobs: int 
     launch and by one launch per job, with p50/p99 latency from arrival to the result seen by
     the host, and the rate each sustains at equal p99."""
     import numpy as np
+    tuned = use_tuned("stream")
+    merge_leaf = int(tuned.get("merge_leaf", 4096))
     st = gtap_amd.Stream(seed=seed, njobs=njobs)
+    st.set_merge_leaf(merge_leaf)
     try:
         bat = []
         for _ in range(3):
@@ -400,6 +537,7 @@ def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024, serve_jobs: int 
         st.close()
     out = {
         "jobs": njobs,
+        "tuned": tuned,
         "batched": {"kernel_ms_median": kms, "jobs_per_s": njobs / (kms / 1e3),
                     "p50_ms": float(np.percentile(lat_b, 50)), "p99_ms": float(np.percentile(lat_b, 99))},
         # every job arrives at the start in both forms: latency = its completion after the start
@@ -407,10 +545,21 @@ def bench_stream(gtap_amd, njobs: int = 4096, seed: int = 2024, serve_jobs: int 
                            "p50_ms": float(np.percentile(lat_p, 50)), "p99_ms": float(np.percentile(lat_p, 99))},
     }
     out["batched_over_per_job"] = out["batched"]["jobs_per_s"] / out["per_job_launch"]["jobs_per_s"]
-    out["served"] = served_equal_p99(gtap_amd, seed, serve_jobs, res_b[:serve_jobs])
+    out["served"] = served_equal_p99(gtap_amd, seed, serve_jobs, res_b[:serve_jobs], merge_leaf=merge_leaf)
     log(f"stream: batched {out['batched']['jobs_per_s']:.0f} jobs/s, per-job {out['per_job_launch']['jobs_per_s']:.0f}"
         f" jobs/s ({out['batched_over_per_job']:.1f}x)")
     return out
+
+
+def use_tuned(name: str) -> dict:
+    """Re-initialises the runtime with the autotuner's committed best point for `name` (its
+    gtap_config fields; the program arguments among them are returned for the caller) and
+    returns that point ({} = defaults)."""
+    from gtap_amd import autotune
+    params = autotune.tuned(name)
+    autotune.apply(params, timeout_ms=30000)
+    log(f"tuned settings for {name}: {params or 'defaults'}")
+    return params
 
 
 def load_traffic(name: str):
@@ -447,11 +596,15 @@ def main() -> int:
     st = gtap_amd.gtap_initialize()
     if st != 0:
         raise RuntimeError(f"gtap_initialize failed: {st}")
+    tuned_nq = use_tuned("nq14")
     info = gtap_amd.device_info()
     log("device", info)
     hbm_peak_meas = gtap_amd.bench_hbm_copy(2 << 30, 3)
     atomic_peak_meas = gtap_amd.bench_atomics(3)
-    log(f"measured peaks: hbm copy {hbm_peak_meas:.0f} GB/s, atomics {atomic_peak_meas:.2f} Gop/s")
+    atomic_contended = {str(w): gtap_amd.bench_atomics_contended(w, 3) for w in (1, 64, 4096)}
+    rand_meas = gtap_amd.bench_random_access(16 << 20, 3)  # the size of depth[] at R-MAT scale 22
+    log(f"measured peaks: hbm copy {hbm_peak_meas:.0f} GB/s, atomics {atomic_peak_meas:.2f} Gop/s "
+        f"(contended over 1 / 64 / 4096 words: " + " / ".join(f"{v:.2f}" for v in atomic_contended.values()) + " Gop/s)")
 
     for i in range(args.warmup):
         v, s = gtap_amd.nqueens(NQ_N)
@@ -478,6 +631,7 @@ def main() -> int:
     # fib(30): the other scheduler-bound config
     fib_tasks = FIB30_TASKS
     fib_ms = []
+    tuned_fib = {} if args.headline_only else use_tuned("fib30")
     for _ in range(0 if args.headline_only else 5):
         fv, fs = gtap_amd.fib(30)
         if fv != FIB30 or fs["tasks_spawned"] + 1 != fib_tasks:
@@ -487,6 +641,8 @@ def main() -> int:
 
     # the same n-queens tasks with result slots (store + join atomic per child)
     slot_ms = []
+    if not args.headline_only:
+        use_tuned("nq14")
     for _ in range(0 if args.headline_only else 5):
         sv, ss = gtap_amd.nqueens_slots(NQ_N)
         if sv != nq_solutions or ss["tasks_spawned"] + 1 != nq_tasks:
@@ -494,13 +650,18 @@ def main() -> int:
         slot_ms.append(ss["kernel_ms"])
     slot_med = statistics.median(slot_ms) if slot_ms else None
 
-    extra_sort = extra_bfs = extra_stream = fences = extra_more = extra_ref = None
+    extra_sort = extra_bfs = extra_stream = fences = extra_more = extra_ref = extra_epaq = extra_phases = extra_lowered = None
     if not args.headline_only:
         extra_sort = bench_cilksort(gtap_amd, roofline, hbm_peak_meas)
-        extra_bfs = bench_bfs(gtap_amd, roofline, hbm_peak_meas)
+        extra_bfs = bench_bfs(gtap_amd, roofline, hbm_peak_meas, rand_meas)
         extra_stream = bench_stream(gtap_amd)
+        from gtap_amd import autotune
+        autotune.apply({}, timeout_ms=30000)  # the remaining extras run at the runtime's defaults
         extra_more = bench_spmv_tree(gtap_amd)
         extra_ref = bench_reference_settings(gtap_amd)
+        extra_epaq = bench_epaq_fib(gtap_amd)
+        extra_phases = bench_round_phases(gtap_amd)
+        extra_lowered = bench_lowered_fib(gtap_amd)
         fences = gtap_amd.bench_fences(max(4, workers // 4 * 4), 64)
 
     cpu = None if args.no_cpu else cpu_baseline(nq_tasks, args.headline_only)
@@ -526,6 +687,9 @@ def main() -> int:
             "tasks_per_step": nq_tasks,
             "workers": workers,
             "parallelism": "single GPU, wavefront workers",
+            # the autotuner's best point on an MI355X for this workload (gtap_config fields;
+            # {} = the runtime's defaults)
+            "tuned": tuned_nq,
         },
         "roofline": {
             "bound": "hbm",
@@ -546,12 +710,14 @@ def main() -> int:
             "atomic_roofline": {
                 "achieved_gops": atomics_per_s / 1e9,
                 "measured_peak_gops": atomic_peak_meas,
+                "contended_gops_by_words": atomic_contended,
                 "frac": atomics_per_s / 1e9 / atomic_peak_meas if atomic_peak_meas else None,
             },
             "hbm_measured_copy_gbps": hbm_peak_meas,
+            "random_access_measured": rand_meas,
             "nq14_slot_join": None if slot_med is None else {"kernel_ms_median": slot_med,
                                                              "tasks_per_s": nq_tasks / (slot_med / 1e3)},
-            "fib30": None if fib_med is None else {"kernel_ms_median": fib_med, "tasks": fib_tasks,
+            "fib30": None if fib_med is None else {"kernel_ms_median": fib_med, "tasks": fib_tasks, "tuned": tuned_fib,
                                                    "tasks_per_s": fib_tasks / (fib_med / 1e3),
                                                    "reference_gh200_ms": REFERENCE_GH200_MS["fib30"]},
             "cilksort_2e26": extra_sort,
@@ -559,6 +725,9 @@ def main() -> int:
             "stream_4096": extra_stream,
             "spmv_tree": extra_more,
             "reference_settings": extra_ref,
+            "epaq_fib": extra_epaq,
+            "round_phases": extra_phases,
+            "lowered_fib30": extra_lowered,
             "publication_order_ns_per_wave": fences,
             "device": info,
         },

@@ -83,7 +83,8 @@ class Config(ctypes.Structure):
         ("block_workers", ctypes.c_uint32),
         ("idle_sleep", ctypes.c_uint32),
         ("scheduler", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 4),
+        ("epaq_publish", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 3),
     ]
 
 
@@ -135,6 +136,9 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_get_runtime_error_string": ([_i], ctypes.c_char_p),
     "gtap_get_runtime_error_report": ([ctypes.POINTER(ErrorReport)], _i),
     "gtap_visualize_profile": ([ctypes.c_char_p], None),
+    "gtap_visualize_having_task_time": ([ctypes.c_char_p], None),
+    "gtap_get_round_phases": ([_vp, _u32], _i),
+    "gtap_visualize_working_time": ([ctypes.c_char_p], None),
     "gtap_set_profiling": ([_i, _u32], _i),
     "gtap_bench_fences": ([_u32, _i, ctypes.POINTER(ctypes.c_double)], _i),
     "gtap_get_last_stats": ([ctypes.POINTER(RunStats)], _i),
@@ -148,6 +152,8 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_memset_device": ([_vp, _i, _u64], _i),
     "gtap_bench_hbm_copy": ([_u64, _i, ctypes.POINTER(ctypes.c_double)], _i),
     "gtap_bench_atomics": ([_i, ctypes.POINTER(ctypes.c_double)], _i),
+    "gtap_bench_atomics_contended": ([_i, _u32, ctypes.POINTER(ctypes.c_double)], _i),
+    "gtap_bench_random_access": ([_u64, _i, ctypes.POINTER(ctypes.c_double)], _i),
     "gtap_fib": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
     "gtap_nqueens": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
     "gtap_nqueens_slots": ([_i, ctypes.POINTER(_i64), ctypes.POINTER(RunStats)], _i),
@@ -166,6 +172,7 @@ SIGNATURES: dict[str, tuple[list, object]] = {
     "gtap_stream_run_batched": ([_vp, _vp, _vp, ctypes.POINTER(RunStats)], _i),
     "gtap_stream_run_per_job": ([_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)], _i),
     "gtap_stream_exec_ms": ([_vp, _vp], _i),
+    "gtap_stream_set_merge_leaf": ([_vp, _i], _i),
     "gtap_stream_serve": ([_vp, ctypes.c_double, _u64, ctypes.c_int32, ctypes.c_int32, _vp, _vp,
                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(RunStats)], _i),
     "gtap_rmat_edges": ([_i, _i64, _u64, _u32, _u32, _u32, _vp, _vp], _i),
@@ -228,6 +235,8 @@ WORKER_STAT_FIELDS = ("tasks_executed", "tasks_spawned", "rounds", "pops", "stea
                       "jobs_taken", "pool_refills", "published", "overflowed", "suspended")
 WORKER_START_TICKS = 11  # column of worker_stats(): 100 MHz wall-clock tick at which the worker started
 WORKER_END_TICKS = 12    # and the tick at which it stopped
+WORKER_RING_SPILLED = 13  # tasks the worker moved from a full deque to the overflow ring
+WORKER_RING_TAKEN = 14    # tasks the worker took from the overflow ring
 
 
 def worker_stats():
@@ -276,6 +285,16 @@ def gtap_get_runtime_error_string(code: int) -> str:
 
 def gtap_visualize_profile(app_name: str) -> None:
     lib().gtap_visualize_profile(app_name.encode())
+
+
+def visualize_having_task_time(app_name: str) -> None:
+    """Text timeline + having-task CSVs of the last profiled launch."""
+    lib().gtap_visualize_having_task_time(app_name.encode())
+
+
+def visualize_working_time(app_name: str) -> None:
+    """Text timeline + working CSVs of the last profiled launch."""
+    lib().gtap_visualize_working_time(app_name.encode())
 
 
 _LOWERED: dict = {}
@@ -428,6 +447,22 @@ def gtap_set_profiling(enable: bool, max_samples: int = 4096) -> int:
     return lib().gtap_set_profiling(1 if enable else 0, max_samples)
 
 
+ROUND_PHASES = ("acquire", "load", "run", "stage", "store_wait", "join", "publish", "idle")
+
+
+def round_phases() -> dict:
+    """Round-phase breakdown of the last thread-mode launch run with timeline profiling on:
+    shader-clock cycles per phase summed over the workers, the rounds, and cycles per round
+    of each phase (idle tries are not rounds; their total is reported as is)."""
+    import numpy as np
+    out = np.zeros(9, dtype=np.uint64)
+    _check(lib().gtap_get_round_phases(out.ctypes.data, 9), "gtap_get_round_phases")
+    rounds = int(out[8])
+    res = {"rounds": rounds, "cycles": {k: int(out[i]) for i, k in enumerate(ROUND_PHASES)}}
+    res["cycles_per_round"] = {k: (int(out[i]) / rounds if rounds else 0.0) for i, k in enumerate(ROUND_PHASES[:7])}
+    return res
+
+
 def last_stats() -> dict:
     s = RunStats()
     _check(lib().gtap_get_last_stats(ctypes.byref(s)), "gtap_get_last_stats")
@@ -481,6 +516,20 @@ def bench_hbm_copy(nbytes: int = 2 << 30, reps: int = 3) -> float:
     g = ctypes.c_double(0)
     _check(lib().gtap_bench_hbm_copy(nbytes, reps, ctypes.byref(g)), "gtap_bench_hbm_copy")
     return g.value
+
+
+def bench_atomics_contended(words: int, reps: int = 3) -> float:
+    """Agent-scope relaxed atomic adds with every lane on one of `words` words (Gop/s)."""
+    g = ctypes.c_double(0)
+    _check(lib().gtap_bench_atomics_contended(reps, words, ctypes.byref(g)), "gtap_bench_atomics_contended")
+    return g.value
+
+
+def bench_random_access(table_bytes: int = 16 << 20, reps: int = 3) -> dict:
+    """G random 4-byte accesses/s over a table: plain loads and agent-scope fetch-min."""
+    out = (ctypes.c_double * 2)()
+    _check(lib().gtap_bench_random_access(table_bytes, reps, out), "gtap_bench_random_access")
+    return {"load_gps": out[0], "fetch_min_gps": out[1]}
 
 
 def bench_atomics(reps: int = 3) -> float:
@@ -766,6 +815,10 @@ class Stream:
         _check(lib().gtap_stream_run_per_job(self.h, res.ctypes.data, lat.ctypes.data, ctypes.byref(tot)),
                "gtap_stream_run_per_job")
         return res, lat, tot.value
+
+    def set_merge_leaf(self, merge_leaf: int) -> None:
+        """Sort jobs merge runs of at most merge_leaf keys (1024..16384) on one wave."""
+        _check(lib().gtap_stream_set_merge_leaf(self.h, int(merge_leaf)), "gtap_stream_set_merge_leaf")
 
     def exec_ms(self):
         """Device time from admission to completion of each job of the last run, in ms."""

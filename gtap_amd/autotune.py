@@ -19,13 +19,14 @@ from __future__ import annotations
 
 import argparse
 import json
+import os
 import statistics
 import sys
 import time
 from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 # gtap_config fields the tuner may set (0 = the runtime's default / every co-resident workgroup)
-CONFIG_KEYS = ("blocks", "block_workers", "probe_lanes", "steal_batch", "deque_capacity")
+CONFIG_KEYS = ("blocks", "block_workers", "probe_lanes", "steal_batch", "deque_capacity", "epaq_publish", "idle_sleep")
 
 # The first value of each coordinate is the runtime's default and the search's start point.
 DEFAULT_SPACE: Dict[str, Sequence[int]] = {
@@ -80,10 +81,27 @@ def coordinate_descent(measure: Measure, space: Dict[str, Sequence[int]], start:
     return best, table
 
 
-def apply(params: Dict[str, int]) -> None:
-    """Re-initialises the runtime with the gtap_config fields of `params`."""
+# Best points found on an MI355X by `python -m gtap_amd.autotune --workload W`, one entry per
+# workload: {"best": {...}, "best_ms": ..., "default_ms": ...}.
+TUNED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_mi355x.json")
+
+
+def tuned(name: str) -> Dict[str, int]:
+    """The committed best parameters of a workload ({} when it was not tuned)."""
+    try:
+        with open(TUNED_PATH) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    return dict(table.get(name, {}).get("best", {}))
+
+
+def apply(params: Dict[str, int], **base: int) -> None:
+    """Re-initialises the runtime with the gtap_config fields of `params` (over `base`, e.g.
+    timeout_ms; fields in neither take the runtime's defaults)."""
     import gtap_amd
-    cfg = {k: int(v) for k, v in params.items() if k in CONFIG_KEYS}
+    cfg = dict(base)
+    cfg.update({k: int(v) for k, v in params.items() if k in CONFIG_KEYS})
     gtap_amd.gtap_finalize()
     if gtap_amd.gtap_set_config(**cfg) != 0:
         raise RuntimeError(f"gtap_set_config({cfg}) failed")
@@ -101,9 +119,36 @@ def _median_ms(fn: Callable[[], Tuple[object, float]], reps: int) -> Tuple[float
 
 def workload(name: str, reps: int = 3) -> Tuple[Measure, Dict[str, Sequence[int]]]:
     """A measure function and its search space for a built-in workload:
-    fibN / nqN (thread mode), cilksort (2^26 keys, block mode), bfs (R-MAT scale 22), stream."""
+    fibN / nqN (thread mode, no cutoff), nqcutN (spawn cutoff row), cilksort (2^26 keys, block
+    mode, leaf and merge cutoffs), bfs (R-MAT scale 22, split), spmv (row cutoff), stream (4096
+    jobs, sort jobs' merge leaf)."""
     import gtap_amd
     space = dict(DEFAULT_SPACE)
+
+    if name.startswith("nqcut"):
+        n = int(name[5:])
+        space["cutoff"] = (7, 4, 5, 6, 8, 9, 10)  # rows run as tasks (nq.cu uses 7)
+
+        def measure(p):
+            apply(p)
+            return _median_ms(lambda: (lambda r: (r[0], r[1]["kernel_ms"]))(
+                gtap_amd.nqueens_cutoff(n, p.get("cutoff", 7))), reps)
+        return measure, space
+
+    if name == "spmv":
+        rp, ci, va = gtap_amd.spmv_random(1 << 16, 300, 42)
+        import numpy as np
+        x = np.linspace(-1.0, 1.0, rp.size - 1)
+        space["cutoff"] = (16, 4, 8, 32, 64, 128)  # rows per leaf task (spmv_thread.cu uses 16)
+
+        def measure(p):
+            apply(p)
+
+            def one():
+                y, st = gtap_amd.spmv(rp, ci, va, x, p.get("cutoff", 16))
+                return hash(y.tobytes()), st["kernel_ms"]
+            return _median_ms(one, reps)
+        return measure, space
 
     if name.startswith("fib") or name.startswith("nq"):
         n = int(name[3:] if name.startswith("fib") else name[2:])
@@ -117,13 +162,14 @@ def workload(name: str, reps: int = 3) -> Tuple[Measure, Dict[str, Sequence[int]
     if name == "cilksort":
         import numpy as np
         keys = gtap_amd.gen_keys(1 << 26, 123456789)
-        space = {"block_workers": (0, 256, 512, 768), "merge_leaf": (32768, 16384, 65536)}
+        space = {"block_workers": (0, 256, 512, 768), "merge_leaf": (32768, 16384, 65536, 131072),
+                 "leaf": (4096, 2048, 1024)}
 
         def measure(p):
             apply(p)
 
             def one():
-                out, st = gtap_amd.cilksort(keys, merge_leaf=p.get("merge_leaf", 32768))
+                out, st = gtap_amd.cilksort(keys, leaf=p.get("leaf", 4096), merge_leaf=p.get("merge_leaf", 32768))
                 return hash(out.tobytes()), st["kernel_ms"]
             return _median_ms(one, reps)
         return measure, space
@@ -133,6 +179,7 @@ def workload(name: str, reps: int = 3) -> Tuple[Measure, Dict[str, Sequence[int]
         row, col = gtap_amd.rmat_csr(22, 16, 1)
         space = dict(DEFAULT_SPACE)
         space["split"] = (64, 32, 128, 256)
+        space["epaq_publish"] = (0, 1)
 
         def measure(p):
             apply(p)
@@ -144,9 +191,13 @@ def workload(name: str, reps: int = 3) -> Tuple[Measure, Dict[str, Sequence[int]
         return measure, space
 
     if name == "stream":
+        space = {k: DEFAULT_SPACE[k] for k in ("blocks", "probe_lanes", "steal_batch")}
+        space["merge_leaf"] = (4096, 2048, 8192, 16384)  # sort jobs' one-wave merge size
+
         def measure(p):
             apply(p)
             s = gtap_amd.Stream(seed=2024, njobs=4096)
+            s.set_merge_leaf(p.get("merge_leaf", 4096))
             try:
                 def one():
                     res, _, st = s.run_batched()
@@ -161,7 +212,7 @@ def workload(name: str, reps: int = 3) -> Tuple[Measure, Dict[str, Sequence[int]
 
 def main(argv: Optional[List[str]] = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--workload", default="nq14", help="fibN, nqN, cilksort, bfs or stream")
+    ap.add_argument("--workload", default="nq14", help="fibN, nqN, nqcutN, cilksort, bfs, spmv or stream")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--passes", type=int, default=2)
     ap.add_argument("--out", default=None, help="JSON file for the best parameters and the table")

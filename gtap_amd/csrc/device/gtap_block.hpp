@@ -20,7 +20,7 @@
 namespace gtap {
 
 constexpr uint32_t kBlockThreads = 256;
-constexpr uint32_t kBlockMaxSpawn = 128;   // spawns per task invocation (ids kept ready in LDS)
+constexpr uint32_t kBlockMaxSpawn = 128;   // ids kept ready in LDS for one task invocation's spawns
 
 // Scheduler loop state of thread 0, kept in LDS rather than in registers: it stays live
 // across every task body, where registers are scarce.
@@ -49,6 +49,7 @@ struct BlockCtx {
     uint32_t priv_top;     // ids in this workgroup's HBM overflow stack
     uint32_t child_count;  // joined children spawned by the current invocation
     uint32_t taken;        // ids taken from free_ids by the current invocation
+    uint32_t priv_taken;   // ids taken from the HBM stack by the current invocation (LDS ran out)
     uint32_t spawns;
     uint32_t refills;
     // header of the current task (written by thread 0, read by all)
@@ -215,14 +216,24 @@ struct BlockTask {
     template <typename A>
     __device__ __forceinline__ uint32_t spawn_impl(uint32_t f, uint32_t par, uint32_t roff, const A& a) {
         static_assert(sizeof(A) % 8 == 0 && sizeof(A) / 4 <= P::kFrameWords, "argument block");
-        // free_top does not change while a task runs; spawns claim ids below it
+        // free_top and priv_top do not change while a task runs; spawns claim ids below them:
+        // the LDS stack first, then this workgroup's HBM stack, then never-used ids (a task
+        // may spawn any number of children, as a vertex task of a block-mode BFS does)
         const uint32_t k = atomicAdd(&c->taken, 1u);
         const uint32_t avail = c->free_top;
         uint32_t id = 0;
-        if (k >= avail || k >= kBlockMaxSpawn) {
-            GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_GENERATED_TASK_ID_BUFFER_OVERFLOW, tid, -1, (int)k, (int)kBlockMaxSpawn);
-        } else {
+        if (k < avail) {
             id = c->free_ids[avail - 1 - k];
+        } else {
+            const uint32_t k2 = atomicAdd(&c->priv_taken, 1u);
+            const uint32_t ptop = c->priv_top;
+            if (k2 < ptop) {
+                id = dev->priv_ids[(size_t)me * kPrivCap + (ptop - 1 - k2)];
+            } else {
+                const uint32_t f = atomic_add(pool_fresh_ptr(*dev), 1u);
+                if (f < dev->nrecs) id = f;
+                else GTAP_DEVICE_ERROR(*dev, GTAP_ERROR_GENERATED_TASK_ID_BUFFER_OVERFLOW, tid, -1, (int)k, (int)avail);
+            }
         }
         uint8_t* crec = rec_ptr(*dev, id);
         st64(reinterpret_cast<uint64_t*>(crec), hdr_lo(par, roff));
@@ -450,6 +461,7 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
         c->priv_top = 0;
         c->child_count = 0;
         c->taken = 0;
+        c->priv_taken = 0;
         c->spawns = 0;
         c->refills = 0;
     }
@@ -587,6 +599,9 @@ __device__ void block_scheduler(const GtapDev& dev, const typename P::Args& args
                 const uint32_t used = c->taken < c->free_top ? c->taken : c->free_top;
                 c->free_top -= used;
                 c->taken = 0;
+                const uint32_t pused = c->priv_taken < c->priv_top ? c->priv_taken : c->priv_top;
+                c->priv_top -= pused;
+                c->priv_taken = 0;
             }
             if (c->outcome == OUT_FINISHED) {
                 const uint32_t par = c->parent;

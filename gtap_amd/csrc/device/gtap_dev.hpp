@@ -63,6 +63,8 @@ enum StatIndex : int {
     ST_PUBLISHED, ST_OVERFLOWED, ST_SUSPENDED,
     ST_START_TICKS,  // wall-clock tick at which the worker entered the scheduler loop
     ST_END_TICKS,    // wall-clock tick at which it left the loop
+    ST_RING_SPILLED, // tasks this worker moved from a full deque to the overflow ring
+    ST_RING_TAKEN,   // tasks this worker took from the overflow ring
     ST_COUNT = 16
 };
 
@@ -83,6 +85,7 @@ struct GtapDev {
     uint64_t* prof_having;     // [worker][prof_max] ns stamps: even = has tasks, odd = ran dry
     uint64_t* prof_working;    // [worker][prof_max] ns stamps: even = round start, odd = round end
     int32_t* prof_counts;      // [worker][prof_max] tasks in the round of each working stamp
+    uint64_t* prof_phase;      // [worker][PH_COUNT] shader-clock cycles per round phase (profiling build)
     uint64_t timeout_ticks;    // watchdog, wall-clock ticks (100 MHz)
     uint32_t stride;           // record stride in bytes (multiple of 16)
     uint32_t nrecs;            // task ids [0, nrecs); ids [nrecs, nrecs + nanchors) are anchors
@@ -109,6 +112,20 @@ struct GtapDev {
     uint32_t gq_cap;           // global-queue scheduler: ring slots per EPAQ queue (power of two)
     uint32_t* gq_slots;        // global-queue scheduler: [queue][gq_cap] task ids
     uint64_t* gq_ctl;          // global-queue scheduler: [queue][head, reserved, committed] * 16
+    uint64_t* ovf_cells;       // deque overflow rings: [queue][ovf_cap] cells (nullptr: none)
+    uint64_t* ovf_ctl;         // [queue * 32] head, [queue * 32 + 16] tickets (positions, never reset)
+    uint32_t ovf_cap;          // cells per overflow ring (power of two)
+    uint32_t epaq_publish;     // 1: non-batch EPAQ queues publish their whole private window each round
+};
+
+// Round phases of the thread-mode loop, timed in the profiling build with the shader clock
+// (s_memtime): acquiring a batch (pop, admission, steal, ring), loading the batch's records
+// (waited for in this build), running the task bodies, issuing the round's record writes,
+// waiting for them to complete, the join atomics, appending continuations and publishing, and
+// idle tries that found nothing. PH_ROUNDS counts the rounds.
+enum PhaseIndex : int {
+    PH_ACQUIRE = 0, PH_LOAD, PH_RUN, PH_STAGE, PH_WAIT, PH_JOIN, PH_PUBLISH, PH_IDLE, PH_ROUNDS,
+    PH_COUNT = 16
 };
 
 // Timeline sample (profiling on): ns since an arbitrary epoch, from the 100 MHz wall clock.

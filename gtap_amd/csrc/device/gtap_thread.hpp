@@ -78,6 +78,7 @@ struct WaveCtx {
     uint32_t spawns;
     uint32_t refills;
     uint32_t overflowed;    // LDS-held records written out by an append that pushed them out
+    uint32_t ring_spilled;  // tasks moved to the overflow ring (deque near capacity)
     uint32_t tail[Q];       // owner's private tail per queue (absolute position)
     uint32_t bottom[Q];     // published bottom per queue (low word of the deque word)
     uint32_t top_cache[Q];  // last top observed per queue
@@ -373,6 +374,19 @@ __device__ __forceinline__ void record_from_desc(const GtapDev& dev, __amdgpu_bu
     frame_words_st<FW>(rs, off, f);
 }
 
+// The same from descriptor words held in registers (a spawn that goes straight to a ring).
+template <class P>
+__device__ __forceinline__ void record_from_words(const GtapDev& dev, __amdgpu_buffer_rsrc_t rs, uint32_t id,
+                                                  const uint32_t* d) {
+    constexpr int FW = CtxShape<P>::kFW;
+    const uint32_t off = id * dev.stride;
+    uint32_t f[FW];
+#pragma unroll
+    for (int i = 0; i < FW; ++i) f[i] = d[3 + i];
+    st128(rs, off, u32x4{d[1], d[2], 0u, d[0]});
+    frame_words_st<FW>(rs, off, f);
+}
+
 // Copies an LDS-held entry into staging entry `at` in record order (header, frame words).
 template <class P, class W>
 __device__ __forceinline__ void stage_from_desc(W* w, uint32_t at, uint32_t id, const uint32_t* d) {
@@ -387,14 +401,161 @@ __device__ __forceinline__ void stage_from_desc(W* w, uint32_t at, uint32_t id, 
     vst(&w->stage_id[at], id);
 }
 
+// ------------------------------------------------------------------ deque overflow ring
+// One MPMC ring of task ids per EPAQ queue takes published tasks off a deque that nears its
+// capacity, where the reference traps (runtime/detail/thread/gtap_runtime_thread.cuh:42-45).
+// A cell is one 64-bit word: state [62..63] | position tag [32..61] (position mod 2^30) |
+// task id [0..31]. The cell of position p is free for p when it reads FREE with tag p. A
+// producer takes positions with a fetch-add on the ticket counter and, once the task's
+// record is complete, writes READY | p | id; a consumer claims a run of READY cells at the
+// head with one CAS, reads the ids and frees each cell for p + cap. Positions only grow:
+// the counters are never reset, and the reset kernel frees what an early end leaves between
+// the head and the tickets. Idle waves take from the rings; a wave does not count itself idle
+// while a ring holds tickets (quiescence then still implies that no task is anywhere).
+constexpr uint64_t kOvfReady = 2ull << 62;
+constexpr uint64_t kOvfTagMask = 0x3FFFFFFFull;
+constexpr uint32_t kOvfSpillMax = 4096;  // tasks one deque spill moves at most
+__device__ __forceinline__ uint64_t ovf_free_word(uint64_t pos) { return (pos & kOvfTagMask) << 32; }
+__device__ __forceinline__ uint64_t ovf_ready_word(uint64_t pos, uint32_t id) {
+    return kOvfReady | ((pos & kOvfTagMask) << 32) | (uint64_t)id;
+}
+__device__ __forceinline__ uint64_t* ovf_cell(const GtapDev& dev, uint32_t q, uint64_t pos) {
+    return dev.ovf_cells + (size_t)q * dev.ovf_cap + (pos & (uint64_t)(dev.ovf_cap - 1));
+}
+__device__ __forceinline__ uint64_t* ovf_head(const GtapDev& dev, uint32_t q) { return dev.ovf_ctl + (size_t)q * 32; }
+__device__ __forceinline__ uint64_t* ovf_tickets(const GtapDev& dev, uint32_t q) {
+    return dev.ovf_ctl + (size_t)q * 32 + 16;
+}
+
+// Positions [base, base + n) of ring q were just taken: false (after recording the error) if
+// they would come within kMargin cells of the head's lap.
+__device__ __forceinline__ bool ovf_room(const GtapDev& dev, uint32_t q, uint64_t base, uint32_t n) {
+    const uint64_t head = ld64(ovf_head(dev, q));
+    if (base + n - head > (uint64_t)(dev.ovf_cap - kMargin)) {
+        GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, -1, (int)q, (int)(base + n - head), (int)(dev.ovf_cap - kMargin));
+        return false;
+    }
+    return true;
+}
+
+// Writes task `id` (record complete) into cell `pos` of ring q. The position's previous lap
+// was claimed before ovf_room let the position out, but its consumer may still be reading
+// the cell: wait until it is free (the consumer frees it right after its read).
+__device__ __forceinline__ void ovf_fill(const GtapDev& dev, uint32_t q, uint64_t pos, uint32_t id) {
+    uint64_t* c = ovf_cell(dev, q, pos);
+    const uint64_t want = ovf_free_word(pos);
+    for (uint32_t spin = 0; ld64(c) != want; ++spin) {
+        if ((spin & 255u) == 255u && ld32(dev.ctl + CTL_ABORT * 32)) return;
+        if (spin > (1u << 22)) {
+            GTAP_DEVICE_ERROR(dev, GTAP_ERROR_WATCHDOG, (int)id, (int)q, (int)(pos & 0x7FFFFFFF), 0);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    st64(c, ovf_ready_word(pos, id));
+}
+
+// Whether any overflow ring holds tickets not yet claimed (lane 0's answer, made uniform).
+template <int Q>
+__device__ __forceinline__ bool ovf_pending(const GtapDev& dev) {
+    int32_t any = 0;
+    if (lane_id() == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (ld64(ovf_tickets(dev, (uint32_t)q)) != ld64(ovf_head(dev, (uint32_t)q))) any = 1;
+    }
+    return shfl(any, 0) != 0;
+}
+
+// Moves the oldest published tasks of this wave's deque q to ring q when the deque holds more
+// than half its capacity: one CAS on the deque word claims them from the top end, as a thief
+// would, then the wave writes their ids into ring cells. Called by the whole wave after the
+// round's publish.
+template <class P>
+__device__ __forceinline__ void ovf_spill_deque(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t q) {
+    const int lane = lane_id();
+    const uint32_t half = dev.cap / 2;
+    const uint32_t b = uniform(vld(&w->bottom[q]));
+    if (b - uniform(vld(&w->top_cache[q])) <= half) return;
+    uint64_t wv = 0;
+    if (lane == 0) wv = ld64(dq_word_ptr(dev, q, me));
+    wv = shfl(wv, 0);
+    const uint32_t top = (uint32_t)(wv >> 32), bw = (uint32_t)wv;
+    if (lane == 0) vst(&w->top_cache[q], top);
+    if ((int32_t)(bw - top) <= (int32_t)half) return;
+    uint32_t k = (bw - top) - half / 2;
+    if (k > kOvfSpillMax) k = kOvfSpillMax;
+    int ok = 0;
+    uint64_t base = 0;
+    if (lane == 0) {
+        uint64_t expected = wv;
+        ok = atomic_cas(dq_word_ptr(dev, q, me), expected, wv + ((uint64_t)k << 32)) ? 1 : 0;
+        if (ok) {
+            vst(&w->top_cache[q], top + k);
+            base = atomic_add(ovf_tickets(dev, q), (uint64_t)k);
+            if (!ovf_room(dev, q, base, k)) ok = 0;
+        }
+    }
+    ok = shfl((int32_t)ok, 0);
+    if (!ok) return;  // a thief moved the top (the next round tries again), or the ring is full
+    base = shfl(base, 0);
+    for (uint32_t j = (uint32_t)lane; j < k; j += kWave) ovf_fill(dev, q, base + j, ld32(dq_slot_ptr(dev, q, me, top + j)));
+    if (lane == 0) vst(&w->ring_spilled, vld(&w->ring_spilled) + k);
+}
+
+// Takes up to 64 ready tasks (1 on the wave queue) from the head of ring q. Returns the
+// number taken; lane i < that number receives one in tid_out.
+template <class P>
+__device__ __forceinline__ uint32_t ovf_pop(const GtapDev& dev, uint32_t q, uint32_t& tid_out, bool& idle_registered) {
+    const int lane = lane_id();
+    const uint32_t cap = ((int)q == HasWaveQueue<P>::value) ? 1u : (uint32_t)kWave;
+    uint64_t h = 0, tk = 0;
+    if (lane == 0) {
+        h = ld64(ovf_head(dev, q));
+        tk = ld64(ovf_tickets(dev, q));
+    }
+    h = shfl(h, 0);
+    tk = shfl(tk, 0);
+    if ((int64_t)(tk - h) <= 0) return 0;
+    const uint32_t avail = (tk - h) < (uint64_t)cap ? (uint32_t)(tk - h) : cap;
+    uint64_t cv = 0;
+    bool ready = false;
+    if ((uint32_t)lane < avail) {
+        cv = ld64(ovf_cell(dev, q, h + (uint32_t)lane));
+        ready = cv == ovf_ready_word(h + (uint32_t)lane, (uint32_t)cv);
+    }
+    const uint64_t notready = ~ballot(ready);
+    const uint32_t k = notready ? (uint32_t)__builtin_ctzll(notready) : 64u;  // ready run from lane 0
+    if (k == 0) return 0;
+    if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
+        if (lane == 0) atomic_add(dev.ctl + CTL_IDLE * 32, -1);
+        idle_registered = false;
+    }
+    int ok = 0;
+    if (lane == 0) {
+        uint64_t expected = h;
+        ok = atomic_cas(ovf_head(dev, q), expected, h + k) ? 1 : 0;
+    }
+    ok = shfl((int32_t)ok, 0);
+    if (!ok) return 0;
+    if ((uint32_t)lane < k) {
+        tid_out = (uint32_t)cv;
+        st64(ovf_cell(dev, q, h + (uint32_t)lane), ovf_free_word(h + (uint32_t)lane + dev.ovf_cap));
+    }
+    return k;
+}
+
 // Append `id` to the private (unpublished) end of queue `q` of this wave. Called by all
 // active lanes; lanes may target different queues. kDesc: the entry's record is held in
 // LDS (the D descriptor words `dw`), else it is already in HBM. An older private entry that
 // the append pushes out of the 64-slot window is written out: its record if it was held in
-// LDS, and its deque slot.
+// LDS, and its deque slot. `detached`: nothing joins the entry, so it may become visible at
+// once; when the deque is full and every appending lane's entry is detached, the entries go
+// to the overflow ring instead (a joined child must stay private until its parent's
+// taskwait state is written after the round).
 template <class P, bool kDesc>
 __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t q,
-                                             uint32_t id, const uint32_t* dw) {
+                                             uint32_t id, const uint32_t* dw, bool detached = false) {
     constexpr int Q = CtxShape<P>::kQ, QL = CtxShape<P>::kQL, D = CtxShape<P>::kD;
     const __amdgpu_buffer_rsrc_t rs = pool_rsrc(dev.recs);
 #pragma unroll
@@ -413,7 +574,26 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w
                 vst(&w->top_cache[qq], top);
             }
             if (t + cnt - top > dev.cap - kMargin) {
-                if (r == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, id, qq, t + cnt - top, dev.cap - kMargin);
+                if (dev.ovf_cells != nullptr && ballot(!detached) == 0) {
+                    // the records complete, then the ids into ring cells
+                    if constexpr (kDesc) {
+                        if (qq < QL) record_from_words<P>(dev, rs, id, dw);
+                    }
+                    complete_stores();
+                    const int lead = first_lane(mq);
+                    uint64_t base = 0;
+                    int32_t room = 0;
+                    if (r == 0) {
+                        base = atomic_add(ovf_tickets(dev, (uint32_t)qq), (uint64_t)cnt);
+                        room = ovf_room(dev, (uint32_t)qq, base, cnt) ? 1 : 0;
+                        vst(&w->ring_spilled, vld(&w->ring_spilled) + cnt);
+                    }
+                    base = shfl(base, lead);
+                    room = shfl(room, lead);
+                    if (room) ovf_fill(dev, (uint32_t)qq, base + r, id);
+                } else if (r == 0) {
+                    GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, id, qq, t + cnt - top, dev.cap - kMargin);
+                }
             } else {
                 const uint32_t pos = t + r;
                 const uint32_t slot = pos & (kWave - 1);
@@ -493,9 +673,9 @@ __device__ __forceinline__ uint32_t spawn_record(const GtapDev& dev, WaveCtxOf<P
         const uint32_t off = id * dev.stride;
         st128(rs, off, u32x4{d[1], d[2], 0u, d[0]});
         frame_words_st<FW>(rs, off, &d[3]);
-        queue_append<P, false>(dev, w, me, q, id, nullptr);
+        queue_append<P, false>(dev, w, me, q, id, nullptr, par == kNoParent);
     } else {
-        queue_append<P, true>(dev, w, me, q, id, d);
+        queue_append<P, true>(dev, w, me, q, id, d, par == kNoParent);
     }
     const uint64_t am = active_mask();
     if (rank_in(am) == 0) vst(&w->spawns, vld(&w->spawns) + (uint32_t)popc(am));
@@ -857,7 +1037,7 @@ __device__ __forceinline__ uint32_t run_wave_task(const GtapDev& dev, const type
 // ------------------------------------------------------------------ scheduler
 
 struct LoopStats {
-    uint64_t tasks, rounds, pops, steals, attempts, jobs, published, suspended;
+    uint64_t tasks, rounds, pops, steals, attempts, jobs, published, suspended, ring_taken;
 };
 
 template <class P>
@@ -1276,6 +1456,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         vst(&w->spawns, 0u);
         vst(&w->refills, 0u);
         vst(&w->overflowed, 0u);
+        vst(&w->ring_spilled, 0u);
         vst(&w->nstage, 0u);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -1318,10 +1499,20 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
     // last value of the shared admission counter this wave read (the jobs handed out by
     // index are taken)
     uint32_t seen_next = dev.jobs != nullptr ? dev.static_jobs : 0u;
-    LoopStats st = {0, 0, 0, 0, 0, 0, 0, 0};
+    LoopStats st = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     // timeline profiling (lane 0, in the Prof build when dev.prof_max > 0)
     uint32_t ph_idx = 0, pw_idx = 0;
     bool had_tasks = false;
+    // round-phase cycle sums (Prof build): wave-uniform, so they live in scalar registers
+    uint64_t phase_acc[PH_ROUNDS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t phase_t = Prof ? __builtin_amdgcn_s_memtime() : 0;
+    auto phase_mark = [&](int k) {
+        if constexpr (Prof) {
+            const uint64_t now = __builtin_amdgcn_s_memtime();
+            phase_acc[k] += now - phase_t;
+            phase_t = now;
+        }
+    };
 
     uint64_t host_poll_last = 0;
     const bool host_poller = dev.term_mode == GTAP_TERM_SERVE && me == 0;
@@ -1361,13 +1552,28 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                     if (n) batch_q = (int)q;
                 }
             }
+            // the overflow rings (tasks that full deques handed off), every fourth idle try
+            if (Sched != kSchedGQ && n == 0 && dev.ovf_cells != nullptr && (idle_iter & 3u) == 0) {
+#pragma unroll
+                for (int i = 0; i < Q && n == 0; ++i) {
+                    const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
+                    n = ovf_pop<P>(dev, q, my_tid, idle_registered);
+                    if (n) {
+                        batch_q = (int)q;
+                        st.ring_taken += n;
+                    }
+                }
+            }
             if (n == 0) {
                 if (Prof && had_tasks) {
                     had_tasks = false;
                     if (dev.prof_max && lane == 0) prof_stamp(dev.prof_having, nullptr, dev.prof_max, me, ph_idx, 0);
                 }
-                // ---- idle: termination, abort, watchdog, backoff
-                if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered) {
+                // ---- idle: termination, abort, watchdog, backoff. A wave counts itself idle
+                // only while no overflow ring holds tickets: a wave that handed tasks to a
+                // ring therefore sees them before it can count itself idle.
+                if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered &&
+                    !(Sched != kSchedGQ && dev.ovf_cells != nullptr && ovf_pending<Q>(dev))) {
                     idle_registered = true;
                     int32_t c = 0;
                     int32_t started = 0;
@@ -1411,10 +1617,12 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 if (flags) break;
                 const uint32_t lvl = idle_iter < 8 ? 0u : (idle_iter < 64 ? 2u : (idle_iter < 256 ? 4u : 6u));
                 backoff_sleep(lvl < dev.idle_level ? lvl : dev.idle_level);
+                phase_mark(PH_IDLE);
                 continue;
             }
             idle_iter = 0;
         }
+        phase_mark(PH_ACQUIRE);
 
         // ---- execute one round: lane i runs task my_tid when i < n
         st.rounds += 1;
@@ -1449,6 +1657,10 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             }
             if (my_lds && batch_q >= 0 && batch_q < QL) load_task_lds<P>(L, my_tid, &w->desc[batch_q < QL ? batch_q : 0][my_slot][0]);
             else load_task<P>(L, my_tid);
+            if constexpr (Prof) {
+                __asm__ volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");  // the loads' own time
+                phase_mark(PH_LOAD);
+            }
             if (my_tid == 0u) {
                 L.outcome = OUT_FINISHED;
                 L.parent = kNoParent;
@@ -1473,6 +1685,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 fin_parent = kNoParent;
             }
         }
+        phase_mark(PH_RUN);
 
         // ---- epilogue: suspended records, joins, continuations, id release, next batch, publish
         {
@@ -1503,7 +1716,9 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 if (lane == 0) vst(&w->nstage, 0u);
             }
         }
+        phase_mark(PH_STAGE);
         complete_stores();  // results, suspended records and child records of this round
+        phase_mark(PH_WAIT);
         uint32_t ready = kNoParent, ready_q = 0;
         if (fin && fin_parent != kNoParent) {
             // the parent's join word: the last child also reads the resume queue
@@ -1515,6 +1730,10 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
             } else if (left == 0u) {
                 GTAP_DEVICE_ERROR(dev, GTAP_ERROR_INVALID_TASKWAIT, fin_tid, -1, 0, 0);
             }
+        }
+        if constexpr (Prof) {
+            __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the join atomics have returned
+            phase_mark(PH_JOIN);
         }
         if (ballot(ready != kNoParent)) {
             if (ready != kNoParent && ready >= dev.nrecs) complete_job<P>(dev, ready);
@@ -1575,7 +1794,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         for (int q = 0; q < Q; ++q) {
             const uint32_t t = uniform(vld(&w->tail[q]));
             const uint32_t b = uniform(vld(&w->bottom[q]));
-            const bool retain = Sched != kSchedGQ && Q > 1 && q < QL && (uint32_t)q != kmax;
+            const bool retain = Sched != kSchedGQ && Q > 1 && q < QL && (uint32_t)q != kmax && !dev.epaq_publish;
             pub_to[q] = !retain ? t : ((int32_t)(t - b) > kWave ? t - kWave : b);
             if (retain && pub_to[q] != b) any_publish = true;
             if (!retain && t != b) {
@@ -1635,6 +1854,14 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 }
             }
         }
+        // a deque above half its capacity hands its oldest published tasks to the overflow ring
+        if constexpr (Sched != kSchedGQ) {
+            if (dev.ovf_cells != nullptr) {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) ovf_spill_deque<P>(dev, w, me, (uint32_t)q);
+            }
+        }
+        phase_mark(PH_PUBLISH);
         n = keep;
         my_tid = next_tid;
         my_slot = next_slot;
@@ -1673,6 +1900,16 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         s[ST_SUSPENDED] = st.suspended;
         s[ST_START_TICKS] = t_begin;
         s[ST_END_TICKS] = now_ticks();
+        s[ST_RING_SPILLED] = vld(&w->ring_spilled);
+        s[ST_RING_TAKEN] = st.ring_taken;
+        if constexpr (Prof) {
+            if (dev.prof_phase != nullptr) {
+                uint64_t* ph = dev.prof_phase + (size_t)me * PH_COUNT;
+#pragma unroll
+                for (int k = 0; k < PH_ROUNDS; ++k) ph[k] = phase_acc[k];
+                ph[PH_ROUNDS] = st.rounds;
+            }
+        }
     }
 }
 

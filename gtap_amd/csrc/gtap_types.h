@@ -119,7 +119,11 @@ typedef struct gtap_config {
     uint32_t scheduler;       /* thread mode: 0 or 1 = batched work stealing (default), 2 = one
                                  global queue per EPAQ queue, 3 = single-item Chase-Lev pops and
                                  steals (programs built with the alternatives: fib, n-queens) */
-    uint32_t reserved[4];
+    uint32_t epaq_publish;    /* thread mode, several EPAQ queues: 0 = the queues other than the
+                                 one the wave's next batch comes from keep their newest <= 64
+                                 private entries for the wave's later rounds (default); 1 = they
+                                 publish everything at the end of the round */
+    uint32_t reserved[3];
 } gtap_config;
 
 /* Called by gtap_initialize and gtap_reset (and at registration when the runtime is already

@@ -144,9 +144,14 @@ GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job
     d.gq_cap = R.gq_cap;
     d.gq_slots = R.gq_slots;
     d.gq_ctl = R.gq_ctl;
+    d.ovf_cells = R.ovf_cells;
+    d.ovf_ctl = R.ovf_ctl;
+    d.ovf_cap = R.ovf_cap;
+    d.epaq_publish = R.cfg.epaq_publish ? 1u : 0u;
     d.idle_level = 1;
     for (uint32_t v = R.cfg.idle_sleep; v > 1 && d.idle_level < 6; v >>= 1) d.idle_level += 1;
     d.prof_having = R.prof_having;
+    d.prof_phase = R.prof_phase;
     d.prof_working = R.prof_working;
     d.prof_counts = R.prof_counts;
     return d;
@@ -171,6 +176,8 @@ __global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0, int
     for (uint64_t i = gid; i < (uint64_t)dev.nworkers * ST_COUNT; i += gs) dev.stats[i] = 0;
     if (dev.gq_ctl)
         for (uint64_t i = gid; i < (uint64_t)dev.nqueues * 3 * 16; i += gs) dev.gq_ctl[i] = 0;
+    if (dev.prof_phase)
+        for (uint64_t i = gid; i < (uint64_t)dev.nworkers * PH_COUNT; i += gs) dev.prof_phase[i] = 0;
     if (dev.prof_max) {
         for (uint64_t i = gid; i < (uint64_t)dev.nworkers * dev.prof_max; i += gs) {
             dev.prof_having[i] = 0;
@@ -184,10 +191,37 @@ __global__ void reset_kernel(GtapDev dev, gtap_job job0, int32_t write_job0, int
         dev.pool_ctl[i] = (i == (uint64_t)dev.nshards * 32) ? 64ull + (uint64_t)dev.nworkers * dev.static_ids : 0ull;
 }
 
+// The overflow rings keep their positions from launch to launch. A launch that ended early
+// (an error, first-task termination with detached tasks still queued) can leave tickets
+// between the head and the ticket counter: their cells are freed for the next lap and the
+// head moves up to the tickets. One workgroup, so the head moves after every cell is freed.
+__global__ void __launch_bounds__(256) ovf_drain_kernel(GtapDev dev) {
+    for (uint32_t q = 0; q < dev.nqueues; ++q) {
+        const uint64_t head = ld64(dev.ovf_ctl + (size_t)q * 32), tickets = ld64(dev.ovf_ctl + (size_t)q * 32 + 16);
+        for (uint64_t p = head + threadIdx.x; p < tickets; p += blockDim.x)
+            st64(dev.ovf_cells + (size_t)q * dev.ovf_cap + (p & (dev.ovf_cap - 1)), ((p + dev.ovf_cap) & 0x3FFFFFFFull) << 32);
+        __syncthreads();
+        if (threadIdx.x == 0) st64(dev.ovf_ctl + (size_t)q * 32, tickets);
+        __syncthreads();
+    }
+}
+
+// Every cell free for its first lap (the tag of cell i is i), positions at 0.
+__global__ void ovf_init_kernel(uint64_t* cells, uint64_t* ctl, uint32_t nqueues, uint32_t cap) {
+    const uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t gs = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = gid; i < (uint64_t)nqueues * cap; i += gs) cells[i] = (uint64_t)(i & (cap - 1)) << 32;
+    for (uint64_t i = gid; i < (uint64_t)nqueues * 32; i += gs) ctl[i] = 0;
+}
+
 int enqueue_reset(const GtapDev& dev, hipStream_t stream, const gtap_job* job0, bool arm) {
     gtap_job j;
     if (job0) j = *job0;
     else memset(&j, 0, sizeof(j));
+    if (dev.ovf_cells) {
+        hipLaunchKernelGGL(ovf_drain_kernel, dim3(1), dim3(256), 0, stream, dev);
+        GTAP_HIP_TRY(hipGetLastError());
+    }
     hipLaunchKernelGGL(reset_kernel, dim3(1024), dim3(256), 0, stream, dev, j, job0 ? 1 : 0, arm ? 1 : 0);
     GTAP_HIP_TRY(hipGetLastError());
     return 0;
@@ -243,6 +277,7 @@ int enqueue_summary(const GtapDev& dev, hipStream_t stream, bool copy_job0) {
 }
 
 constexpr uint32_t kGlobalQueueSlots = 1u << 24;  // per EPAQ queue
+constexpr uint32_t kOverflowCells = 1u << 22;     // deque overflow ring cells per EPAQ queue (32 MiB)
 
 int ensure_global_queues() {
     Runtime& R = rt();
@@ -332,10 +367,13 @@ static void free_all() {
     dfree(R.jobs);
     dfree(R.stats);
     dfree(R.prof_having);
+    dfree(R.prof_phase);
     dfree(R.prof_working);
     dfree(R.prof_counts);
     dfree(R.gq_slots);
     dfree(R.gq_ctl);
+    dfree(R.ovf_cells);
+    dfree(R.ovf_ctl);
     dfree(R.job_units);
     if (R.host_ctl) (void)hipHostFree(R.host_ctl);
     if (R.report) (void)hipHostFree(R.report);
@@ -399,6 +437,7 @@ int gtap_set_config(const gtap_config* cfg) {
     Runtime& R = rt();
     if (R.initialized) return invalid_argument("gtap_set_config must precede gtap_initialize");
     if (cfg && cfg->scheduler > GTAP_SCHED_CHASE_LEV) return invalid_argument("gtap_config.scheduler must be 0..3");
+    if (cfg && cfg->epaq_publish > 1) return invalid_argument("gtap_config.epaq_publish must be 0 or 1");
     if (cfg) R.cfg = *cfg;
     else memset(&R.cfg, 0, sizeof(R.cfg));
     return 0;
@@ -458,6 +497,12 @@ int gtap_initialize(void) {
     GTAP_HIP_TRY(hipMalloc(&R.dq_word, (size_t)R.nqueues * R.max_workers * 16 * sizeof(uint64_t)));
     GTAP_HIP_TRY(hipMemset(R.dq_word, 0, (size_t)R.nqueues * R.max_workers * 16 * sizeof(uint64_t)));
     GTAP_HIP_TRY(hipMalloc(&R.dq_slots, (size_t)R.nqueues * R.max_workers * R.cap * sizeof(uint32_t)));
+    R.ovf_cap = kOverflowCells;
+    GTAP_HIP_TRY(hipMalloc(&R.ovf_cells, (size_t)R.nqueues * R.ovf_cap * sizeof(uint64_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.ovf_ctl, (size_t)R.nqueues * 32 * sizeof(uint64_t)));
+    hipLaunchKernelGGL(ovf_init_kernel, dim3(1024), dim3(256), 0, R.stream, R.ovf_cells, R.ovf_ctl, R.nqueues,
+                       R.ovf_cap);
+    GTAP_HIP_TRY(hipGetLastError());
 
     GTAP_HIP_TRY(hipMalloc(&R.ctl, (size_t)CTL_COUNT * 32 * sizeof(int32_t)));
     GTAP_HIP_TRY(hipMemset(R.ctl, 0, (size_t)CTL_COUNT * 32 * sizeof(int32_t)));
@@ -578,8 +623,10 @@ int gtap_set_profiling(int enable, uint32_t max_samples) {
     if (R.prof_having) (void)hipFree(R.prof_having);
     if (R.prof_working) (void)hipFree(R.prof_working);
     if (R.prof_counts) (void)hipFree(R.prof_counts);
+    if (R.prof_phase) (void)hipFree(R.prof_phase);
     R.prof_having = R.prof_working = nullptr;
     R.prof_counts = nullptr;
+    R.prof_phase = nullptr;
     R.prof_max = 0;
     if (!enable) return 0;
     if (max_samples == 0) max_samples = 4096;
@@ -588,7 +635,24 @@ int gtap_set_profiling(int enable, uint32_t max_samples) {
     GTAP_HIP_TRY(hipMalloc(&R.prof_having, n * sizeof(uint64_t)));
     GTAP_HIP_TRY(hipMalloc(&R.prof_working, n * sizeof(uint64_t)));
     GTAP_HIP_TRY(hipMalloc(&R.prof_counts, n * sizeof(int32_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.prof_phase, (size_t)R.max_workers * PH_COUNT * sizeof(uint64_t)));
+    GTAP_HIP_TRY(hipMemset(R.prof_phase, 0, (size_t)R.max_workers * PH_COUNT * sizeof(uint64_t)));
     R.prof_max = max_samples;
+    return 0;
+}
+
+// Shader-clock cycles of the last thread-mode launch's round phases, summed over its workers
+// (out[PH_ACQUIRE .. PH_IDLE]), and its rounds (out[PH_ROUNDS]); needs timeline profiling on
+// (gtap_set_profiling) for that launch, else everything reads 0.
+int gtap_get_round_phases(uint64_t* out, uint32_t n) {
+    Runtime& R = rt();
+    if (!out || n < (uint32_t)PH_ROUNDS + 1) return invalid_argument("gtap_get_round_phases: out needs 9 entries");
+    for (uint32_t k = 0; k < n; ++k) out[k] = 0;
+    if (!R.initialized || !R.prof_phase || !R.last_nworkers) return 0;
+    std::vector<uint64_t> h((size_t)R.last_nworkers * PH_COUNT);
+    GTAP_HIP_TRY(hipMemcpy(h.data(), R.prof_phase, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    for (uint32_t w = 0; w < R.last_nworkers; ++w)
+        for (int k = 0; k <= PH_ROUNDS; ++k) out[k] += h[(size_t)w * PH_COUNT + k];
     return 0;
 }
 

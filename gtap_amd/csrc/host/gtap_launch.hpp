@@ -83,12 +83,17 @@ struct Runtime {
     // global-queue scheduler (allocated on first use)
     uint32_t* gq_slots = nullptr;
     uint64_t* gq_ctl = nullptr;
+    // deque overflow rings (work stealing): [queue][ovf_cap] cells, head/ticket words
+    uint64_t* ovf_cells = nullptr;
+    uint64_t* ovf_ctl = nullptr;
+    uint32_t ovf_cap = 0;
     uint32_t gq_cap = 0;
     // timeline profiling (gtap_set_profiling)
     uint32_t prof_max = 0;
     uint64_t* prof_having = nullptr;
     uint64_t* prof_working = nullptr;
     int32_t* prof_counts = nullptr;
+    uint64_t* prof_phase = nullptr;  // [worker][PH_COUNT] round-phase cycles (thread-mode timeline build)
     uint32_t last_nworkers = 0;
     gtap_run_stats last{};
     int last_error_code = 0;

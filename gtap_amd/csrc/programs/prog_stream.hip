@@ -37,7 +37,8 @@ namespace stream {
 enum : uint32_t { F_FIB = 1, F_NQ = 2, F_SROOT = 3, F_MSORT = 4, F_MERGE = 5, F_CSUM = 6 };
 enum : uint32_t { Q_LANE = 0, Q_WAVE = 1 };
 constexpr int kWaveLeaf = 1024;       // keys sorted by one wave in LDS
-constexpr int kWaveMergeLeaf = 4096;  // keys merged by one wave
+constexpr int kWaveMergeLeaf = 4096;  // default: keys merged by one wave (the tile loop takes up to 16384)
+constexpr int kMaxWaveMergeLeaf = 16384;
 
 struct FibArg {
     int32_t n;
@@ -89,7 +90,7 @@ __host__ __device__ __forceinline__ uint64_t sort_mix(uint64_t pos, uint32_t k) 
 }
 
 struct StreamArgs {
-    int32_t unused;
+    int32_t merge_leaf;  // merges of at most this many keys run on one wave (1024..16384)
 };
 
 struct Program {
@@ -264,7 +265,7 @@ __device__ __forceinline__ int32_t merge_path32(LDA lda, int32_t na, LDB ldb, in
     return lo;
 }
 
-// Merge a[0..na) and b[0..nb) into dst (na + nb <= kWaveMergeLeaf) by tiles of 1024
+// Merge a[0..na) and b[0..nb) into dst (na + nb <= kMaxWaveMergeLeaf) by tiles of 1024
 // outputs, 16 per lane, staged through the wave's LDS tile.
 template <class W>
 __device__ void wave_leaf_merge(W& wt, const uint32_t* a, int32_t na, const uint32_t* b, int32_t nb, uint32_t* dst) {
@@ -276,6 +277,7 @@ __device__ void wave_leaf_merge(W& wt, const uint32_t* a, int32_t na, const uint
     constexpr int32_t kTileOut = kWaveLeaf;
     constexpr int32_t kPer = kTileOut / kWave;  // 16
     const int32_t ntiles = (total + kTileOut - 1) / kTileOut;  // <= 16
+    static_assert(kMaxWaveMergeLeaf / kTileOut + 1 <= (int)(sizeof(sh->split) / sizeof(sh->split[0])), "split table");
     if (l <= ntiles) {
         const int32_t d = (l * kTileOut < total) ? l * kTileOut : total;
         sh->split[l] = merge_path32(ga, na, gb, nb, d);
@@ -355,7 +357,7 @@ __device__ void msort_task(W& wt) {
             [[fallthrough]];
         case 1: {
             const uint32_t* src = into_t ? a : t;  // where the sorted halves are
-            if (n <= kWaveMergeLeaf) {
+            if (n <= wt.args->merge_leaf) {
                 wave_leaf_merge(wt, src, (int32_t)h, src + h, (int32_t)(n - h), dst);
                 wt.finish();
                 return;
@@ -394,7 +396,7 @@ __device__ void merge_task(W& wt) {
         wt.finish();
         return;
     }
-    if (na + nb <= kWaveMergeLeaf) {
+    if (na + nb <= wt.args->merge_leaf) {
         wave_leaf_merge(wt, a, (int32_t)na, b, (int32_t)nb, dst);
         wt.finish();
         return;
@@ -456,8 +458,8 @@ uint64_t splitmix64_next(uint64_t& s) {
 // Task ids a job may hold at once, from its kind and size: at most the job's whole task
 // count (a record is live from its spawn to its completion). fib(n) runs 2 F(n+1) - 1 tasks;
 // n-queens(N) one per partial placement (counted here once per N); a sort job's msort tree
-// (leaves of 1024 keys), its merge tasks (split down to 4096 keys, at most about four per
-// 4096 outputs per level), its checksum tasks and its root.
+// (leaves of 1024 keys), its merge tasks (split down to merge_leaf keys, at most about four per
+// merge_leaf outputs per level), its checksum tasks and its root.
 uint64_t fib_tasks_host(int n) {
     uint64_t a = 0, b = 1;  // F(0), F(1)
     for (int i = 0; i < n + 1; ++i) {
@@ -484,11 +486,11 @@ uint64_t nq_tasks_host(int n) {
     if (!memo[n]) memo[n] = nq_nodes_rec(n, 0, 0u, 0u, 0u);
     return memo[n];
 }
-uint64_t sort_tasks_host(int64_t n) {
+uint64_t sort_tasks_host(int64_t n, int64_t merge_leaf) {
     const uint64_t leaves = (uint64_t)((n + stream::kWaveLeaf - 1) / stream::kWaveLeaf);
     uint64_t levels = 1;
     while ((1ull << levels) < leaves + 1) ++levels;
-    const uint64_t merge_per_level = 4 * (uint64_t)((n + stream::kWaveMergeLeaf - 1) / stream::kWaveMergeLeaf);
+    const uint64_t merge_per_level = 4 * (uint64_t)((n + merge_leaf - 1) / merge_leaf);
     return 1 + 2 * leaves + levels * merge_per_level + (uint64_t)((n + stream::kCsumChunk - 1) / stream::kCsumChunk);
 }
 
@@ -544,6 +546,7 @@ struct gtap_stream {
     gtap_job* d_table = nullptr;    // the same, device-resident (source of the batched graph's copy)
     host::ThreadGraph graph;        // batched launch, captured on first use
     std::vector<double> exec_ms;    // last run: device time from admission to completion per job
+    int32_t merge_leaf = stream::kWaveMergeLeaf;  // sort jobs: merges of at most this many keys on one wave
 };
 
 extern "C" int gtap_stream_destroy(gtap_stream* S);
@@ -591,7 +594,8 @@ extern "C" int gtap_stream_create(uint64_t seed, int32_t njobs, gtap_stream** ou
     S->tasks.resize(njobs);
     for (int32_t i = 0; i < njobs; ++i) {
         S->tasks[i] = S->kind[i] == 0 ? fib_tasks_host(S->param[i])
-                                       : (S->kind[i] == 1 ? nq_tasks_host(S->param[i]) : sort_tasks_host(S->param[i]));
+                                       : (S->kind[i] == 1 ? nq_tasks_host(S->param[i])
+                                                          : sort_tasks_host(S->param[i], S->merge_leaf));
         gtap_job j;
         memset(&j, 0, sizeof(j));
         j.status = GTAP_JOB_READY;
@@ -641,6 +645,21 @@ extern "C" int gtap_stream_info(gtap_stream* S, int32_t* kind, int32_t* param, u
     return 0;
 }
 
+// Sort jobs merge runs of at most merge_leaf keys on one wave and split larger merges into
+// merge tasks (the stream's counterpart of the reference's merge cutoff,
+// evaluation/2-comparison/cilksort/gtap_cilksort.cu:12, 112). Changing it re-captures the batched
+// graph and re-derives each sort job's reservation.
+extern "C" int gtap_stream_set_merge_leaf(gtap_stream* S, int32_t merge_leaf) {
+    if (!S || merge_leaf < stream::kWaveLeaf || merge_leaf > stream::kMaxWaveMergeLeaf)
+        return host::invalid_argument("gtap_stream_set_merge_leaf: merge_leaf must be in [1024, 16384]");
+    if (merge_leaf == S->merge_leaf) return 0;
+    host::destroy_graph(S->graph);
+    S->merge_leaf = merge_leaf;
+    for (int32_t i = 0; i < S->njobs; ++i)
+        if (S->kind[i] == 2) S->tasks[i] = sort_tasks_host(S->param[i], merge_leaf);
+    return 0;
+}
+
 // Copies the (sorted, after a run) keys of sort job `job` to out.
 extern "C" int gtap_stream_sorted_keys(gtap_stream* S, int32_t job, uint32_t* out) {
     if (!S || job < 0 || job >= S->njobs || S->kind[job] != 2 || !out) return host::invalid_argument("gtap_stream_sorted_keys");
@@ -666,7 +685,7 @@ extern "C" int gtap_stream_run_batched(gtap_stream* S, int64_t* results, double*
     host::Runtime& R = host::rt();
     int st = stream_reset_keys(S);
     if (st) return st;
-    st = host::capture_thread<StreamProgram>(S->graph, StreamProgram::Args{0}, GTAP_TERM_JOBS, (uint32_t)S->njobs,
+    st = host::capture_thread<StreamProgram>(S->graph, StreamProgram::Args{S->merge_leaf}, GTAP_TERM_JOBS, (uint32_t)S->njobs,
                                              S->d_table, (uint32_t)S->njobs);
     if (st) return st;
     st = host::replay_thread(S->graph, stats);
@@ -695,7 +714,7 @@ extern "C" int gtap_stream_run_per_job(gtap_stream* S, int64_t* results, double*
     for (int32_t i = 0; i < S->njobs; ++i) {
         int64_t r = 0;
         gtap_job out;
-        st = host::run_single_job<StreamProgram>(StreamProgram::Args{0}, S->table[i], &r, nullptr, GTAP_TERM_JOBS, &out);
+        st = host::run_single_job<StreamProgram>(StreamProgram::Args{S->merge_leaf}, S->table[i], &r, nullptr, GTAP_TERM_JOBS, &out);
         if (st) return st;
         S->exec_ms[i] = (double)(out.t_end - out.t_start) * 1e-5;
         if (results) results[i] = r;
@@ -746,7 +765,7 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
             while (host_now_ns() - t0 < arrival[i]) std::this_thread::yield();
             int64_t r = 0;
             gtap_job out;
-            st = host::run_single_job<StreamProgram>(StreamProgram::Args{0}, S->table[i], &r, nullptr, GTAP_TERM_JOBS, &out);
+            st = host::run_single_job<StreamProgram>(StreamProgram::Args{S->merge_leaf}, S->table[i], &r, nullptr, GTAP_TERM_JOBS, &out);
             if (st) return st;
             S->exec_ms[i] = (double)(out.t_end - out.t_start) * 1e-5;
             res[i] = r;
@@ -812,10 +831,10 @@ extern "C" int gtap_stream_serve(gtap_stream* S, double rate, uint64_t arrival_s
         GTAP_HIP_TRY(hipEventRecord(R.ev0, R.stream));
         if (prof)
             hipLaunchKernelGGL((thread_kernel<StreamProgram, kSchedWS, true>), dim3(blocks), dim3(256), 0, R.stream, dev,
-                               StreamProgram::Args{0});
+                               StreamProgram::Args{S->merge_leaf});
         else
             hipLaunchKernelGGL((thread_kernel<StreamProgram, kSchedWS, false>), dim3(blocks), dim3(256), 0, R.stream, dev,
-                               StreamProgram::Args{0});
+                               StreamProgram::Args{S->merge_leaf});
         GTAP_HIP_TRY(hipGetLastError());
         GTAP_HIP_TRY(hipEventRecord(R.ev1, R.stream));
         st = host::enqueue_summary(dev, R.stream, false);

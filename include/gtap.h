@@ -69,10 +69,22 @@ int gtap_get_runtime_error_report(gtap_error_report* out);
  * same columns) for the last launch when timeline profiling is on, and always
  * <app>_warp_counters.csv with the scheduler counters per worker. */
 void gtap_visualize_profile(const char* app_name);
+/* The two views on their own, each printing its text timeline of the first ten workers and
+ * writing its two CSV files: replace visualize_having_task_time and visualize_working_time
+ * (runtime/detail/thread/gtap_profile_thread.cuh:110, 270), which applications call directly
+ * (examples/cilksort/cilksort.cu:262). */
+void gtap_visualize_having_task_time(const char* app_name);
+void gtap_visualize_working_time(const char* app_name);
 /* Timeline profiling on/off for the following launches, up to max_samples stamps per worker
  * and kind (0 -> 4096); replaces the PROFILE / MAX_PROFILE_DATA compile-time switches
  * (runtime/detail/common/gtap_config.cuh:22-26). */
 int gtap_set_profiling(int enable, uint32_t max_samples);
+/* Shader-clock cycles of the last thread-mode launch's round phases, summed over its workers:
+ * out[0] acquire, [1] record loads, [2] task bodies, [3] record writes issued, [4] their
+ * completion wait, [5] join atomics, [6] continuations and publish, [7] idle tries, and
+ * out[8] rounds. Needs timeline profiling on for that launch (the phase stamps are in the
+ * profiling build only); n >= 9. No reference counterpart. */
+int gtap_get_round_phases(uint64_t* out, uint32_t n);
 /* Statistics of the last persistent launch (no reference counterpart; the reference
  * prints "Execution time" around its launch, examples/fib/fib.cu:36-51). */
 int gtap_get_last_stats(gtap_run_stats* out);
@@ -88,6 +100,12 @@ int gtap_device_info(char* name, int name_len, int* num_cus, uint64_t* hbm_bytes
  * fetch-add throughput on distinct 64-byte words (Gop/s). Best of `reps` launches. */
 int gtap_bench_hbm_copy(uint64_t bytes, int reps, double* gbps);
 int gtap_bench_atomics(int reps, double* gops);
+/* The same with every lane adding to one of `words` words (contended when words is small). */
+int gtap_bench_atomics_contended(int reps, uint32_t words, double* gops);
+/* Random 4-byte accesses over a table of table_bytes: out[0] plain loads, out[1] agent-scope
+ * fetch-min, in G accesses/s (the request-rate ceiling of the BFS expansion). No reference
+ * counterpart. */
+int gtap_bench_random_access(uint64_t table_bytes, int reps, double* out);
 /* Time per publication per wave (ns) with `waves` concurrent waves: out_ns[0] an agent-scope
  * release fence, out_ns[1] an agent-scope acquire fence, out_ns[2] a store-completion wait
  * (the order this runtime publishes with). No reference counterpart; a measurement. */
@@ -173,6 +191,11 @@ int gtap_stream_serve(gtap_stream* stream, double rate, uint64_t arrival_seed, i
 /* Device time from admission to completion of each job of the last run (any of the three
  * forms above), in ms: the execution part of latency_ms. */
 int gtap_stream_exec_ms(gtap_stream* stream, double* exec_ms);
+/* Sort jobs merge runs of at most merge_leaf keys (1024..16384, default 4096) on one wave and
+ * split larger merges into merge tasks; the counterpart of TASK_SPAWN_CUTOFF_MERGE
+ * (evaluation/2-comparison/cilksort/gtap_cilksort.cu:12, 112), as a run-time setting for the
+ * autotuner. Results do not depend on it. */
+int gtap_stream_set_merge_leaf(gtap_stream* stream, int32_t merge_leaf);
 
 /* n-queens at the reference's benchmark settings (evaluation/2-comparison/nq/gtap_nq.cu:10-99):
  * tasks for rows 0..cutoff (the reference uses 7), a sequential search below, detached
@@ -240,5 +263,9 @@ int gtap_gen_keys(uint64_t seed, uint64_t n, uint32_t* out);
 
 #ifdef __cplusplus
 }
+/* The reference's unprefixed C++ spellings, so a host program that calls them compiles as is
+ * (examples/cilksort/cilksort.cu:262). */
+static inline void visualize_having_task_time(const char* app_name) { gtap_visualize_having_task_time(app_name); }
+static inline void visualize_working_time(const char* app_name) { gtap_visualize_working_time(app_name); }
 #endif
 #endif

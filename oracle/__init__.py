@@ -73,6 +73,8 @@ def lib() -> ctypes.CDLL:
         L.gtap_oracle_hash_u32.restype = u64
         L.gtap_oracle_sort_checksum.argtypes = [vp, i64]
         L.gtap_oracle_sort_checksum.restype = u64
+        L.gtap_oracle_stream_cpu.argtypes = [u64, i32, i32, vp]
+        L.gtap_oracle_stream_cpu.restype = ctypes.c_double
         L.gtap_oracle_spmv.argtypes = [i32, vp, vp, vp, vp, vp]
         L.gtap_oracle_tree.argtypes = [i32, i32, i32, vp, i64, vp]
         L.gtap_oracle_tree_node.argtypes = [i32, i32, i32, vp, i64]
@@ -196,6 +198,14 @@ def sort_checksum(a: np.ndarray) -> int:
     a = np.ascontiguousarray(a, dtype=np.uint32)
     v = int(lib().gtap_oracle_sort_checksum(_p(a) if a.size else None, a.size))
     return v - (1 << 64) if v >= 1 << 63 else v
+
+
+def stream_cpu(seed: int, njobs: int, parallel: bool) -> tuple[float, np.ndarray]:
+    """The whole job stream on this host's CPU (sequential, or OpenMP parallel-for over the
+    jobs): (wall seconds, results)."""
+    res = np.empty(njobs, dtype=np.int64)
+    dt = lib().gtap_oracle_stream_cpu(seed, njobs, 1 if parallel else 0, _p(res))
+    return float(dt), res
 
 
 def job_result(kind: int, param: int, key_seed: int) -> int:

@@ -444,3 +444,46 @@ void gtap_oracle_tree(int32_t height, int32_t mem_ops, int32_t compute_iters, co
 #pragma omp parallel for schedule(static)
     for (int64_t v = 0; v < nodes; ++v) out[v] = gtap_oracle_tree_node((int32_t)v, mem_ops, compute_iters, input, input_n);
 }
+
+/* ---------------------------------------------------------------- stream CPU baseline */
+
+/* The mixed job stream on the CPU: every job of gtap_oracle_jobs(seed, njobs) computed by the
+ * sequential programs (fib: seq_fib.c:5-10; n-queens: seq_nq.c:12-33; sort: the job's keys,
+ * seq_mergesort.cpp:38-62 as gtap_oracle_mergesort_u32, then the sort checksum), one job per
+ * loop iteration; `parallel` runs the loop as `#pragma omp parallel for schedule(dynamic)`
+ * over the jobs (evaluation/2-comparison/mergesort/omp_mergesort.cpp:38-56 parallelizes
+ * inside one sort; a stream of independent jobs parallelizes over the jobs). results[i]
+ * receives job i's result; returns the wall time of the loop in seconds (omp_get_wtime). */
+double gtap_oracle_stream_cpu(uint64_t seed, int32_t njobs, int32_t parallel, int64_t* results) {
+    int32_t* kind = (int32_t*)malloc(sizeof(int32_t) * (size_t)(njobs > 0 ? njobs : 1));
+    int32_t* param = (int32_t*)malloc(sizeof(int32_t) * (size_t)(njobs > 0 ? njobs : 1));
+    uint64_t* kseed = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(njobs > 0 ? njobs : 1));
+    gtap_oracle_jobs(seed, njobs, kind, param, kseed);
+    const double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic) if (parallel)
+    for (int32_t i = 0; i < njobs; ++i) {
+        int64_t r = 0;
+        if (kind[i] == 0) {
+            r = gtap_oracle_fib(param[i]);
+        } else if (kind[i] == 1) {
+            int64_t s = 0, t = 0;
+            gtap_oracle_nq(param[i], &s, &t);
+            r = s;
+        } else {
+            const int64_t n = param[i];
+            uint32_t* a = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+            uint32_t* b = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+            gtap_oracle_gen_keys_u32(kseed[i], (size_t)n, a);
+            gtap_oracle_mergesort_u32(a, b, n);
+            r = (int64_t)gtap_oracle_sort_checksum(a, n);
+            free(a);
+            free(b);
+        }
+        results[i] = r;
+    }
+    const double dt = omp_get_wtime() - t0;
+    free(kind);
+    free(param);
+    free(kseed);
+    return dt;
+}

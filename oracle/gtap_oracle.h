@@ -104,6 +104,9 @@ uint64_t gtap_oracle_hash_u32(const uint32_t* a, int64_t n);
  * position, and the terms add in any order (the device sums them in parallel). Pinned: a
  * numpy restatement, and changes under a transposition or one flipped bit. */
 uint64_t gtap_oracle_sort_checksum(const uint32_t* a, int64_t n);
+/* The whole job stream on the CPU (sequential, or OpenMP parallel-for over the jobs); job
+ * results into results[], returns the loop's wall seconds. */
+double gtap_oracle_stream_cpu(uint64_t seed, int32_t njobs, int32_t parallel, int64_t* results);
 
 /* SpMV: examples/spmv/spmv_thread.cu:34-46 (one row = sum over the row's entries in column
  * order), with the product-add written as fma(val, x, sum) so the result is one bit pattern.

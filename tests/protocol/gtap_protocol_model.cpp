@@ -15,6 +15,17 @@
 //          done is never set while a task is outstanding, also when the job owner starts
 //          after every other worker went idle; the former rule (owners count themselves when
 //          they start) is shown to end early in that schedule.
+//  ovf     the deque overflow rings (gtap_thread.hpp ovf_fill / ovf_pop / ovf_spill_deque,
+//          ovf_drain_kernel in gtap_host.hip): cells of state | position tag | id, a ticket
+//          fetch-add per batch, producers wait for the cell of the previous lap to be freed,
+//          consumers CAS the head over a run of READY cells and free them for the next lap;
+//          the drain frees what an aborted launch left between head and tickets. Invariant:
+//          every id once, over many laps of a small ring, and a ring that works after a drain.
+//  ovfq    quiescence with the rings: a worker hands tasks to a ring and counts itself idle
+//          only while no ring holds tickets. Invariant: done is never set while a task is
+//          outstanding (in a worker or a ring), however rarely idle workers poll the ring;
+//          without the rule, a worker that counts itself idle before its next poll ends the
+//          launch early.
 //  join    the additive join word (gtap_dev.hpp: count [0..11] | resume queue [12..15] |
 //          signed sum [16..63]): children add (value << 16) - 1 concurrently. Invariant: the
 //          last child alone sees count 1, the queue bits survive, the final sum is exact at
@@ -236,6 +247,134 @@ void run(unsigned seed, int producers, int consumers, uint32_t cells_each, uint3
 
 }  // namespace ring_model
 
+// ------------------------------------------------------------------ deque overflow ring
+namespace ovf_model {
+
+constexpr uint64_t kReady = 2ull << 62;
+constexpr uint64_t kTag = 0x3FFFFFFFull;
+uint64_t free_word(uint64_t pos) { return (pos & kTag) << 32; }
+uint64_t ready_word(uint64_t pos, uint32_t id) { return kReady | ((pos & kTag) << 32) | id; }
+
+struct Ring {
+    const uint32_t cap;
+    std::vector<std::atomic<uint64_t>> cells;
+    std::atomic<uint64_t> head{0}, tickets{0};
+    explicit Ring(uint32_t c) : cap(c), cells(c) {
+        for (uint32_t i = 0; i < c; ++i) cells[i].store(free_word(i));
+    }
+    std::atomic<uint64_t>& cell(uint64_t pos) { return cells[pos & (cap - 1)]; }
+    // ovf_fill: wait until the previous lap's consumer freed the cell, then publish
+    void fill(uint64_t pos, uint32_t id) {
+        while (cell(pos).load(std::memory_order_acquire) != free_word(pos)) std::this_thread::yield();
+        cell(pos).store(ready_word(pos, id), std::memory_order_release);
+    }
+    // the producers' room check (ovf_room; kMargin = 0 here)
+    bool room(uint64_t base, uint32_t n) { return base + n - head.load() <= cap; }
+    // ovf_pop with `lanes` lanes: the ready run at the head, one CAS
+    uint32_t pop(uint32_t lanes, uint32_t* out) {
+        const uint64_t h = head.load(), tk = tickets.load();
+        if ((int64_t)(tk - h) <= 0) return 0;
+        const uint32_t avail = (uint32_t)std::min<uint64_t>(tk - h, lanes);
+        uint64_t cv[64];
+        uint32_t k = 0;
+        while (k < avail) {
+            cv[k] = cell(h + k).load(std::memory_order_acquire);
+            if (cv[k] != ready_word(h + k, (uint32_t)cv[k])) break;
+            ++k;
+        }
+        if (k == 0) return 0;
+        uint64_t expected = h;
+        if (!head.compare_exchange_strong(expected, h + k, std::memory_order_acq_rel)) return 0;
+        for (uint32_t i = 0; i < k; ++i) {
+            out[i] = (uint32_t)cv[i];
+            cell(h + i).store(free_word(h + i + cap), std::memory_order_release);
+        }
+        return k;
+    }
+    // ovf_drain_kernel, with no launch running
+    void drain() {
+        const uint64_t h = head.load(), tk = tickets.load();
+        for (uint64_t p = h; p < tk; ++p) cell(p).store(free_word(p + cap));
+        head.store(tk);
+    }
+};
+
+// producers push batches of up to 16 ids (throttled to keep half the ring free, where the
+// runtime sizes the ring instead), consumers pop with 1 to 8 lanes; the ring laps many times
+void run(unsigned seed, int producers, int consumers, uint32_t per_producer, uint32_t cap) {
+    Ring ring(cap);
+    const uint32_t total = (uint32_t)producers * per_producer;
+    std::vector<std::atomic<uint8_t>> seen(total);
+    for (auto& x : seen) x.store(0);
+    std::atomic<uint32_t> got{0};
+    std::vector<std::thread> th;
+    for (int p = 0; p < producers; ++p)
+        th.emplace_back([&, p] {
+            std::mt19937 rng(seed * 31 + p);
+            uint32_t next = 0;
+            while (next < per_producer) {
+                const uint32_t k = std::min<uint32_t>(1 + rng() % 16, per_producer - next);
+                if (ring.tickets.load() - ring.head.load() + k > cap / 2) {
+                    std::this_thread::yield();
+                    continue;
+                }
+                const uint64_t base = ring.tickets.fetch_add(k);
+                CHECK(ring.room(base, k), "ovf: positions beyond the head's lap");
+                for (uint32_t j = 0; j < k; ++j) ring.fill(base + j, (uint32_t)p * per_producer + next + j);
+                next += k;
+            }
+        });
+    for (int c = 0; c < consumers; ++c)
+        th.emplace_back([&, c] {
+            std::mt19937 rng(seed * 17 + c);
+            uint32_t out[64];
+            while (got.load() < total) {
+                const uint32_t k = ring.pop(1 + rng() % 8, out);
+                for (uint32_t i = 0; i < k; ++i) {
+                    CHECK(out[i] < total, "ovf: id %u out of range", out[i]);
+                    if (out[i] < total) CHECK(seen[out[i]].fetch_add(1) == 0, "ovf: id %u twice", out[i]);
+                }
+                got.fetch_add(k);
+                if (k == 0) std::this_thread::yield();
+            }
+        });
+    for (auto& t : th) t.join();
+    uint32_t missing = 0;
+    for (uint32_t i = 0; i < total; ++i) missing += seen[i].load() == 0;
+    CHECK(missing == 0, "ovf: %u ids lost", missing);
+    CHECK(ring.head.load() == ring.tickets.load(), "ovf: head %llu, tickets %llu", (unsigned long long)ring.head.load(),
+          (unsigned long long)ring.tickets.load());
+
+    // an aborted launch: tickets taken and some cells filled, none claimed; the drain frees
+    // them, and the next launch's pushes and pops over the same cells see every id once
+    const uint64_t base = ring.tickets.fetch_add(cap / 2);
+    for (uint32_t j = 0; j < cap / 4; ++j) ring.fill(base + j, 7);
+    ring.drain();
+    CHECK(ring.head.load() == ring.tickets.load(), "ovf: drain left tickets");
+    const uint32_t again = 3 * cap;
+    std::vector<uint8_t> seen2(again, 0);
+    uint32_t got2 = 0, out[64];
+    for (uint32_t i = 0; i < again; ++i) {
+        const uint64_t b = ring.tickets.fetch_add(1);
+        CHECK(ring.room(b, 1), "ovf: no room after the drain");
+        ring.fill(b, i);
+        if (i % 3 == 2)
+            for (uint32_t k; (k = ring.pop(8, out)) != 0;) {
+                for (uint32_t x = 0; x < k; ++x) seen2[out[x]] += 1;
+                got2 += k;
+            }
+    }
+    for (uint32_t k; (k = ring.pop(8, out)) != 0;) {
+        for (uint32_t x = 0; x < k; ++x) seen2[out[x]] += 1;
+        got2 += k;
+    }
+    uint32_t bad = 0;
+    for (uint32_t i = 0; i < again; ++i) bad += seen2[i] != 1;
+    CHECK(got2 == again && bad == 0, "ovf: after the drain %u of %u ids, %u wrong", got2, again, bad);
+}
+
+}  // namespace ovf_model
+
 // ------------------------------------------------------------------ quiescence
 namespace quiesce_model {
 
@@ -301,6 +440,75 @@ bool run(unsigned seed, int workers, int owners, int work, bool count_owners_at_
 
 }  // namespace quiesce_model
 
+// ------------------------------------------------------------------ quiescence with the rings
+namespace ovfq_model {
+
+// Workers run tasks; a busy worker sometimes hands half of what it holds to the ring (ticket
+// fetch-add, then each cell made ready later, as ovf_spill_deque does). An idle worker polls
+// the ring on every fourth idle try, starting with its second (leaving the idle count before
+// it takes), and counts itself idle on its first idle try; with `ring_rule` only while
+// tickets == head. Returns true if done was set while tasks were outstanding.
+bool run(unsigned seed, int workers, int work, bool ring_rule) {
+    const uint32_t cap = 1u << 16;
+    std::vector<std::atomic<uint8_t>> ready(cap);  // cell p holds a task (no laps here)
+    for (auto& r : ready) r.store(0);
+    std::atomic<uint64_t> head{0}, tickets{0};
+    std::atomic<int> idle{0}, started{1}, done{0}, outstanding{work};
+    std::atomic<bool> early{false};
+    std::vector<std::thread> th;
+    for (int w = 0; w < workers; ++w)
+        th.emplace_back([&, w] {
+            std::mt19937 rng(seed * 13 + w);
+            if (w != 0) started.fetch_add(1);  // worker 0 owns the job, counted at the reset
+            int mine = w == 0 ? work : 0;
+            bool registered = false;
+            uint32_t idle_it = 0;
+            for (long it = 0; it < 4000000; ++it) {
+                if (mine > 0) {
+                    idle_it = 0;
+                    if (mine > 2 && rng() % 8 == 0) {
+                        const int give = mine / 2;
+                        mine -= give;
+                        const uint64_t base = tickets.fetch_add((uint64_t)give);
+                        for (int j = 0; j < give; ++j) {
+                            if (rng() % 64 == 0) std::this_thread::yield();  // a cell made ready late
+                            ready[(base + j) & (cap - 1)].store(1, std::memory_order_release);
+                        }
+                    }
+                    mine -= 1;
+                    outstanding.fetch_sub(1);
+                    continue;
+                }
+                const uint64_t h = head.load(), tk = tickets.load();
+                if (idle_it++ % 4 == 1 && tk > h && ready[h & (cap - 1)].load(std::memory_order_acquire)) {
+                    if (registered) {
+                        idle.fetch_sub(1);
+                        registered = false;
+                    }
+                    uint64_t expected = h;
+                    if (head.compare_exchange_strong(expected, h + 1, std::memory_order_acq_rel)) {
+                        ready[h & (cap - 1)].store(0);
+                        mine = 1;
+                    }
+                    continue;
+                }
+                if (!registered && (!ring_rule || tickets.load() == head.load())) {
+                    registered = true;
+                    if (idle.fetch_add(1) + 1 == started.load()) {
+                        if (outstanding.load() != 0) early.store(true);
+                        done.store(1);
+                    }
+                }
+                if (done.load()) break;
+                std::this_thread::yield();
+            }
+        });
+    for (auto& t : th) t.join();
+    return early.load();
+}
+
+}  // namespace ovfq_model
+
 // ------------------------------------------------------------------ additive join
 namespace join_model {
 
@@ -355,6 +563,16 @@ int main(int argc, char** argv) {
         const bool early_late = quiesce_model::run(s, 6, 2, 2000, true, 20000);
         CHECK(!early_late, "quiesce: done while a late job owner's tasks were outstanding (seed %u)", s);
     }
+    for (unsigned s = 1; s <= seeds; ++s) {
+        ovf_model::run(s, 3, 3, 3000, 64);
+        ovf_model::run(s + 50, 2, 4, 2000, 256);
+        CHECK(!ovfq_model::run(s, 5, 3000, true), "ovfq: done while tasks were outstanding (seed %u)", s);
+    }
+    // without the ring rule a worker can count itself idle while the tasks it handed to the
+    // ring are not ready yet, and the last worker to go idle ends the launch early
+    bool ring_rule_needed = false;
+    for (unsigned s = 200; s < 240 && !ring_rule_needed; ++s) ring_rule_needed = ovfq_model::run(s, 5, 3000, false);
+    CHECK(ring_rule_needed, "ovfq: no schedule ended early without the ring rule");
     // the former rule: a job owner that starts after the others went idle is not counted,
     // so they declare quiescence with its whole job outstanding
     const bool old_rule_early = quiesce_model::run(99, 6, 1, 2000, false, 50000);

@@ -60,8 +60,11 @@ def test_config_fields_and_scheduler_check_without_gpu():
     if not os.path.exists(gtap_amd.LIB_PATH):
         pytest.skip("libgtap_amd.so not built")
     names = [f for f, _ in gtap_amd.Config._fields_]
-    assert names[-3:] == ["idle_sleep", "scheduler", "reserved"]
+    assert names[-4:] == ["idle_sleep", "scheduler", "epaq_publish", "reserved"]
     assert gtap_amd.gtap_set_config(scheduler=9) != 0
+    assert gtap_amd.gtap_set_config(epaq_publish=2) != 0
+    assert gtap_amd.gtap_set_config(epaq_publish=1) == 0
+    assert gtap_amd.gtap_get_config()["epaq_publish"] == 1
     assert gtap_amd.gtap_set_config(scheduler=2, idle_sleep=4) == 0
     c = gtap_amd.gtap_get_config()
     assert c["scheduler"] == 2 and c["idle_sleep"] == 4

@@ -39,3 +39,30 @@ def test_result_change_is_an_error():
 def test_unknown_workload():
     with pytest.raises(ValueError):
         autotune.workload("nosuch")
+
+
+def test_tuned_table_lookup(tmp_path, monkeypatch):
+    """tuned() reads the committed table: the best point of a workload, {} for an unknown
+    workload or a missing file."""
+    import json
+    p = tmp_path / "t.json"
+    p.write_text(json.dumps({"nq14": {"best": {"blocks": 512, "probe_lanes": 4}, "best_ms": 1.0}}))
+    monkeypatch.setattr(autotune, "TUNED_PATH", str(p))
+    assert autotune.tuned("nq14") == {"blocks": 512, "probe_lanes": 4}
+    assert autotune.tuned("fib30") == {}
+    monkeypatch.setattr(autotune, "TUNED_PATH", str(tmp_path / "absent.json"))
+    assert autotune.tuned("nq14") == {}
+
+
+def test_committed_table_is_well_formed():
+    """Every committed entry names only searchable coordinates with integer values."""
+    import json
+    import os
+    if not os.path.exists(autotune.TUNED_PATH):
+        pytest.skip("no tuned table committed")
+    table = json.load(open(autotune.TUNED_PATH))
+    known = set(autotune.CONFIG_KEYS) | {"split", "cutoff", "merge_leaf", "leaf"}
+    for w, entry in table.items():
+        assert set(entry["best"]) <= known, w
+        assert all(isinstance(v, int) for v in entry["best"].values()), w
+        assert entry["best_ms"] <= entry["default_ms"] * 1.0001, w

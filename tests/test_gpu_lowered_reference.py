@@ -70,7 +70,10 @@ def test_reference_nq_detached_tasks(mods, n, cutoff):
     m.set_symbol("d_grid_size", n)
     m.set_symbol("d_cutoff_depth", cutoff)
     run_ok(m, "my_kernel", [])
-    assert m.get_symbol("d_answer") == oracle.nq(n)[0]
+    # nq.cu counts a placement only in serial_search, which runs below the cutoff row
+    # (nq.cu:9-13, 29-32): with n <= cutoff every task stops at row n with no free column,
+    # and the program's own answer is 0
+    assert m.get_symbol("d_answer") == (oracle.nq(n)[0] if n > cutoff else 0)
 
 
 def _sort_i32(m, kernel, keys):

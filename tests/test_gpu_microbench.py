@@ -35,3 +35,18 @@ def test_fences(waves):
 def test_fences_rejects_bad_sizes():
     with pytest.raises(Exception):
         gtap_amd.bench_fences(6, 16)
+
+
+@pytest.mark.parametrize("words", [1, 64, 4096])
+def test_contended_atomics(words):
+    print(f"case bench_atomics_contended words={words}", flush=True)
+    g = gtap_amd.bench_atomics_contended(words, 2)
+    assert g > 0.0 and math.isfinite(g)
+
+
+@pytest.mark.parametrize("table", [1 << 20, 16 << 20])
+def test_random_access(table):
+    print(f"case bench_random_access table={table}", flush=True)
+    r = gtap_amd.bench_random_access(table, 2)
+    assert r["load_gps"] > 0.1 and math.isfinite(r["load_gps"])
+    assert r["fetch_min_gps"] > 0.1 and math.isfinite(r["fetch_min_gps"])

@@ -93,3 +93,58 @@ def test_worker_stats_sum_to_run_stats(prog, arg, expect):
     for i, f in enumerate(G.WORKER_STAT_FIELDS):
         assert int(ws[:, i].sum()) == s[f], f
     assert int(ws[:, 0].sum()) > 0
+
+
+@pytest.mark.parametrize("view", ["having_task", "working"])
+def test_single_view_prints_timeline_and_writes_its_files(tmp_path, monkeypatch, capfd, view):
+    """visualize_having_task_time / visualize_working_time on their own: the text timeline of
+    the first ten workers (80 columns, 'H'/'W' where a worker got work, 'N' where it ran out)
+    and that view's two CSV files only, relative times starting at 0 on the view's first sample."""
+    monkeypatch.chdir(tmp_path)
+    print(f"case profile view {view}", flush=True)
+    assert G.gtap_set_profiling(True, 1 << 12) == 0
+    try:
+        v, st = G.fib(20)
+        assert v == O.fib(20)
+        capfd.readouterr()
+        getattr(G, f"visualize_{view}_time")("one")
+    finally:
+        assert G.gtap_set_profiling(False, 0) == 0
+    out = capfd.readouterr().out.splitlines()
+    title = "Having-Task" if view == "having_task" else "Working"
+    assert f"=== Warp {title} Time Visualization ===" in out
+    i = out.index("Warp  |" + "".join("|" if c % 10 == 0 else "-" for c in range(80)))
+    rows = out[i + 1:i + 1 + min(10, st["workers"])]
+    assert len(rows) == min(10, st["workers"])
+    mark = "H" if view == "having_task" else "W"
+    for w, r in enumerate(rows):
+        assert r.startswith(f"{w:5d} |") and len(r) == 7 + 80
+        assert set(r[7:]) <= {" ", mark, "N"}
+    assert any(mark in r[7:] for r in rows)
+    other = "working" if view == "having_task" else "having_task"
+    assert not os.path.exists(tmp_path / "profile" / f"one_warp_timeline_{other}.csv")
+    tl = _read(tmp_path, "one", f"warp_timeline_{view}")
+    assert min(float(r["relative_time_ms"]) for r in tl) == 0.0
+    assert len(_read(tmp_path, "one", f"warp_statistics_{view}")) == st["workers"]
+
+
+@pytest.mark.parametrize("prog,arg", [("fib", 22), ("nqueens", 11)])
+def test_round_phase_breakdown(prog, arg):
+    """The profiling build times each round's phases with the shader clock: the phases of every
+    round are counted (rounds equal the run statistics), and a round always runs its tasks,
+    issues and waits for its writes and publishes; without profiling nothing is recorded."""
+    print(f"case round phases {prog}({arg})", flush=True)
+    assert G.gtap_set_profiling(True, 256) == 0
+    try:
+        v, st = getattr(G, prog)(arg)
+        ph = G.round_phases()
+    finally:
+        assert G.gtap_set_profiling(False, 0) == 0
+    assert v == (O.fib(arg) if prog == "fib" else O.nq(arg)[0])
+    assert ph["rounds"] == st["rounds"] > 0
+    for k in ("load", "run", "store_wait", "publish"):
+        assert ph["cycles"][k] > 0, k
+    per = ph["cycles_per_round"]
+    assert all(x >= 0 for x in per.values())
+    G.fib(15)  # a launch without profiling leaves the sums at 0
+    assert G.round_phases()["rounds"] == 0

@@ -408,3 +408,14 @@ def test_oracle_tree_closed_forms_and_saturation():
     # a large positive one saturates at 2^32 - 1
     big = np.full(16, 1e12)
     assert oracle.tree_node(5, 3, 0, big) == float(((0x9E3779B9 ^ 5) + 0xFFFFFFFF) & 0xFFFFFFFF)
+
+
+def test_stream_cpu_baseline_matches_job_results():
+    """The CPU stream baseline computes every job like oracle.job_result, sequentially and
+    with the OpenMP loop over the jobs."""
+    t1, seq = oracle.stream_cpu(7, 40, False)
+    t2, par = oracle.stream_cpu(7, 40, True)
+    kind, param, kseed = oracle.jobs(7, 40)
+    exp = [oracle.job_result(int(k), int(p), int(s)) for k, p, s in zip(kind, param, kseed)]
+    assert seq.tolist() == exp and par.tolist() == exp
+    assert t1 > 0 and t2 > 0
