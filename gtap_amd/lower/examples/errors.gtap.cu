@@ -2,7 +2,9 @@
 // runtime record an error and end the launch normally.
 //   chain(n, q, w): a chain of n tasks; the child goes to queue q, the parent resumes on w
 //   (queue indices must be below GTAP_NUM_QUEUES);
-//   leave(n): at n == 2 a task returns before the taskwait that joins its child.
+//   leave(n): at n == 2 a task returns before the taskwait that joins its child;
+//   wide(n): one task joins n children, more than a deque holds when n is large (joined
+//   children must stay private until the round ends, so they cannot go to the overflow ring).
 #include <cuda_runtime.h>
 #include "gtap_thread.cuh"
 
@@ -31,6 +33,21 @@ __device__ long long leave(int n) {
     return a + 1;
 }
 
+#pragma gtap function
+__device__ long long one(int i) {
+    return i >= 0 ? 1 : 0;
+}
+
+#pragma gtap function
+__device__ long long wide(int n) {
+    for (int i = 0; i < n; ++i) {
+        #pragma gtap task
+        one(i);
+    }
+    #pragma gtap taskwait
+    return n;
+}
+
 __global__ void chain_kernel(int n, int q, int w) {
     #pragma gtap entry
     d_out = chain(n, q, w);
@@ -39,4 +56,9 @@ __global__ void chain_kernel(int n, int q, int w) {
 __global__ void leave_kernel(int n) {
     #pragma gtap entry
     d_out = leave(n);
+}
+
+__global__ void wide_kernel(int n) {
+    #pragma gtap entry
+    d_out = wide(n);
 }

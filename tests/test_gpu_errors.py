@@ -51,6 +51,25 @@ def test_invalid_queue_index(mods, q, w, code):
     clean_chain(m)
 
 
+def test_joined_children_beyond_deque_capacity(mods):
+    """Joined children stay private until their parent's round ends, so a task that joins more
+    children than a deque holds (32768 slots, 31744 usable) ends the launch with a queue
+    overflow (3), where the reference traps (gtap_runtime_thread.cuh:42-45); the same program
+    within the capacity, and the next launches, are exact."""
+    m = mods["errors"]
+    print("case wide(20000) (fits)", flush=True)
+    m.set_symbol("d_out", -1, ctypes.c_longlong)
+    assert m.launch("wide_kernel", [I], 20000) == 0
+    assert gtap_amd.gtap_get_runtime_error_code() == 0
+    assert m.get_symbol("d_out", ctypes.c_longlong) == 20000
+    print("case wide(40000) (queue overflow 3 expected)", flush=True)
+    assert m.launch("wide_kernel", [I], 40000) == 0
+    assert gtap_amd.gtap_get_runtime_error_code() == 3
+    r = report()
+    assert r.valid == 1 and r.code == 3 and r.limit == 32768 - 1024
+    clean_chain(m)
+
+
 def test_joined_child_left_behind(mods):
     print("case task returning before its taskwait (error 8 expected)", flush=True)
     m = mods["errors"]
