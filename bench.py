@@ -176,6 +176,8 @@ def bench_cilksort(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
         ms.append(s.kernel_ms)
     out = work.to_array(np.empty(n, dtype=np.uint32))
     ok = bool(np.array_equal(out, np.sort(keys)))
+    if not ok:
+        raise RuntimeError("cilksort 2^26: the output is not the sorted input")
     for b in (src, work, tmp):
         b.free()
     med = statistics.median(ms[1:])
@@ -218,6 +220,16 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, rand_meas=None, reps: int = 3) -> di
     reached = depth < gtap_amd.INF_DEPTH
     vr = int(reached.sum())
     er = int(np.diff(row)[reached].sum())
+    # the distances' invariants (unique shortest-hop distances): depth[0] = 0, no edge from a
+    # reached vertex improves its head, every other reached vertex has a tight in-edge
+    src = np.repeat(np.arange(nv, dtype=np.int64), np.diff(row))
+    dv, du = depth[src].astype(np.int64), depth[col].astype(np.int64)
+    rs = dv < gtap_amd.INF_DEPTH
+    tight = np.zeros(nv, dtype=bool)
+    tight[col[rs & (du == dv + 1)]] = True
+    tight[0] = True
+    if depth[0] != 0 or np.any(du[rs] > dv[rs] + 1) or not np.all(tight[reached]):
+        raise RuntimeError("bfs rmat22: the distances break the BFS invariants")
     med4, tasks4 = res[4]
     gbps = roofline.bfs_bytes(vr, er) / (med4 / 1e3) / 1e9
     log(f"bfs rmat22: epaq4 {med4:.3f} ms, 1 queue {res[1][0]:.3f} ms, {gbps:.0f} GB/s algorithmic")

@@ -114,6 +114,7 @@ struct GtapDev {
     uint64_t* gq_ctl;          // global-queue scheduler: [queue][head, reserved, committed] * 16
     uint64_t* ovf_cells;       // deque overflow rings: [queue][ovf_cap] cells (nullptr: none)
     uint64_t* ovf_ctl;         // [queue * 32] head, [queue * 32 + 16] tickets (positions, never reset)
+    uint32_t* ovf_epoch;       // [64 * 32]: per group of waves (me % 64), bumped after every push to a ring
     uint32_t ovf_cap;          // cells per overflow ring (power of two)
     uint32_t epaq_publish;     // 1: non-batch EPAQ queues publish their whole private window each round
 };

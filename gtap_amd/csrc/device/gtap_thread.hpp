@@ -374,19 +374,6 @@ __device__ __forceinline__ void record_from_desc(const GtapDev& dev, __amdgpu_bu
     frame_words_st<FW>(rs, off, f);
 }
 
-// The same from descriptor words held in registers (a spawn that goes straight to a ring).
-template <class P>
-__device__ __forceinline__ void record_from_words(const GtapDev& dev, __amdgpu_buffer_rsrc_t rs, uint32_t id,
-                                                  const uint32_t* d) {
-    constexpr int FW = CtxShape<P>::kFW;
-    const uint32_t off = id * dev.stride;
-    uint32_t f[FW];
-#pragma unroll
-    for (int i = 0; i < FW; ++i) f[i] = d[3 + i];
-    st128(rs, off, u32x4{d[1], d[2], 0u, d[0]});
-    frame_words_st<FW>(rs, off, f);
-}
-
 // Copies an LDS-held entry into staging entry `at` in record order (header, frame words).
 template <class P, class W>
 __device__ __forceinline__ void stage_from_desc(W* w, uint32_t at, uint32_t id, const uint32_t* d) {
@@ -410,8 +397,12 @@ __device__ __forceinline__ void stage_from_desc(W* w, uint32_t at, uint32_t id, 
 // record is complete, writes READY | p | id; a consumer claims a run of READY cells at the
 // head with one CAS, reads the ids and frees each cell for p + cap. Positions only grow:
 // the counters are never reset, and the reset kernel frees what an early end leaves between
-// the head and the tickets. Idle waves take from the rings; a wave does not count itself idle
-// while a ring holds tickets (quiescence then still implies that no task is anywhere).
+// the head and the tickets. Idle waves take from the rings. Every push is followed by a bump
+// of 64 epoch words, one per group of waves (wave me reads word me % 64, so the reads spread
+// over 64 lines): a wave looks at the rings only when its group's epoch moved since it last
+// found them all empty, and counts itself idle only when it has not moved. A wave that pushed
+// therefore sees its own push before it can count itself idle, and quiescence still implies
+// that no task is anywhere.
 constexpr uint64_t kOvfReady = 2ull << 62;
 constexpr uint64_t kOvfTagMask = 0x3FFFFFFFull;
 constexpr uint32_t kOvfSpillMax = 4096;  // tasks one deque spill moves at most
@@ -419,20 +410,37 @@ __device__ __forceinline__ uint64_t ovf_free_word(uint64_t pos) { return (pos & 
 __device__ __forceinline__ uint64_t ovf_ready_word(uint64_t pos, uint32_t id) {
     return kOvfReady | ((pos & kOvfTagMask) << 32) | (uint64_t)id;
 }
-__device__ __forceinline__ uint64_t* ovf_cell(const GtapDev& dev, uint32_t q, uint64_t pos) {
-    return dev.ovf_cells + (size_t)q * dev.ovf_cap + (pos & (uint64_t)(dev.ovf_cap - 1));
+
+// What the ring code uses of the launch, by value: the ring code is out of line (none of it
+// sits in the hot loop's registers) and takes scalars only, so the kernel argument block is
+// never copied to scratch for it.
+struct OvfView {
+    uint64_t* cells;
+    uint64_t* ctl;
+    int32_t* dctl;
+    gtap_error_report* report;
+    uint32_t* epoch;
+    uint32_t cap;
+    uint32_t term_mode;
+};
+__device__ __forceinline__ OvfView ovf_view(const GtapDev& dev) {
+    return OvfView{dev.ovf_cells, dev.ovf_ctl, dev.ctl, dev.report, dev.ovf_epoch, dev.ovf_cap, dev.term_mode};
 }
-__device__ __forceinline__ uint64_t* ovf_head(const GtapDev& dev, uint32_t q) { return dev.ovf_ctl + (size_t)q * 32; }
-__device__ __forceinline__ uint64_t* ovf_tickets(const GtapDev& dev, uint32_t q) {
-    return dev.ovf_ctl + (size_t)q * 32 + 16;
+constexpr uint32_t kOvfEmpty = 0x40000000u;  // ovf_pop: the ring held no tickets
+constexpr uint32_t kOvfLeftIdle = 0x80000000u;  // ovf_pop: the wave left the idle count
+__device__ __forceinline__ uint64_t* ovf_cell(const OvfView& v, uint32_t q, uint64_t pos) {
+    return v.cells + (size_t)q * v.cap + (pos & (uint64_t)(v.cap - 1));
 }
+__device__ __forceinline__ uint64_t* ovf_head(const OvfView& v, uint32_t q) { return v.ctl + (size_t)q * 32; }
+__device__ __forceinline__ uint64_t* ovf_tickets(const OvfView& v, uint32_t q) { return v.ctl + (size_t)q * 32 + 16; }
 
 // Positions [base, base + n) of ring q were just taken: false (after recording the error) if
 // they would come within kMargin cells of the head's lap.
-__device__ __forceinline__ bool ovf_room(const GtapDev& dev, uint32_t q, uint64_t base, uint32_t n) {
-    const uint64_t head = ld64(ovf_head(dev, q));
-    if (base + n - head > (uint64_t)(dev.ovf_cap - kMargin)) {
-        GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, -1, (int)q, (int)(base + n - head), (int)(dev.ovf_cap - kMargin));
+__device__ __forceinline__ bool ovf_room(const OvfView& v, uint32_t q, uint64_t base, uint32_t n) {
+    const uint64_t head = ld64(ovf_head(v, q));
+    if (base + n - head > (uint64_t)(v.cap - kMargin)) {
+        record_error(v.dctl, v.report, GTAP_ERROR_QUEUE_OVERFLOW, -1, (int)q, (int)(base + n - head),
+                     (int)(v.cap - kMargin), __LINE__);
         return false;
     }
     return true;
@@ -441,13 +449,13 @@ __device__ __forceinline__ bool ovf_room(const GtapDev& dev, uint32_t q, uint64_
 // Writes task `id` (record complete) into cell `pos` of ring q. The position's previous lap
 // was claimed before ovf_room let the position out, but its consumer may still be reading
 // the cell: wait until it is free (the consumer frees it right after its read).
-__device__ __forceinline__ void ovf_fill(const GtapDev& dev, uint32_t q, uint64_t pos, uint32_t id) {
-    uint64_t* c = ovf_cell(dev, q, pos);
+__device__ __forceinline__ void ovf_fill(const OvfView& v, uint32_t q, uint64_t pos, uint32_t id) {
+    uint64_t* c = ovf_cell(v, q, pos);
     const uint64_t want = ovf_free_word(pos);
     for (uint32_t spin = 0; ld64(c) != want; ++spin) {
-        if ((spin & 255u) == 255u && ld32(dev.ctl + CTL_ABORT * 32)) return;
+        if ((spin & 255u) == 255u && ld32(v.dctl + CTL_ABORT * 32)) return;
         if (spin > (1u << 22)) {
-            GTAP_DEVICE_ERROR(dev, GTAP_ERROR_WATCHDOG, (int)id, (int)q, (int)(pos & 0x7FFFFFFF), 0);
+            record_error(v.dctl, v.report, GTAP_ERROR_WATCHDOG, (int)id, (int)q, (int)(pos & 0x7FFFFFFF), 0, __LINE__);
             return;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -455,107 +463,98 @@ __device__ __forceinline__ void ovf_fill(const GtapDev& dev, uint32_t q, uint64_
     st64(c, ovf_ready_word(pos, id));
 }
 
-// Whether any overflow ring holds tickets not yet claimed (lane 0's answer, made uniform).
-template <int Q>
-__device__ __forceinline__ bool ovf_pending(const GtapDev& dev) {
-    int32_t any = 0;
-    if (lane_id() == 0) {
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
-            if (ld64(ovf_tickets(dev, (uint32_t)q)) != ld64(ovf_head(dev, (uint32_t)q))) any = 1;
-    }
-    return shfl(any, 0) != 0;
-}
-
-// Moves the oldest published tasks of this wave's deque q to ring q when the deque holds more
-// than half its capacity: one CAS on the deque word claims them from the top end, as a thief
-// would, then the wave writes their ids into ring cells. Called by the whole wave after the
-// round's publish.
-template <class P>
-__device__ __forceinline__ void ovf_spill_deque(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t q) {
+// Moves the oldest published tasks of a deque to ring q: called by the whole wave after its
+// round's publish when the deque holds more than half its capacity. One CAS on the deque word
+// claims them from the top end, as a thief would, then the wave writes their ids into ring
+// cells. `word` / `slots` are the deque's word and slot array, `bottom` its published bottom.
+// Returns the top the deque word was seen at (or moved to) and, in the high half, the number
+// of tasks moved.
+__device__ __forceinline__ uint64_t ovf_spill_deque(OvfView v, uint32_t q, uint64_t* word, uint32_t* slots,
+                                                 uint32_t dq_cap) {
     const int lane = lane_id();
-    const uint32_t half = dev.cap / 2;
-    const uint32_t b = uniform(vld(&w->bottom[q]));
-    if (b - uniform(vld(&w->top_cache[q])) <= half) return;
+    const uint32_t half = dq_cap / 2;
     uint64_t wv = 0;
-    if (lane == 0) wv = ld64(dq_word_ptr(dev, q, me));
+    if (lane == 0) wv = ld64(word);
     wv = shfl(wv, 0);
     const uint32_t top = (uint32_t)(wv >> 32), bw = (uint32_t)wv;
-    if (lane == 0) vst(&w->top_cache[q], top);
-    if ((int32_t)(bw - top) <= (int32_t)half) return;
+    if ((int32_t)(bw - top) <= (int32_t)half) return top;
     uint32_t k = (bw - top) - half / 2;
     if (k > kOvfSpillMax) k = kOvfSpillMax;
-    int ok = 0;
+    int32_t ok = 0;
     uint64_t base = 0;
     if (lane == 0) {
         uint64_t expected = wv;
-        ok = atomic_cas(dq_word_ptr(dev, q, me), expected, wv + ((uint64_t)k << 32)) ? 1 : 0;
+        ok = atomic_cas(word, expected, wv + ((uint64_t)k << 32)) ? 1 : 0;
         if (ok) {
-            vst(&w->top_cache[q], top + k);
-            base = atomic_add(ovf_tickets(dev, q), (uint64_t)k);
-            if (!ovf_room(dev, q, base, k)) ok = 0;
+            base = atomic_add(ovf_tickets(v, q), (uint64_t)k);
+            if (!ovf_room(v, q, base, k)) ok = 2;
         }
     }
-    ok = shfl((int32_t)ok, 0);
-    if (!ok) return;  // a thief moved the top (the next round tries again), or the ring is full
+    ok = shfl(ok, 0);
+    if (ok == 0) return top;                 // a thief moved the top: the next round tries again
+    if (ok == 2) return top + k;             // the ring is full: the error ends the launch
     base = shfl(base, 0);
-    for (uint32_t j = (uint32_t)lane; j < k; j += kWave) ovf_fill(dev, q, base + j, ld32(dq_slot_ptr(dev, q, me, top + j)));
-    if (lane == 0) vst(&w->ring_spilled, vld(&w->ring_spilled) + k);
+    for (uint32_t j = (uint32_t)lane; j < k; j += kWave) ovf_fill(v, q, base + j, ld32(slots + ((top + j) & (dq_cap - 1))));
+    complete_stores();
+    atomic_add_noret(v.epoch + (uint32_t)lane * 32u, 1u);  // every group's epoch (one lane each)
+    return (uint64_t)(top + k) | ((uint64_t)k << 32);
 }
 
-// Takes up to 64 ready tasks (1 on the wave queue) from the head of ring q. Returns the
-// number taken; lane i < that number receives one in tid_out.
-template <class P>
-__device__ __forceinline__ uint32_t ovf_pop(const GtapDev& dev, uint32_t q, uint32_t& tid_out, bool& idle_registered) {
+// Takes up to `cap` ready tasks (64, or 1 on the wave queue) from the head of ring q. Low
+// half of the result: the number taken (lane i < that number receives one in the high half),
+// or kOvfEmpty when the ring held no tickets; kOvfLeftIdle: the wave left the idle count
+// before its claim.
+__device__ __forceinline__ uint64_t ovf_pop(OvfView v, uint32_t q, uint32_t cap, int32_t idle_registered) {
     const int lane = lane_id();
-    const uint32_t cap = ((int)q == HasWaveQueue<P>::value) ? 1u : (uint32_t)kWave;
     uint64_t h = 0, tk = 0;
     if (lane == 0) {
-        h = ld64(ovf_head(dev, q));
-        tk = ld64(ovf_tickets(dev, q));
+        h = ld64(ovf_head(v, q));
+        tk = ld64(ovf_tickets(v, q));
     }
     h = shfl(h, 0);
     tk = shfl(tk, 0);
-    if ((int64_t)(tk - h) <= 0) return 0;
+    if ((int64_t)(tk - h) <= 0) return kOvfEmpty;
     const uint32_t avail = (tk - h) < (uint64_t)cap ? (uint32_t)(tk - h) : cap;
     uint64_t cv = 0;
     bool ready = false;
     if ((uint32_t)lane < avail) {
-        cv = ld64(ovf_cell(dev, q, h + (uint32_t)lane));
+        cv = ld64(ovf_cell(v, q, h + (uint32_t)lane));
         ready = cv == ovf_ready_word(h + (uint32_t)lane, (uint32_t)cv);
     }
     const uint64_t notready = ~ballot(ready);
     const uint32_t k = notready ? (uint32_t)__builtin_ctzll(notready) : 64u;  // ready run from lane 0
     if (k == 0) return 0;
-    if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
-        if (lane == 0) atomic_add(dev.ctl + CTL_IDLE * 32, -1);
-        idle_registered = false;
+    uint32_t left = 0;
+    if (v.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
+        if (lane == 0) atomic_add(v.dctl + CTL_IDLE * 32, -1);
+        left = kOvfLeftIdle;
     }
-    int ok = 0;
+    int32_t ok = 0;
     if (lane == 0) {
         uint64_t expected = h;
-        ok = atomic_cas(ovf_head(dev, q), expected, h + k) ? 1 : 0;
+        ok = atomic_cas(ovf_head(v, q), expected, h + k) ? 1 : 0;
     }
-    ok = shfl((int32_t)ok, 0);
-    if (!ok) return 0;
+    ok = shfl(ok, 0);
+    if (!ok) return left;
+    uint32_t tid = 0;
     if ((uint32_t)lane < k) {
-        tid_out = (uint32_t)cv;
-        st64(ovf_cell(dev, q, h + (uint32_t)lane), ovf_free_word(h + (uint32_t)lane + dev.ovf_cap));
+        tid = (uint32_t)cv;
+        st64(ovf_cell(v, q, h + (uint32_t)lane), ovf_free_word(h + (uint32_t)lane + v.cap));
     }
-    return k;
+    return (uint64_t)(k | left) | ((uint64_t)tid << 32);
 }
 
 // Append `id` to the private (unpublished) end of queue `q` of this wave. Called by all
 // active lanes; lanes may target different queues. kDesc: the entry's record is held in
 // LDS (the D descriptor words `dw`), else it is already in HBM. An older private entry that
 // the append pushes out of the 64-slot window is written out: its record if it was held in
-// LDS, and its deque slot. `detached`: nothing joins the entry, so it may become visible at
-// once; when the deque is full and every appending lane's entry is detached, the entries go
-// to the overflow ring instead (a joined child must stay private until its parent's
-// taskwait state is written after the round).
+// LDS, and its deque slot. A round that appends more than the deque holds beyond its
+// published tasks ends the launch with GTAP_ERROR_QUEUE_OVERFLOW (the round's children are
+// private until the round ends: their parents' taskwait states are written then); what
+// accumulates over rounds goes to the overflow ring instead (ovf_spill_deque).
 template <class P, bool kDesc>
 __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t q,
-                                             uint32_t id, const uint32_t* dw, bool detached = false) {
+                                             uint32_t id, const uint32_t* dw) {
     constexpr int Q = CtxShape<P>::kQ, QL = CtxShape<P>::kQL, D = CtxShape<P>::kD;
     const __amdgpu_buffer_rsrc_t rs = pool_rsrc(dev.recs);
 #pragma unroll
@@ -574,26 +573,7 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w
                 vst(&w->top_cache[qq], top);
             }
             if (t + cnt - top > dev.cap - kMargin) {
-                if (dev.ovf_cells != nullptr && ballot(!detached) == 0) {
-                    // the records complete, then the ids into ring cells
-                    if constexpr (kDesc) {
-                        if (qq < QL) record_from_words<P>(dev, rs, id, dw);
-                    }
-                    complete_stores();
-                    const int lead = first_lane(mq);
-                    uint64_t base = 0;
-                    int32_t room = 0;
-                    if (r == 0) {
-                        base = atomic_add(ovf_tickets(dev, (uint32_t)qq), (uint64_t)cnt);
-                        room = ovf_room(dev, (uint32_t)qq, base, cnt) ? 1 : 0;
-                        vst(&w->ring_spilled, vld(&w->ring_spilled) + cnt);
-                    }
-                    base = shfl(base, lead);
-                    room = shfl(room, lead);
-                    if (room) ovf_fill(dev, (uint32_t)qq, base + r, id);
-                } else if (r == 0) {
-                    GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, id, qq, t + cnt - top, dev.cap - kMargin);
-                }
+                if (r == 0) GTAP_DEVICE_ERROR(dev, GTAP_ERROR_QUEUE_OVERFLOW, id, qq, t + cnt - top, dev.cap - kMargin);
             } else {
                 const uint32_t pos = t + r;
                 const uint32_t slot = pos & (kWave - 1);
@@ -673,9 +653,9 @@ __device__ __forceinline__ uint32_t spawn_record(const GtapDev& dev, WaveCtxOf<P
         const uint32_t off = id * dev.stride;
         st128(rs, off, u32x4{d[1], d[2], 0u, d[0]});
         frame_words_st<FW>(rs, off, &d[3]);
-        queue_append<P, false>(dev, w, me, q, id, nullptr, par == kNoParent);
+        queue_append<P, false>(dev, w, me, q, id, nullptr);
     } else {
-        queue_append<P, true>(dev, w, me, q, id, d, par == kNoParent);
+        queue_append<P, true>(dev, w, me, q, id, d);
     }
     const uint64_t am = active_mask();
     if (rank_in(am) == 0) vst(&w->spawns, vld(&w->spawns) + (uint32_t)popc(am));
@@ -1516,6 +1496,13 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
 
     uint64_t host_poll_last = 0;
     const bool host_poller = dev.term_mode == GTAP_TERM_SERVE && me == 0;
+    // this wave's group epoch of the overflow rings when it last found them all empty; the
+    // rings start a launch empty (the reset drains them), so the epoch read at the start will do
+    uint32_t ring_seen = 0;
+    if (Sched != kSchedGQ && dev.ovf_cells != nullptr) {
+        if (lane == 0) ring_seen = ld32(dev.ovf_epoch + (me & 63u) * 32u);
+        ring_seen = uniform(shfl(ring_seen, 0));
+    }
     if (dev.jobs != nullptr) {
         n = admit_static<P>(dev, w, me, my_tid);
         st.jobs += n;
@@ -1553,15 +1540,31 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 }
             }
             // the overflow rings (tasks that full deques handed off), every fourth idle try
-            if (Sched != kSchedGQ && n == 0 && dev.ovf_cells != nullptr && (idle_iter & 3u) == 0) {
-#pragma unroll
-                for (int i = 0; i < Q && n == 0; ++i) {
-                    const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
-                    n = ovf_pop<P>(dev, q, my_tid, idle_registered);
-                    if (n) {
-                        batch_q = (int)q;
-                        st.ring_taken += n;
+            // the overflow rings, when this wave's group epoch moved since it last found them empty
+            bool rings_seen_empty = true;
+            if (Sched != kSchedGQ && n == 0 && dev.ovf_cells != nullptr) {
+                uint32_t ep = 0;
+                if (lane == 0) ep = ld32(dev.ovf_epoch + (me & 63u) * 32u);
+                ep = uniform(shfl(ep, 0));
+                if (ep != ring_seen) {
+                    bool all_empty = true;
+#pragma unroll 1
+                    for (int i = 0; i < Q && n == 0; ++i) {
+                        const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
+                        const uint32_t cap = ((int)q == WQ) ? 1u : (uint32_t)kWave;
+                        const uint64_t got = ovf_pop(ovf_view(dev), q, cap, idle_registered ? 1 : 0);
+                        const uint32_t code = uniform((uint32_t)got);
+                        if (code & kOvfLeftIdle) idle_registered = false;
+                        if (!(code & kOvfEmpty)) all_empty = false;
+                        n = code & ~(kOvfLeftIdle | kOvfEmpty);
+                        if (n) {
+                            if ((uint32_t)lane < n) my_tid = (uint32_t)(got >> 32);
+                            batch_q = (int)q;
+                            st.ring_taken += n;
+                        }
                     }
+                    if (all_empty) ring_seen = ep;
+                    else rings_seen_empty = false;
                 }
             }
             if (n == 0) {
@@ -1572,8 +1575,7 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 // ---- idle: termination, abort, watchdog, backoff. A wave counts itself idle
                 // only while no overflow ring holds tickets: a wave that handed tasks to a
                 // ring therefore sees them before it can count itself idle.
-                if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered &&
-                    !(Sched != kSchedGQ && dev.ovf_cells != nullptr && ovf_pending<Q>(dev))) {
+                if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered && rings_seen_empty) {
                     idle_registered = true;
                     int32_t c = 0;
                     int32_t started = 0;
@@ -1857,8 +1859,18 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
         // a deque above half its capacity hands its oldest published tasks to the overflow ring
         if constexpr (Sched != kSchedGQ) {
             if (dev.ovf_cells != nullptr) {
-#pragma unroll
-                for (int q = 0; q < Q; ++q) ovf_spill_deque<P>(dev, w, me, (uint32_t)q);
+#pragma unroll 1
+                for (int q = 0; q < Q; ++q) {
+                    const uint32_t b = uniform(vld(&w->bottom[q]));
+                    if (b - uniform(vld(&w->top_cache[q])) > dev.cap / 2) {
+                        const uint64_t r = ovf_spill_deque(ovf_view(dev), (uint32_t)q, dq_word_ptr(dev, q, me),
+                                                           dq_slot_ptr(dev, q, me, 0), dev.cap);
+                        if (lane == 0) {
+                            vst(&w->top_cache[q], (uint32_t)r);
+                            vst(&w->ring_spilled, vld(&w->ring_spilled) + (uint32_t)(r >> 32));
+                        }
+                    }
+                }
             }
         }
         phase_mark(PH_PUBLISH);

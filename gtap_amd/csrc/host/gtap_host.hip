@@ -147,6 +147,7 @@ GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job
     d.ovf_cells = R.ovf_cells;
     d.ovf_ctl = R.ovf_ctl;
     d.ovf_cap = R.ovf_cap;
+    d.ovf_epoch = R.ovf_epoch;
     d.epaq_publish = R.cfg.epaq_publish ? 1u : 0u;
     d.idle_level = 1;
     for (uint32_t v = R.cfg.idle_sleep; v > 1 && d.idle_level < 6; v >>= 1) d.idle_level += 1;
@@ -374,6 +375,7 @@ static void free_all() {
     dfree(R.gq_ctl);
     dfree(R.ovf_cells);
     dfree(R.ovf_ctl);
+    dfree(R.ovf_epoch);
     dfree(R.job_units);
     if (R.host_ctl) (void)hipHostFree(R.host_ctl);
     if (R.report) (void)hipHostFree(R.report);
@@ -500,6 +502,8 @@ int gtap_initialize(void) {
     R.ovf_cap = kOverflowCells;
     GTAP_HIP_TRY(hipMalloc(&R.ovf_cells, (size_t)R.nqueues * R.ovf_cap * sizeof(uint64_t)));
     GTAP_HIP_TRY(hipMalloc(&R.ovf_ctl, (size_t)R.nqueues * 32 * sizeof(uint64_t)));
+    GTAP_HIP_TRY(hipMalloc(&R.ovf_epoch, 64 * 32 * sizeof(uint32_t)));
+    GTAP_HIP_TRY(hipMemset(R.ovf_epoch, 0, 64 * 32 * sizeof(uint32_t)));
     hipLaunchKernelGGL(ovf_init_kernel, dim3(1024), dim3(256), 0, R.stream, R.ovf_cells, R.ovf_ctl, R.nqueues,
                        R.ovf_cap);
     GTAP_HIP_TRY(hipGetLastError());
@@ -699,20 +703,33 @@ int gtap_device_free(void* ptr) {
     if (ptr) GTAP_HIP_TRY(hipFree(ptr));
     return 0;
 }
+// The helpers complete before they return: the runtime launches on its own non-blocking
+// stream, which does not wait for work on the null stream, so a copy still in flight would
+// race with the next launch that reads its destination (a device-to-device copy and a memset
+// may return before they are done).
 int gtap_memcpy_htod(void* dst, const void* src, uint64_t bytes) {
     if (bytes) GTAP_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    GTAP_HIP_TRY(hipStreamSynchronize(nullptr));
     return 0;
 }
 int gtap_memcpy_dtoh(void* dst, const void* src, uint64_t bytes) {
+    Runtime& R = rt();
+    if (R.initialized) GTAP_HIP_TRY(hipStreamSynchronize(R.stream));  // the launches that wrote src
     if (bytes) GTAP_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return 0;
 }
 int gtap_memcpy_dtod(void* dst, const void* src, uint64_t bytes) {
+    Runtime& R = rt();
+    if (R.initialized) GTAP_HIP_TRY(hipStreamSynchronize(R.stream));
     if (bytes) GTAP_HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    GTAP_HIP_TRY(hipStreamSynchronize(nullptr));
     return 0;
 }
 int gtap_memset_device(void* dst, int value, uint64_t bytes) {
+    Runtime& R = rt();
+    if (R.initialized) GTAP_HIP_TRY(hipStreamSynchronize(R.stream));
     if (bytes) GTAP_HIP_TRY(hipMemset(dst, value, bytes));
+    GTAP_HIP_TRY(hipStreamSynchronize(nullptr));
     return 0;
 }
 

@@ -86,6 +86,7 @@ struct Runtime {
     // deque overflow rings (work stealing): [queue][ovf_cap] cells, head/ticket words
     uint64_t* ovf_cells = nullptr;
     uint64_t* ovf_ctl = nullptr;
+    uint32_t* ovf_epoch = nullptr;
     uint32_t ovf_cap = 0;
     uint32_t gq_cap = 0;
     // timeline profiling (gtap_set_profiling)

@@ -112,7 +112,8 @@ int gtap_bench_random_access(uint64_t table_bytes, int reps, double* out);
 int gtap_bench_fences(uint32_t waves, int iters, double* out_ns);
 
 /* ------------------------------------------------------------ device memory helpers */
-/* For callers that only hold host pointers (ctypes, plain C). */
+/* For callers that only hold host pointers (ctypes, plain C). Each completes before it returns
+ * and after the runtime's earlier launches (which run on the runtime's own stream). */
 int gtap_device_malloc(void** ptr, uint64_t bytes);
 int gtap_device_free(void* ptr);
 int gtap_memcpy_htod(void* dst, const void* src, uint64_t bytes);

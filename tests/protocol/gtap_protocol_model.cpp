@@ -21,11 +21,12 @@
 //          consumers CAS the head over a run of READY cells and free them for the next lap;
 //          the drain frees what an aborted launch left between head and tickets. Invariant:
 //          every id once, over many laps of a small ring, and a ring that works after a drain.
-//  ovfq    quiescence with the rings: a worker hands tasks to a ring and counts itself idle
-//          only while no ring holds tickets. Invariant: done is never set while a task is
-//          outstanding (in a worker or a ring), however rarely idle workers poll the ring;
-//          without the rule, a worker that counts itself idle before its next poll ends the
-//          launch early.
+//  ovfq    quiescence with the rings (thread_scheduler's idle path): a producer bumps every
+//          group's epoch after a push; a worker looks at the ring only when its group's epoch
+//          moved since it last found the ring empty, and counts itself idle only when it has
+//          not moved. Invariant: done is never set while a task is outstanding (in a worker or
+//          the ring); without the epoch rule, a worker that counts itself idle before its next
+//          look ends the launch early.
 //  join    the additive join word (gtap_dev.hpp: count [0..11] | resume queue [12..15] |
 //          signed sum [16..63]): children add (value << 16) - 1 concurrently. Invariant: the
 //          last child alone sees count 1, the queue bits survive, the final sum is exact at
@@ -443,16 +444,23 @@ bool run(unsigned seed, int workers, int owners, int work, bool count_owners_at_
 // ------------------------------------------------------------------ quiescence with the rings
 namespace ovfq_model {
 
+constexpr int kGroups = 4;
+
 // Workers run tasks; a busy worker sometimes hands half of what it holds to the ring (ticket
-// fetch-add, then each cell made ready later, as ovf_spill_deque does). An idle worker polls
-// the ring on every fourth idle try, starting with its second (leaving the idle count before
-// it takes), and counts itself idle on its first idle try; with `ring_rule` only while
-// tickets == head. Returns true if done was set while tasks were outstanding.
-bool run(unsigned seed, int workers, int work, bool ring_rule) {
+// fetch-add, each cell made ready, then every group's epoch bumped, as ovf_spill_deque does).
+// An idle worker looks at the ring when its group's epoch (worker w reads group w % kGroups)
+// moved since it last found the ring empty, taking a ready head task (leaving the idle count
+// first), and records the epoch when it finds the ring empty. With `epoch_rule` it counts
+// itself idle only when the epoch has not moved; without it, on its first idle try, and it
+// looks at the ring only on every fourth. Returns true if done was set while tasks were
+// outstanding.
+bool run(unsigned seed, int workers, int work, bool epoch_rule) {
     const uint32_t cap = 1u << 16;
     std::vector<std::atomic<uint8_t>> ready(cap);  // cell p holds a task (no laps here)
     for (auto& r : ready) r.store(0);
     std::atomic<uint64_t> head{0}, tickets{0};
+    std::atomic<uint32_t> epoch[kGroups];
+    for (auto& e : epoch) e.store(0);
     std::atomic<int> idle{0}, started{1}, done{0}, outstanding{work};
     std::atomic<bool> early{false};
     std::vector<std::thread> th;
@@ -462,6 +470,7 @@ bool run(unsigned seed, int workers, int work, bool ring_rule) {
             if (w != 0) started.fetch_add(1);  // worker 0 owns the job, counted at the reset
             int mine = w == 0 ? work : 0;
             bool registered = false;
+            uint32_t seen = epoch[w % kGroups].load();
             uint32_t idle_it = 0;
             for (long it = 0; it < 4000000; ++it) {
                 if (mine > 0) {
@@ -474,25 +483,35 @@ bool run(unsigned seed, int workers, int work, bool ring_rule) {
                             if (rng() % 64 == 0) std::this_thread::yield();  // a cell made ready late
                             ready[(base + j) & (cap - 1)].store(1, std::memory_order_release);
                         }
+                        for (auto& e : epoch) e.fetch_add(1);
                     }
                     mine -= 1;
                     outstanding.fetch_sub(1);
                     continue;
                 }
-                const uint64_t h = head.load(), tk = tickets.load();
-                if (idle_it++ % 4 == 1 && tk > h && ready[h & (cap - 1)].load(std::memory_order_acquire)) {
-                    if (registered) {
-                        idle.fetch_sub(1);
-                        registered = false;
+                const uint32_t ep = epoch[w % kGroups].load();
+                bool seen_empty = ep == seen;
+                const bool look = epoch_rule ? ep != seen : (idle_it % 4 == 1);
+                ++idle_it;
+                if (look) {
+                    const uint64_t h = head.load(), tk = tickets.load();
+                    if (tk <= h) {
+                        seen = ep;
+                        seen_empty = true;
+                    } else if (ready[h & (cap - 1)].load(std::memory_order_acquire)) {
+                        if (registered) {
+                            idle.fetch_sub(1);
+                            registered = false;
+                        }
+                        uint64_t expected = h;
+                        if (head.compare_exchange_strong(expected, h + 1, std::memory_order_acq_rel)) {
+                            ready[h & (cap - 1)].store(0);
+                            mine = 1;
+                        }
+                        continue;
                     }
-                    uint64_t expected = h;
-                    if (head.compare_exchange_strong(expected, h + 1, std::memory_order_acq_rel)) {
-                        ready[h & (cap - 1)].store(0);
-                        mine = 1;
-                    }
-                    continue;
                 }
-                if (!registered && (!ring_rule || tickets.load() == head.load())) {
+                if (!registered && (!epoch_rule || seen_empty)) {
                     registered = true;
                     if (idle.fetch_add(1) + 1 == started.load()) {
                         if (outstanding.load() != 0) early.store(true);
@@ -568,8 +587,8 @@ int main(int argc, char** argv) {
         ovf_model::run(s + 50, 2, 4, 2000, 256);
         CHECK(!ovfq_model::run(s, 5, 3000, true), "ovfq: done while tasks were outstanding (seed %u)", s);
     }
-    // without the ring rule a worker can count itself idle while the tasks it handed to the
-    // ring are not ready yet, and the last worker to go idle ends the launch early
+    // without the epoch rule a worker can count itself idle before it looks at the ring, and
+    // the last worker to go idle ends the launch early with tasks still in the ring
     bool ring_rule_needed = false;
     for (unsigned s = 200; s < 240 && !ring_rule_needed; ++s) ring_rule_needed = ovfq_model::run(s, 5, 3000, false);
     CHECK(ring_rule_needed, "ovfq: no schedule ended early without the ring rule");

@@ -1,7 +1,7 @@
 """The deque overflow ring with few workers: a star graph's source splits into ~64-edge tasks
-that each spawn a task per leaf, so with 8 wavefront workers a round spawns up to 64 x 64
-detached tasks, more than a 4096-slot deque takes. The appends that do not fit go to the
-overflow ring, the full deques hand their oldest tasks to it after the round, and idle waves
+that each spawn a task per leaf, so with 8 wavefront workers a wave publishes up to 64 x 64
+detached tasks per round, far more than the others take. A deque above half of its 16384
+slots hands its oldest published tasks to the overflow ring after the round, and idle waves
 take them back; distances stay bit-exact and every task moved to a ring is taken again."""
 import numpy as np
 import pytest
@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module", autouse=True)
 def few_workers_small_deques():
-    gtap_amd.gtap_set_config(timeout_ms=30000, deque_capacity=4096, blocks=2)
+    gtap_amd.gtap_set_config(timeout_ms=30000, deque_capacity=16384, blocks=2)
     assert gtap_amd.gtap_initialize() == 0
     yield
     assert gtap_amd.gtap_finalize() == 0
@@ -38,7 +38,7 @@ def star(leaves):
 @pytest.mark.parametrize("queues", [1, 4])
 @pytest.mark.parametrize("leaves", [1 << 18, 1 << 21])
 def test_star_overflows_into_the_ring(queues, leaves):
-    print(f"case bfs star leaves={leaves} queues={queues} deque capacity 4096", flush=True)
+    print(f"case bfs star leaves={leaves} queues={queues} deque capacity 16384, 8 workers", flush=True)
     row, col = star(leaves)
     depth, st = gtap_amd.bfs(row, col, 0, num_queues=queues)
     assert st["error_code"] == 0
