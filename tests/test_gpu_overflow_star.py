@@ -1,8 +1,9 @@
-"""The deque overflow ring with few workers: a star graph's source splits into ~64-edge tasks
-that each spawn a task per leaf, so with 8 wavefront workers a wave publishes up to 64 x 64
-detached tasks per round, far more than the others take. A deque above half of its 16384
-slots hands its oldest published tasks to the overflow ring after the round, and idle waves
-take them back; distances stay bit-exact and every task moved to a ring is taken again."""
+"""The deque overflow ring with few workers: a star graph's source range splits into 8-edge
+tasks that each spawn a task per leaf, so with 4 wavefront workers each round of a wave adds
+up to 64 x 8 detached tasks to its deque while it runs only 64, and the 3 others take little.
+A deque above half of its 4096 slots hands its oldest published tasks to the overflow ring
+after the round (a round adds at most 576, which the other half holds), and idle waves take
+them back; distances stay bit-exact and every task moved to a ring is taken again."""
 import numpy as np
 import pytest
 
@@ -14,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module", autouse=True)
 def few_workers_small_deques():
-    gtap_amd.gtap_set_config(timeout_ms=30000, deque_capacity=16384, blocks=2)
+    gtap_amd.gtap_set_config(timeout_ms=30000, deque_capacity=4096, blocks=1)
     assert gtap_amd.gtap_initialize() == 0
     yield
     assert gtap_amd.gtap_finalize() == 0
@@ -36,11 +37,11 @@ def star(leaves):
 
 
 @pytest.mark.parametrize("queues", [1, 4])
-@pytest.mark.parametrize("leaves", [1 << 18, 1 << 21])
+@pytest.mark.parametrize("leaves", [1 << 18, 1 << 20])
 def test_star_overflows_into_the_ring(queues, leaves):
-    print(f"case bfs star leaves={leaves} queues={queues} deque capacity 16384, 8 workers", flush=True)
+    print(f"case bfs star leaves={leaves} queues={queues} split 8, deque capacity 4096, 4 workers", flush=True)
     row, col = star(leaves)
-    depth, st = gtap_amd.bfs(row, col, 0, num_queues=queues)
+    depth, st = gtap_amd.bfs(row, col, 0, num_queues=queues, split=8)
     assert st["error_code"] == 0
     spilled, taken = ring_counts()
     print(f"ring: {spilled} tasks spilled, {taken} taken", flush=True)
