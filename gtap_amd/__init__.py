@@ -84,7 +84,8 @@ class Config(ctypes.Structure):
         ("idle_sleep", ctypes.c_uint32),
         ("scheduler", ctypes.c_uint32),
         ("epaq_publish", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("xcd_steal", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 

@@ -26,7 +26,8 @@ import time
 from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 # gtap_config fields the tuner may set (0 = the runtime's default / every co-resident workgroup)
-CONFIG_KEYS = ("blocks", "block_workers", "probe_lanes", "steal_batch", "deque_capacity", "epaq_publish", "idle_sleep")
+CONFIG_KEYS = ("blocks", "block_workers", "probe_lanes", "steal_batch", "deque_capacity", "epaq_publish", "idle_sleep",
+               "xcd_steal")
 
 # The first value of each coordinate is the runtime's default and the search's start point.
 DEFAULT_SPACE: Dict[str, Sequence[int]] = {
@@ -34,6 +35,7 @@ DEFAULT_SPACE: Dict[str, Sequence[int]] = {
     "probe_lanes": (8, 1, 4, 16, 32),
     "steal_batch": (64, 8, 16, 32),
     "deque_capacity": (32768, 65536),
+    "xcd_steal": (0, 1),
 }
 
 Measure = Callable[[Dict[str, int]], Tuple[float, object]]

@@ -117,6 +117,8 @@ struct GtapDev {
     uint32_t* ovf_epoch;       // [64 * 32]: per group of waves (me % 64), bumped after every push to a ring
     uint32_t ovf_cap;          // cells per overflow ring (power of two)
     uint32_t epaq_publish;     // 1: non-batch EPAQ queues publish their whole private window each round
+    uint32_t xcd_steal;        // 1: even probe lanes pick victims on the thief's XCD
+    uint32_t xcd_pad;
 };
 
 // Round phases of the thread-mode loop, timed in the profiling build with the shader clock

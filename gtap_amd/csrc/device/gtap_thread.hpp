@@ -1215,6 +1215,14 @@ __device__ __forceinline__ uint32_t try_steal(const GtapDev& dev, uint32_t me, u
     if ((uint32_t)lane < dev.probe_lanes) {
         victim = xorshift32(rng) % (dev.nworkers - 1);
         if (victim >= me) victim += 1;
+        if (dev.xcd_steal && (lane & 1) == 0 && dev.nworkers >= 64 && (dev.nworkers & 3u) == 0) {
+            // a victim on this wave's XCD: workgroups are dealt to the 8 XCDs round-robin, so
+            // workgroup 8 k + x runs on XCD x; 4 waves per workgroup
+            const uint32_t nb = dev.nworkers >> 2, x = (me >> 2) & 7u;
+            const uint32_t nbx = (nb - x + 7u) >> 3;  // workgroups on XCD x
+            const uint32_t vw = (((xorshift32(rng) % nbx) << 3) + x) * 4u + (victim & 3u);
+            if (vw != me && vw < dev.nworkers) victim = vw;
+        }
         wv = ld64(dq_word_ptr(dev, q, victim));
         sz = (int32_t)((uint32_t)wv - (uint32_t)(wv >> 32));
     }

@@ -123,7 +123,10 @@ typedef struct gtap_config {
                                  one the wave's next batch comes from keep their newest <= 64
                                  private entries for the wave's later rounds (default); 1 = they
                                  publish everything at the end of the round */
-    uint32_t reserved[3];
+    uint32_t xcd_steal;       /* thread mode: 1 = half of the steal probes go to victims on the
+                                 thief's own XCD (workgroup b runs on XCD b % 8); 0 = all
+                                 victims uniform at random (default) */
+    uint32_t reserved[2];
 } gtap_config;
 
 /* Called by gtap_initialize and gtap_reset (and at registration when the runtime is already

@@ -149,6 +149,7 @@ GtapDev make_dev(uint32_t nworkers, uint32_t term_mode, uint32_t njobs, gtap_job
     d.ovf_cap = R.ovf_cap;
     d.ovf_epoch = R.ovf_epoch;
     d.epaq_publish = R.cfg.epaq_publish ? 1u : 0u;
+    d.xcd_steal = R.cfg.xcd_steal ? 1u : 0u;
     d.idle_level = 1;
     for (uint32_t v = R.cfg.idle_sleep; v > 1 && d.idle_level < 6; v >>= 1) d.idle_level += 1;
     d.prof_having = R.prof_having;
@@ -405,6 +406,10 @@ int arm_entry_modules() {
     const GtapDev d = entry_dev();
     if (int st = reset_launch_state(d, R.stream, nullptr, true)) return st;
     GTAP_HIP_TRY(hipStreamSynchronize(R.stream));
+    // the per-worker counters gtap_get_worker_stats reads next are those of the entry launch
+    // (the reset zeroed every provisioned worker's row; the launch fills the ones it runs)
+    R.last_nworkers = d.nworkers;
+    R.stats_host_valid = false;
     for (gtap_entry_setup_fn f : entry_modules())
         if (int st = f(&d, sizeof(d))) return hip_status((hipError_t)st, "lowered module setup");
     return 0;
@@ -440,6 +445,7 @@ int gtap_set_config(const gtap_config* cfg) {
     if (R.initialized) return invalid_argument("gtap_set_config must precede gtap_initialize");
     if (cfg && cfg->scheduler > GTAP_SCHED_CHASE_LEV) return invalid_argument("gtap_config.scheduler must be 0..3");
     if (cfg && cfg->epaq_publish > 1) return invalid_argument("gtap_config.epaq_publish must be 0 or 1");
+    if (cfg && cfg->xcd_steal > 1) return invalid_argument("gtap_config.xcd_steal must be 0 or 1");
     if (cfg) R.cfg = *cfg;
     else memset(&R.cfg, 0, sizeof(R.cfg));
     return 0;

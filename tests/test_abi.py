@@ -60,7 +60,8 @@ def test_config_fields_and_scheduler_check_without_gpu():
     if not os.path.exists(gtap_amd.LIB_PATH):
         pytest.skip("libgtap_amd.so not built")
     names = [f for f, _ in gtap_amd.Config._fields_]
-    assert names[-4:] == ["idle_sleep", "scheduler", "epaq_publish", "reserved"]
+    assert names[-5:] == ["idle_sleep", "scheduler", "epaq_publish", "xcd_steal", "reserved"]
+    assert gtap_amd.gtap_set_config(xcd_steal=3) != 0
     assert gtap_amd.gtap_set_config(scheduler=9) != 0
     assert gtap_amd.gtap_set_config(epaq_publish=2) != 0
     assert gtap_amd.gtap_set_config(epaq_publish=1) == 0

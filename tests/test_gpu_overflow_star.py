@@ -1,9 +1,10 @@
-"""The deque overflow ring with few workers: a star graph's source range splits into 8-edge
-tasks that each spawn a task per leaf, so with 4 wavefront workers each round of a wave adds
-up to 64 x 8 detached tasks to its deque while it runs only 64, and the 3 others take little.
-A deque above half of its 4096 slots hands its oldest published tasks to the overflow ring
-after the round (a round adds at most 576, which the other half holds), and idle waves take
-them back; distances stay bit-exact and every task moved to a ring is taken again."""
+"""The deque overflow ring at 4096-slot deques. A lowered program in which one task spawns n
+detached children in one round (gtap_amd/lower/examples/burst.gtap.cu) publishes more than
+half of its wave's deque at the round's end, so the wave hands its oldest tasks to the
+overflow ring and idle waves take them back: every child runs once (the counters it adds to
+are exact) and every task moved to the ring is taken again, launch after launch. A star
+graph's BFS with 4 wavefront workers, where the deques fill and drain quickly, stays
+bit-exact."""
 import numpy as np
 import pytest
 
@@ -20,6 +21,13 @@ def few_workers_small_deques():
     yield
     assert gtap_amd.gtap_finalize() == 0
     gtap_amd.gtap_set_config()
+
+
+@pytest.fixture(scope="module")
+def burst_module():
+    import ctypes
+    from gtap_amd import build as gbuild
+    return gtap_amd.EntryModule(gbuild.lowered_lib("burst"), "burst"), ctypes
 
 
 def ring_counts():
@@ -45,7 +53,7 @@ def test_star_overflows_into_the_ring(queues, leaves):
     assert st["error_code"] == 0
     spilled, taken = ring_counts()
     print(f"ring: {spilled} tasks spilled, {taken} taken", flush=True)
-    assert spilled > 0 and taken == spilled
+    assert taken == spilled
     expect = np.ones(leaves + 1, dtype=np.int32)
     expect[0] = 0
     assert np.array_equal(depth, expect)
@@ -57,3 +65,26 @@ def test_joined_program_after_ring_use():
     print("case fib(25) after ring use", flush=True)
     v, st = gtap_amd.fib(25)
     assert v == oracle.fib(25) and st["error_code"] == 0
+
+
+@pytest.mark.parametrize("n", [3050, 2000, 1])
+def test_burst_spills_into_the_ring(burst_module, n):
+    m, ctypes = burst_module
+    u64 = ctypes.c_uint64
+    total_spilled = 0
+    for rep in range(3):
+        print(f"case burst n={n} rep={rep} deque capacity 4096", flush=True)
+        m.set_symbol("d_count", 0, u64)
+        m.set_symbol("d_sum", 0, u64)
+        assert m.launch("burst_kernel", [ctypes.c_int], n) == 0
+        assert gtap_amd.gtap_get_runtime_error_code() == 0
+        assert m.get_symbol("d_count", u64) == n
+        assert m.get_symbol("d_sum", u64) == n * (n - 1) // 2
+        spilled, taken = ring_counts()
+        print(f"ring: {spilled} tasks spilled, {taken} taken", flush=True)
+        assert taken == spilled
+        total_spilled += spilled
+    if n == 3050:
+        # 2986 published at the round's end, past the 2048 half: thieves would have to take
+        # 939 of them before the wave's own claim for a launch not to spill
+        assert total_spilled > 0
