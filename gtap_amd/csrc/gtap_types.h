@@ -129,9 +129,9 @@ typedef struct gtap_config {
     uint32_t reserved[2];
 } gtap_config;
 
-/* Called by gtap_initialize and gtap_reset (and at registration when the runtime is already
- * initialized) with the runtime descriptor (`bytes` = its size): a lowered module copies it to
- * its device symbol, so its host-launched entry kernels reach the runtime's pools. */
+/* Called by gtap_initialize and gtap_reset with the runtime descriptor (`bytes` = its size): a
+ * lowered module copies it to its device symbol, so its host-launched entry kernels reach the
+ * runtime's pools. A module loaded after gtap_initialize gets it at the next gtap_reset. */
 typedef int (*gtap_entry_setup_fn)(const void* desc, uint64_t bytes);
 
 #ifdef __cplusplus

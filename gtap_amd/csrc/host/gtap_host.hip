@@ -557,14 +557,14 @@ int gtap_reset(void) {
     return 0;
 }
 
+// Only records the module: the descriptor goes to it at the next gtap_initialize or gtap_reset.
+// A module registers from its static initializer, which may run before the HIP runtime has
+// registered that module's device symbols, so no symbol may be written from here.
 int gtap_register_entry_module(gtap_entry_setup_fn setup) {
     if (setup == nullptr) return invalid_argument("gtap_register_entry_module: null setup function");
     std::vector<gtap_entry_setup_fn>& mods = entry_modules();
     if (std::find(mods.begin(), mods.end(), setup) == mods.end()) mods.push_back(setup);
-    Runtime& R = rt();
-    if (!R.initialized) return 0;  // gtap_initialize pushes the descriptor
-    const GtapDev d = entry_dev();
-    return setup(&d, sizeof(d));
+    return 0;
 }
 
 int gtap_get_runtime_error_code(int* code) {

@@ -35,7 +35,8 @@ int gtap_reset(void);
  * launches itself (the reference's `kernel<<<GTAP_GRID_SIZE, GTAP_BLOCK_SIZE>>>(...)` with
  * `#pragma gtap entry` inside, examples/fib/fib.cu:21-46). gtap_initialize and gtap_reset
  * reset the runtime state, arm it for one such launch, and call `setup` with the runtime
- * descriptor; the generated code calls this from a static initializer. No reference
+ * descriptor (a module loaded after gtap_initialize gets it at the next gtap_reset); the
+ * generated code calls this from a static initializer. No reference
  * counterpart: the reference runtime is compiled into the program's own translation unit. */
 int gtap_register_entry_module(gtap_entry_setup_fn setup);
 /* Runtime knobs that the reference fixes at compile time with GTAP_* macros
