@@ -1200,6 +1200,75 @@ __device__ __forceinline__ void complete_job(const GtapDev& dev, uint32_t anchor
     if (dev.term_mode == GTAP_TERM_JOBS && (uint32_t)done == dev.njobs) st32(dev.ctl + CTL_DONE * 32, 1);
 }
 
+// A random victim other than `me` for one probe lane. With xcd_steal, even lanes pick one on
+// this wave's XCD: workgroups are dealt to the 8 XCDs round-robin, so workgroup 8 k + x runs
+// on XCD x; 4 waves per workgroup.
+__device__ __forceinline__ uint32_t pick_victim(const GtapDev& dev, uint32_t me, uint32_t& rng, int lane) {
+    uint32_t victim = xorshift32(rng) % (dev.nworkers - 1);
+    if (victim >= me) victim += 1;
+    if (dev.xcd_steal && (lane & 1) == 0 && dev.nworkers >= 64 && (dev.nworkers & 3u) == 0) {
+        const uint32_t nb = dev.nworkers >> 2, x = (me >> 2) & 7u;
+        const uint32_t nbx = (nb - x + 7u) >> 3;  // workgroups on XCD x
+        const uint32_t vw = (((xorshift32(rng) % nbx) << 3) + x) * 4u + (victim & 3u);
+        if (vw != me && vw < dev.nworkers) victim = vw;
+    }
+    return victim;
+}
+
+// With several EPAQ queues, one probe of every queue at once: lane l probes queue
+// (cq + l) % Q of a random victim (probe_lanes victims per queue, at most 64 probes), and one
+// CAS claims up to half (<= steal_batch; 1 on the wave queue) of a non-empty one. The thief
+// finds work in one round trip instead of one per queue. Returns the number stolen into
+// `tid_out` and the queue in `q_out`.
+template <class P>
+__device__ __forceinline__ uint32_t try_steal_any(const GtapDev& dev, uint32_t me, uint32_t cq, uint32_t& rng,
+                                                  uint32_t& tid_out, bool& idle_registered, LoopStats& st,
+                                                  uint32_t& q_out) {
+    constexpr uint32_t Q = (uint32_t)P::kQueues;
+    const int lane = lane_id();
+    if (dev.nworkers < 2) return 0;
+    uint32_t probes = dev.probe_lanes * Q;
+    if (probes > (uint32_t)kWave) probes = kWave;
+    uint32_t victim = 0, q = (cq + (uint32_t)lane) % Q;
+    uint64_t wv = 0;
+    int32_t sz = 0;
+    if ((uint32_t)lane < probes) {
+        victim = pick_victim(dev, me, rng, lane);
+        wv = ld64(dq_word_ptr(dev, q, victim));
+        sz = (int32_t)((uint32_t)wv - (uint32_t)(wv >> 32));
+    }
+    uint64_t cand = ballot(sz > 0);
+    if (cand == 0) return 0;
+    const uint32_t rot = uniform(xorshift32(rng)) & 63u;
+    const uint64_t rotated = rot ? ((cand >> rot) | (cand << (64 - rot))) : cand;
+    const int src = (first_lane(rotated) + (int)rot) & 63;
+    victim = shfl(victim, src);
+    wv = shfl(wv, src);
+    sz = shfl(sz, src);
+    q = uniform(shfl(q, src));
+    uint32_t k = (uint32_t)(sz + 1) / 2;
+    const uint32_t batch_cap = ((int)q == HasWaveQueue<P>::value) ? 1u : dev.steal_batch;
+    if (k > batch_cap) k = batch_cap;
+    if (k == 0) k = 1;
+    if (dev.term_mode == GTAP_TERM_QUIESCE && idle_registered) {
+        if (lane == 0) atomic_add(dev.ctl + CTL_IDLE * 32, -1);
+        idle_registered = false;
+    }
+    int ok = 0;
+    if (lane == 0) {
+        uint64_t expected = wv;
+        ok = atomic_cas(dq_word_ptr(dev, q, victim), expected, wv + ((uint64_t)k << 32)) ? 1 : 0;
+    }
+    ok = shfl((int32_t)ok, 0);
+    st.attempts += 1;
+    if (!ok) return 0;
+    const uint32_t t = (uint32_t)(wv >> 32);
+    if ((uint32_t)lane < k) tid_out = ld32(dq_slot_ptr(dev, q, victim, t + (uint32_t)lane));
+    st.steals += 1;
+    q_out = q;
+    return k;
+}
+
 // Try to steal from random victims on queue q. Returns the number of ids stolen into `tid_out`.
 template <class P>
 __device__ __forceinline__ uint32_t try_steal(const GtapDev& dev, uint32_t me, uint32_t q, uint32_t& rng,
@@ -1213,16 +1282,7 @@ __device__ __forceinline__ uint32_t try_steal(const GtapDev& dev, uint32_t me, u
     uint64_t wv = 0;
     int32_t sz = 0;
     if ((uint32_t)lane < dev.probe_lanes) {
-        victim = xorshift32(rng) % (dev.nworkers - 1);
-        if (victim >= me) victim += 1;
-        if (dev.xcd_steal && (lane & 1) == 0 && dev.nworkers >= 64 && (dev.nworkers & 3u) == 0) {
-            // a victim on this wave's XCD: workgroups are dealt to the 8 XCDs round-robin, so
-            // workgroup 8 k + x runs on XCD x; 4 waves per workgroup
-            const uint32_t nb = dev.nworkers >> 2, x = (me >> 2) & 7u;
-            const uint32_t nbx = (nb - x + 7u) >> 3;  // workgroups on XCD x
-            const uint32_t vw = (((xorshift32(rng) % nbx) << 3) + x) * 4u + (victim & 3u);
-            if (vw != me && vw < dev.nworkers) victim = vw;
-        }
+        victim = pick_victim(dev, me, rng, lane);
         wv = ld64(dq_word_ptr(dev, q, victim));
         sz = (int32_t)((uint32_t)wv - (uint32_t)(wv >> 32));
     }
@@ -1302,6 +1362,8 @@ __device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtxOf<P>* w,
 //   kSchedCL  the owner pops and thieves steal one task per CAS; a wave fills its batch
 //             with up to 64 such single-item operations in a row.
 enum : int { kSchedWS = 0, kSchedGQ = 1, kSchedCL = 2 };
+// Work stealing with several EPAQ queues probes all of them in one steal attempt (try_steal_any).
+constexpr bool kStealAnyQueue = true;
 constexpr uint32_t kGqMargin = 1024;  // ring slots kept free beyond the head
 
 // Control words of global queue q: head, reservation counter and committed tail, each on its
@@ -1539,12 +1601,18 @@ __device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& arg
                 batch_q = -1;
             }
             if (n == 0) {
+                if constexpr (kStealAnyQueue && Sched == kSchedWS && Q > 1) {
+                    uint32_t sq = 0;
+                    n = try_steal_any<P>(dev, me, cq, rng, my_tid, idle_registered, st, sq);
+                    if (n) batch_q = (int)sq;
+                } else {
 #pragma unroll
-                for (int i = 0; i < Q && n == 0; ++i) {
-                    const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
-                    if constexpr (Sched == kSchedGQ) n = gq_pop<P>(dev, q, my_tid, idle_registered, st);
-                    else n = try_steal<P>(dev, me, q, rng, my_tid, idle_registered, st, Sched == kSchedCL ? 1u : (uint32_t)kWave);
-                    if (n) batch_q = (int)q;
+                    for (int i = 0; i < Q && n == 0; ++i) {
+                        const uint32_t q = (cq + (uint32_t)i) % (uint32_t)Q;
+                        if constexpr (Sched == kSchedGQ) n = gq_pop<P>(dev, q, my_tid, idle_registered, st);
+                        else n = try_steal<P>(dev, me, q, rng, my_tid, idle_registered, st, Sched == kSchedCL ? 1u : (uint32_t)kWave);
+                        if (n) batch_q = (int)q;
+                    }
                 }
             }
             // the overflow rings (tasks that full deques handed off), every fourth idle try
