@@ -213,7 +213,7 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, rand_meas=None, reps: int = 3) -> di
             if st != 0:
                 raise RuntimeError(f"gtap_bfs failed: {st}")
             ms.append(s.kernel_ms)
-        res[q] = (statistics.median(ms[1:]), s.tasks_executed)
+        res[q] = (statistics.median(ms[1:]), s.tasks_executed, s.rounds, s.steal_attempts)
     ddep.to_array(depth)
     for b in (drow, dcol, ddep):
         b.free()
@@ -230,7 +230,7 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, rand_meas=None, reps: int = 3) -> di
     tight[0] = True
     if depth[0] != 0 or np.any(du[rs] > dv[rs] + 1) or not np.all(tight[reached]):
         raise RuntimeError("bfs rmat22: the distances break the BFS invariants")
-    med4, tasks4 = res[4]
+    med4, tasks4 = res[4][0], res[4][1]
     gbps = roofline.bfs_bytes(vr, er) / (med4 / 1e3) / 1e9
     log(f"bfs rmat22: epaq4 {med4:.3f} ms, 1 queue {res[1][0]:.3f} ms, {gbps:.0f} GB/s algorithmic")
     # request-rate roofline: the expansion's random accesses (a depth probe per reached edge,
@@ -250,6 +250,7 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, rand_meas=None, reps: int = 3) -> di
             "request_roofline": request_roofline,
             "epaq_speedup": res[1][0] / med4, "reached_vertices": vr, "reached_edges": er,
             "edges_per_s": er / (med4 / 1e3), "tasks_executed_epaq4": tasks4, "tuned": tuned,
+            "counts": {f"{q}_queues": {"tasks_executed": r[1], "rounds": r[2], "steal_attempts": r[3]} for q, r in res.items()},
             "roofline": {"bound": "hbm", "achieved": gbps, "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
                          "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas,
                          "algorithmic_bytes": roofline.bfs_bytes(vr, er), "traffic": load_traffic("bfs_rmat22")}}
