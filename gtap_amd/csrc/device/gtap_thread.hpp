@@ -937,6 +937,12 @@ struct WaveTask {
     __device__ __forceinline__ void spawn_detached(uint32_t f, uint32_t q, const A& a) {
         if (lane == 0) spawn_record<P>(*dev, w, me, tid, f, q, kNoParent, 0u, a);
     }
+    // Every calling lane spawns a detached child of its own (function, queue and arguments
+    // per lane); any subset of the lanes may call, in divergent control flow too.
+    template <typename A>
+    __device__ __forceinline__ void spawn_detached_lane(uint32_t f, uint32_t q, const A& a) {
+        spawn_record<P>(*dev, w, me, tid, f, q, kNoParent, 0u, a);
+    }
     __device__ __forceinline__ bool taskwait(uint32_t next_state, uint32_t q_after) {
         static_assert(!NoTaskwait<P>::value, "a program declared kNoTaskwait calls taskwait");
         if (q_after >= (uint32_t)Q) q_after = 0;
@@ -1491,7 +1497,7 @@ __device__ __forceinline__ void backoff_sleep(uint32_t level) {
 // Prof: the kernel records the timeline (gtap_set_profiling). It is a separate build of the
 // kernel, because the timeline's loop state costs registers every launch would pay for.
 template <class P, int Sched, bool Prof>
-__device__ void thread_scheduler(const GtapDev& dev, const typename P::Args& args, WaveCtxOf<P>* w,
+__device__ __forceinline__ void thread_scheduler(const GtapDev& dev, const typename P::Args& args, WaveCtxOf<P>* w,
                                  void* wave_shared) {
     constexpr int Q = P::kQueues;
     constexpr int WQ = HasWaveQueue<P>::value;

@@ -7,12 +7,13 @@
 //   #pragma gtap function
 //   __device__ void bfs(int v, int s, int e, int d) {
 //       if (depth[v] < d) return;                      // a shorter path already re-expanded v
-//       if (e - s > SPLIT) {                           // split a long adjacency range in halves
-//           int m = s + (e - s) / 2;
-//           #pragma gtap task queue(bucket(e - s))
-//           bfs(v, s, m, d);
-//           #pragma gtap task queue(bucket(e - s))
-//           bfs(v, m, e, d);
+//       if (e - s > SPLIT) {                           // split a long adjacency range into
+//           int k = min(8, ceil((e - s) / SPLIT));     // k near-equal pieces
+//           for (int i = 0; i < k; ++i) {
+//               int ps = s + (e - s) * i / k, pe = s + (e - s) * (i + 1) / k;
+//               #pragma gtap task queue(bucket(pe - ps))
+//               bfs(v, ps, pe, d);
+//           }
 //           return;
 //       }
 //       for (int i = s; i < e; ++i) {
@@ -27,6 +28,16 @@
 // bucket(deg) = 0 for deg < 8, 1 for < 64, 2 for < 512, 3 otherwise (4 EPAQ queues); with one
 // queue every task goes to queue 0. No task waits for its children (detached spawns); the
 // launch ends by quiescence. Distances are the BFS hop counts whatever the schedule.
+//
+// With 4 queues, queue 3 (ranges of 512 edges and more) is a wave queue: its task runs on a
+// whole wavefront, which reads the range 512 edges at a time (lane l takes edges b + l + 64 j,
+// so each column load is one contiguous 256-byte read) and spawns every improved neighbour
+// from the lane that found it. A range above kWaveSplitEdges (or an eighth of the deque
+// capacity, if smaller) is first cut into up to 64 near-equal pieces, one spawned per lane.
+// A hub's adjacency list is thus scanned by a few wave rounds instead of a tree of 64-edge
+// lane tasks and their split tasks. One queue keeps the lane program for every range. (The
+// reference cuts a long list into 32-edge tasks all spawned by one task,
+// asynchronous_bfs_thread.cu:25-33; a bounded fan-out keeps one lane's spawn loop short.)
 #include "../device/gtap_abi.hpp"
 #include "../device/gtap_block.hpp"
 #include "../host/gtap_launch.hpp"
@@ -54,17 +65,27 @@ __host__ __device__ __forceinline__ uint32_t degree_bucket(int32_t deg) {
     return deg < 8 ? 0u : (deg < 64 ? 1u : (deg < 512 ? 2u : 3u));
 }
 
+constexpr int32_t kSplitFanout = 8;         // pieces per split of a lane task's range
+constexpr int32_t kWaveSplitEdges = 2048;  // a wave task scans at most this many edges (and cap / 8)
+constexpr int kWaveChunk = 8;               // edges per lane per step of a wave scan
+
 template <int NQ>
 struct Program {
     static constexpr int kQueues = NQ;
+    static constexpr int kWaveQueue = NQ == 4 ? 3 : -1;
     static constexpr bool kNoTaskwait = true;  // detached spawns only
     static constexpr int kWavesPerSimd = 3;    // 3072 wavefront workers with 4 queues too (4 per SIMD measured slower)
     static constexpr int kFrameWords = 4;
     static constexpr uint32_t kMaxFunc = F_BFS;
     static constexpr uint32_t kFrameBytes = 16;
     using Args = BfsArgs;
+    struct WaveShared {
+        int32_t unused;
+    };
     template <class L>
     __device__ __forceinline__ static void run(L& lane);
+    template <class W>
+    __device__ __forceinline__ static void run_wave(W& wt);
 };
 
 template <int NQ>
@@ -88,10 +109,15 @@ __device__ __forceinline__ void bfs_task(L& lane) {
         return;
     }
     if (e - s > g.split) {
-        const int32_t m = s + (e - s) / 2;
-        const uint32_t q = route<NQ>(e - s);
-        __gtap_spawn_task_detached(lane, F_BFS, q, BfsArg{v, s, m, d});
-        __gtap_spawn_task_detached(lane, F_BFS, q, BfsArg{v, m, e, d});
+        // up to kSplitFanout near-equal pieces, each routed by its own length
+        const int64_t len = (int64_t)e - s;
+        int32_t k = (int32_t)((len + g.split - 1) / g.split);
+        if (k > kSplitFanout) k = kSplitFanout;
+        for (int32_t i = 0; i < k; ++i) {
+            const int32_t ps = s + (int32_t)(len * i / k);
+            const int32_t pe = s + (int32_t)(len * (i + 1) / k);
+            __gtap_spawn_task_detached(lane, F_BFS, route<NQ>(pe - ps), BfsArg{v, ps, pe, d});
+        }
         __gtap_finish_task(lane);
         return;
     }
@@ -154,6 +180,93 @@ template <int NQ>
 template <class L>
 __device__ __forceinline__ void Program<NQ>::run(L& lane) {
     bfs_task<NQ>(lane);
+}
+
+// The wave form of the same task (queue 3 with 4 queues): all 64 lanes enter with one (v, s, e, d).
+template <int NQ>
+template <class W>
+__device__ __forceinline__ void Program<NQ>::run_wave(W& wt) {
+    const BfsArgs& g = *wt.args;
+    const int32_t v = wt.template arg<int32_t>(0);
+    const int32_t s = wt.template arg<int32_t>(4);
+    const int32_t e = wt.template arg<int32_t>(8);
+    const int32_t d = wt.template arg<int32_t>(12);
+    if (wt.func != F_BFS) {
+        if (wt.lane == 0) GTAP_DEVICE_ERROR(*wt.dev, GTAP_ERROR_INVALID_FUNCTION, wt.tid, wt.queue, wt.func, F_BFS);
+        wt.finish();
+        return;
+    }
+    if ((int32_t)uniform((uint32_t)ld32(reinterpret_cast<const int32_t*>(g.depth) + v)) < d) {  // stale
+        wt.finish();
+        return;
+    }
+    // a scan spawns at most one task per edge in one round: an eighth of the deque capacity
+    // stays within what a round may append (the deque keeps up to half its capacity published
+    // and a margin free)
+    const int32_t wsplit = kWaveSplitEdges < (int32_t)(wt.dev->cap / 8) ? kWaveSplitEdges : (int32_t)(wt.dev->cap / 8);
+    if (e - s > wsplit) {
+        // up to 64 near-equal pieces, lane i spawning piece i
+        const int64_t len = (int64_t)e - s;
+        int32_t k = (int32_t)((len + wsplit - 1) / wsplit);
+        if (k > kWave) k = kWave;
+        if (wt.lane < k) {
+            const int32_t ps = s + (int32_t)(len * wt.lane / k);
+            const int32_t pe = s + (int32_t)(len * (wt.lane + 1) / k);
+            wt.spawn_detached_lane(F_BFS, route<NQ>(pe - ps), BfsArg{v, ps, pe, d});
+        }
+        wt.finish();
+        return;
+    }
+    const int32_t nd = d + 1;
+    const int32_t l = wt.lane;
+    for (int32_t b = s; b < e; b += kWave * kWaveChunk) {
+        int32_t u[kWaveChunk];
+#pragma unroll
+        for (int j = 0; j < kWaveChunk; ++j) {
+            const int32_t i = b + j * kWave + l;
+            u[j] = (i < e) ? g.col[i] : -1;
+        }
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < kWaveChunk; ++j) {
+            if (u[j] >= g.nv || u[j] < -1) {
+                bad = true;
+                u[j] = -1;
+            }
+        }
+        if (ballot(bad)) {
+            if (bad) GTAP_DEVICE_ERROR(*wt.dev, GTAP_ERROR_INVALID_ARGUMENT, wt.tid, -1, -1, g.nv);
+            break;
+        }
+        int32_t old[kWaveChunk];
+#pragma unroll
+        for (int j = 0; j < kWaveChunk; ++j) old[j] = (u[j] >= 0) ? g.depth[u[j]] : 0;
+#pragma unroll
+        for (int j = 0; j < kWaveChunk; ++j) {
+            if (u[j] >= 0 && old[j] > nd)
+                old[j] = __hip_atomic_fetch_min(g.depth + u[j], nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                old[j] = 0;
+        }
+        uint32_t hits = 0;
+#pragma unroll
+        for (int j = 0; j < kWaveChunk; ++j) hits |= (old[j] > nd ? 1u : 0u) << j;
+        while (hits) {
+            const int j = __builtin_ctz(hits);
+            hits &= hits - 1u;
+            int32_t uj = u[0];
+#pragma unroll
+            for (int k = 1; k < kWaveChunk; ++k) uj = (j == k) ? u[k] : uj;
+            const int32_t us = g.row[uj];
+            const int32_t ue = g.row[uj + 1];
+            if (us < 0 || ue < us || (int64_t)ue > g.ne) {
+                GTAP_DEVICE_ERROR(*wt.dev, GTAP_ERROR_INVALID_ARGUMENT, wt.tid, -1, uj, g.nv);
+                continue;
+            }
+            wt.spawn_detached_lane(F_BFS, route<NQ>(ue - us), BfsArg{uj, us, ue, nd});
+        }
+    }
+    wt.finish();
 }
 
 }  // namespace bfs
