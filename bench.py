@@ -192,11 +192,9 @@ def bench_cilksort(gtap_amd, roofline, hbm_meas, reps: int = 3) -> dict:
 
 def bench_bfs(gtap_amd, roofline, hbm_meas, rand_meas=None, reps: int = 3) -> dict:
     """Asynchronous BFS on the R-MAT scale-22 graph (edge factor 16, seed 1), EPAQ with 4
-    degree-bucket queues vs one queue; CSR resident on the GPU."""
+    degree-bucket queues vs one queue, each at its own tuned settings; CSR resident on the GPU."""
     import ctypes
     import numpy as np
-    tuned = use_tuned("bfs")
-    split = int(tuned.get("split", 64))
     row, col = gtap_amd.rmat_csr(22, 16, 1)
     nv = row.size - 1
     drow = gtap_amd.DeviceBuffer.from_array(row)
@@ -204,8 +202,11 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, rand_meas=None, reps: int = 3) -> di
     ddep = gtap_amd.DeviceBuffer(nv * 4)
     L = gtap_amd.lib()
     res = {}
+    tuned = {}
     depth = np.empty(nv, dtype=np.int32)
     for q in (4, 1):
+        tuned[q] = use_tuned("bfs" if q == 4 else "bfs1")
+        split = int(tuned[q].get("split", 64))
         ms = []
         for _ in range(reps + 1):
             s = gtap_amd.RunStats()
@@ -249,7 +250,8 @@ def bench_bfs(gtap_amd, roofline, hbm_meas, rand_meas=None, reps: int = 3) -> di
     return {"kernel_ms_median_epaq4": med4, "kernel_ms_median_1queue": res[1][0],
             "request_roofline": request_roofline,
             "epaq_speedup": res[1][0] / med4, "reached_vertices": vr, "reached_edges": er,
-            "edges_per_s": er / (med4 / 1e3), "tasks_executed_epaq4": tasks4, "tuned": tuned,
+            "edges_per_s": er / (med4 / 1e3), "tasks_executed_epaq4": tasks4,
+            "tuned": {"4_queues": tuned[4], "1_queue": tuned[1]},
             "counts": {f"{q}_queues": {"tasks_executed": r[1], "rounds": r[2], "steal_attempts": r[3]} for q, r in res.items()},
             "roofline": {"bound": "hbm", "achieved": gbps, "peak": roofline.MI355X_HBM_GBPS, "unit": "GB/s",
                          "frac": gbps / roofline.MI355X_HBM_GBPS, "measured_copy_peak": hbm_meas,

@@ -122,8 +122,8 @@ def _median_ms(fn: Callable[[], Tuple[object, float]], reps: int) -> Tuple[float
 def workload(name: str, reps: int = 3) -> Tuple[Measure, Dict[str, Sequence[int]]]:
     """A measure function and its search space for a built-in workload:
     fibN / nqN (thread mode, no cutoff), nqcutN (spawn cutoff row), cilksort (2^26 keys, block
-    mode, leaf and merge cutoffs), bfs (R-MAT scale 22, split), spmv (row cutoff), stream (4096
-    jobs, sort jobs' merge leaf)."""
+    mode, leaf and merge cutoffs), bfs / bfs1 (R-MAT scale 22 with 4 EPAQ queues / one queue,
+    split), spmv (row cutoff), stream (4096 jobs, sort jobs' merge leaf)."""
     import gtap_amd
     space = dict(DEFAULT_SPACE)
 
@@ -176,18 +176,19 @@ def workload(name: str, reps: int = 3) -> Tuple[Measure, Dict[str, Sequence[int]
             return _median_ms(one, reps)
         return measure, space
 
-    if name == "bfs":
-        import numpy as np
+    if name in ("bfs", "bfs1"):
         row, col = gtap_amd.rmat_csr(22, 16, 1)
+        queues = 4 if name == "bfs" else 1
         space = dict(DEFAULT_SPACE)
         space["split"] = (64, 32, 128, 256)
-        space["epaq_publish"] = (0, 1)
+        if queues > 1:
+            space["epaq_publish"] = (0, 1)
 
         def measure(p):
             apply(p)
 
             def one():
-                dist, st = gtap_amd.bfs(row, col, 0, 4, p.get("split", 64))
+                dist, st = gtap_amd.bfs(row, col, 0, queues, p.get("split", 64))
                 return hash(dist.tobytes()), st["kernel_ms"]
             return _median_ms(one, reps)
         return measure, space
@@ -214,7 +215,7 @@ def workload(name: str, reps: int = 3) -> Tuple[Measure, Dict[str, Sequence[int]
 
 def main(argv: Optional[List[str]] = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--workload", default="nq14", help="fibN, nqN, nqcutN, cilksort, bfs, spmv or stream")
+    ap.add_argument("--workload", default="nq14", help="fibN, nqN, nqcutN, cilksort, bfs, bfs1, spmv or stream")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--passes", type=int, default=2)
     ap.add_argument("--out", default=None, help="JSON file for the best parameters and the table")
