@@ -1322,7 +1322,8 @@ __device__ __forceinline__ uint32_t try_steal(const GtapDev& dev, uint32_t me, u
     return k;
 }
 
-// Pop up to 64 of this wave's newest published tasks from queue q.
+// Pop up to 64 (max_items) of this wave's newest published tasks from queue q. The queue must
+// hold no private entries (its tail equals its bottom): both move to the new bottom.
 template <class P>
 __device__ __forceinline__ uint32_t try_pop(const GtapDev& dev, WaveCtxOf<P>* w, uint32_t me, uint32_t q,
                                             uint32_t& tid_out, uint32_t max_items = kWave) {
@@ -1837,7 +1838,7 @@ __device__ __forceinline__ void thread_scheduler(const GtapDev& dev, const typen
             }
         }
         const uint32_t keep_cap = ((int)kmax == WQ) ? 1u : (uint32_t)kWave;
-        const uint32_t keep = cmax < keep_cap ? cmax : keep_cap;
+        uint32_t keep = cmax < keep_cap ? cmax : keep_cap;
         uint32_t next_tid = 0, next_slot = 0;
         bool next_lds = false;
         if (keep) {
@@ -1952,6 +1953,27 @@ __device__ __forceinline__ void thread_scheduler(const GtapDev& dev, const typen
                             vst(&w->ring_spilled, vld(&w->ring_spilled) + (uint32_t)(r >> 32));
                         }
                     }
+                }
+            }
+        }
+        // A kept batch of fewer than 64 tasks (the queue then has no private entries left) is
+        // topped up with the wave's own newest published tasks of the same queue, one CAS on
+        // its deque word, so the next round runs up to 64 tasks instead of waiting until the
+        // wave runs dry. Their records are in HBM and load at the round's start.
+        if constexpr (Sched == kSchedWS) {
+            if (keep < (uint32_t)kWave && (int)kmax != WQ) {
+                uint32_t ptid = 0;
+                const uint32_t k = try_pop<P>(dev, w, me, kmax, ptid, (uint32_t)kWave - keep);
+                if (k) {
+                    const int src = lane - (int)keep;
+                    const uint32_t v = shfl(ptid, src < 0 ? 0 : src);
+                    if ((uint32_t)lane >= keep && (uint32_t)lane < keep + k) {
+                        next_tid = v;
+                        next_lds = false;
+                    }
+                    keep += k;
+                    st.pops += 1;
+                    if (lane == 0) vst(&w->cur_queue, kmax);
                 }
             }
         }
