@@ -391,13 +391,20 @@ __device__ __forceinline__ uint64_t shfl(uint64_t v, int src) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Volatile accessors for wave-private LDS state. Lanes of one wave update this state
-// inside divergent code; volatile keeps the compiler from reusing a value loaded
-// before another lane's store.
+// Volatile accessors for wave-private LDS state (the wave contexts, which live in __shared__
+// arrays only). Lanes of one wave update this state inside divergent code; volatile keeps the
+// compiler from reusing a value loaded before another lane's store. The pointer is cast to the
+// LDS address space: through a generic pointer a volatile access becomes a flat instruction,
+// which the compiler follows with a wait for every outstanding global load and store of the
+// wave (s_waitcnt vmcnt(0)); an LDS access waits for nothing but itself.
 template <typename T>
-__device__ __forceinline__ T vld(const T* p) { return *reinterpret_cast<const volatile T*>(p); }
+__device__ __forceinline__ T vld(const T* p) {
+    return *(const volatile __attribute__((address_space(3))) T*)(p);
+}
 template <typename T>
-__device__ __forceinline__ void vst(T* p, T v) { *reinterpret_cast<volatile T*>(p) = v; }
+__device__ __forceinline__ void vst(T* p, T v) {
+    *(volatile __attribute__((address_space(3))) T*)(p) = v;
+}
 
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
 
