@@ -201,6 +201,18 @@ __device__ __forceinline__ void st32(int32_t* p, int32_t v) {
 __device__ __forceinline__ void st64(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// The same agent-coherent access for a pointer known to address device (global) memory, such
+// as key arrays whose addresses travel in task frames: the compiler cannot tell the address
+// space of such a pointer and would use a flat instruction, which also occupies the LDS
+// counter, so the LDS waits of the code around it wait for these loads too. Never pass an LDS
+// or stack address.
+__device__ __forceinline__ uint32_t gld32(const uint32_t* p) {
+    return __hip_atomic_load((const __attribute__((address_space(1))) uint32_t*)p, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gst32(uint32_t* p, uint32_t v) {
+    __hip_atomic_store((__attribute__((address_space(1))) uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ int32_t ld32_sys(const int32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }

@@ -378,13 +378,13 @@ __device__ __forceinline__ void record_from_desc(const GtapDev& dev, __amdgpu_bu
 template <class P, class W>
 __device__ __forceinline__ void stage_from_desc(W* w, uint32_t at, uint32_t id, const uint32_t* d) {
     constexpr int FW = CtxShape<P>::kFW, FWr = CtxShape<P>::kFWr;
-    volatile uint32_t* dst = &w->stage[at][0];
-    dst[0] = vld(&d[1]);
-    dst[1] = vld(&d[2]);
-    dst[2] = 0u;
-    dst[3] = vld(&d[0]);
+    uint32_t* dst = &w->stage[at][0];
+    vst(&dst[0], vld(&d[1]));
+    vst(&dst[1], vld(&d[2]));
+    vst(&dst[2], 0u);
+    vst(&dst[3], vld(&d[0]));
 #pragma unroll
-    for (int i = 0; i < FWr; ++i) dst[4 + i] = (i < FW) ? vld(&d[3 + (i < FW ? i : 0)]) : 0u;
+    for (int i = 0; i < FWr; ++i) vst(&dst[4 + i], (i < FW) ? vld(&d[3 + (i < FW ? i : 0)]) : 0u);
     vst(&w->stage_id[at], id);
 }
 

@@ -71,8 +71,9 @@ struct Program {
     __device__ __forceinline__ static void run(B& blk);
 };
 
-__device__ __forceinline__ uint32_t ldk(const uint32_t* p) { return ld32(p); }
-__device__ __forceinline__ void stk(uint32_t* p, uint32_t v) { st32(p, v); }
+// keys live in device memory (the caller's arrays and the scratch buffer)
+__device__ __forceinline__ uint32_t ldk(const uint32_t* p) { return gld32(p); }
+__device__ __forceinline__ void stk(uint32_t* p, uint32_t v) { gst32(p, v); }
 
 // gtap_cilksort.cu:40-50: largest index with x[index] <= val, or -1.
 __device__ __forceinline__ int64_t binary_search_gm(const uint32_t* x, int64_t len, uint32_t val) {

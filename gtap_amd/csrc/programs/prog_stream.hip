@@ -218,8 +218,9 @@ __device__ __forceinline__ void Program::run(L& lane) {
 
 // ---------------------------------------------------------------- wave tasks
 
-__device__ __forceinline__ uint32_t ldk(const uint32_t* p) { return ld32(p); }
-__device__ __forceinline__ void stk(uint32_t* p, uint32_t v) { st32(p, v); }
+// keys live in device memory (the caller's arrays and the scratch buffer)
+__device__ __forceinline__ uint32_t ldk(const uint32_t* p) { return gld32(p); }
+__device__ __forceinline__ void stk(uint32_t* p, uint32_t v) { gst32(p, v); }
 
 // Sort src[0..n) (n <= 1024) into dst with a bitonic network over 1024 LDS slots.
 template <class W>
