@@ -1,5 +1,5 @@
-"""Deque overflow: with a small deque capacity, a deque that fills up hands its oldest
-published tasks (and, mid-round, detached children) to the per-queue overflow ring instead of
+"""Deque overflow: with a small deque capacity, a deque holding more than half its capacity at
+the end of a round hands its oldest published tasks to the per-queue overflow ring instead of
 failing with GTAP_ERROR_QUEUE_OVERFLOW (the reference traps there:
 runtime/detail/thread/gtap_runtime_thread.cuh:42-45). Results stay bit-exact against the
 oracle; every task moved to a ring is taken from it again."""
