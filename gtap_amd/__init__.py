@@ -818,7 +818,7 @@ class Stream:
         return res, lat, tot.value
 
     def set_merge_leaf(self, merge_leaf: int) -> None:
-        """Sort jobs merge runs of at most merge_leaf keys (1024..16384) on one wave."""
+        """Sort jobs merge runs of at most merge_leaf keys (512..16384) on one wave."""
         _check(lib().gtap_stream_set_merge_leaf(self.h, int(merge_leaf)), "gtap_stream_set_merge_leaf")
 
     def exec_ms(self):

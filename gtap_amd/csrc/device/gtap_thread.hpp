@@ -106,6 +106,17 @@ struct NoTaskwait<P, decltype((void)P::kNoTaskwait)> {
     static constexpr bool value = P::kNoTaskwait;
 };
 
+// Staging entries per wave (evicted records, plus a round's suspended tasks for programs that
+// wait for children): two rounds' worth by default; a program short of LDS may ask for 64
+// (P::kStageEntries), and its evicted records are then written out at once instead of staged.
+template <class P, class = void>
+struct StageEntries {
+    static constexpr int value = NoTaskwait<P>::value ? kWave : 2 * kWave;
+};
+template <class P>
+struct StageEntries<P, decltype((void)P::kStageEntries)> {
+    static constexpr int value = P::kStageEntries;
+};
 template <class P>
 struct CtxShape {
     static constexpr int kQ = P::kQueues;
@@ -118,7 +129,10 @@ struct CtxShape {
     static_assert(kFWr <= 12, "at most 12 frame words are loaded with a task (one 64-byte line)");
     static_assert(P::kFrameBytes >= 4 * kFWr, "the frame must cover the loaded frame words");
     static constexpr int kD = (3 + kFW) | 1;
-    static constexpr int kStage = NoTaskwait<P>::value ? kWave : 2 * kWave;  // evicted (+ suspended) records
+    static constexpr int kStage = StageEntries<P>::value;  // evicted (+ suspended) records
+    static_assert(kStage >= kWave, "the staging area holds at least one round's suspended tasks");
+    // evicted records staged per round; the rest of the area is for the suspended tasks
+    static constexpr uint32_t kEvictStage = (uint32_t)kStage - (NoTaskwait<P>::value ? 0u : (uint32_t)kWave);
     using type = WaveCtx<kQ, (kQL > 0 ? kQL : 1), kD, 4 * kPieces, kStage>;
 };
 template <class P>
@@ -591,7 +605,7 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w
                         const uint32_t base = uniform(vld(&w->nstage));
                         const uint32_t at = base + (uint32_t)rank_in(om);
                         const uint32_t* d = &w->desc[qq < QL ? qq : 0][slot][0];
-                        if (base + (uint32_t)popc(om) <= (uint32_t)kWave) {  // + <= 64 suspended after the round
+                        if (base + (uint32_t)popc(om) <= CtxShape<P>::kEvictStage) {  // + <= 64 suspended after the round
                             stage_from_desc<P>(w, at, oid, d);
                         } else {
                             record_from_desc<P>(dev, rs, oid, d);
@@ -601,7 +615,7 @@ __device__ __forceinline__ void queue_append(const GtapDev& dev, WaveCtxOf<P>* w
                 }
                 if (om && r == 0) {
                     const uint32_t base = vld(&w->nstage);
-                    if (base + (uint32_t)popc(om) <= (uint32_t)kWave) vst(&w->nstage, base + (uint32_t)popc(om));
+                    if (base + (uint32_t)popc(om) <= CtxShape<P>::kEvictStage) vst(&w->nstage, base + (uint32_t)popc(om));
                     vst(&w->overflowed, vld(&w->overflowed) + (uint32_t)popc(om));
                 }
                 vst(&w->mirror[qq][slot], id);

@@ -10,7 +10,7 @@
 //   sort_root(arr, tmp, n)       lane  spawns msort on queue 1, taskwait; spawns csum over
 //                                      chunks of the sorted keys, taskwait; returns their sum
 //   msort(a, t, n, into_t)       wave  2-way mergesort of examples/mergesort/mergesort.cu:34-50:
-//                                      n <= 1024 -> the wave sorts in LDS; else two msort
+//                                      n <= 512 -> the wave sorts in LDS; else two msort
 //                                      children (ping-pong buffers), taskwait, then merge
 //   merge(a, na, b, nb, dst)     wave  cilkmerge split of gtap_cilksort.cu:89-127 down to
 //                                      4096 keys, merged by the wave with merge-path tiles
@@ -36,7 +36,7 @@ namespace stream {
 
 enum : uint32_t { F_FIB = 1, F_NQ = 2, F_SROOT = 3, F_MSORT = 4, F_MERGE = 5, F_CSUM = 6 };
 enum : uint32_t { Q_LANE = 0, Q_WAVE = 1 };
-constexpr int kWaveLeaf = 1024;       // keys sorted by one wave in LDS
+constexpr int kWaveLeaf = 512;        // keys sorted by one wave in LDS
 constexpr int kWaveMergeLeaf = 4096;  // default: keys merged by one wave (the tile loop takes up to 16384)
 constexpr int kMaxWaveMergeLeaf = 16384;
 
@@ -90,19 +90,24 @@ __host__ __device__ __forceinline__ uint64_t sort_mix(uint64_t pos, uint32_t k) 
 }
 
 struct StreamArgs {
-    int32_t merge_leaf;  // merges of at most this many keys run on one wave (1024..16384)
+    int32_t merge_leaf;  // merges of at most this many keys run on one wave (512..16384)
 };
 
 struct Program {
     static constexpr int kQueues = 2;
     static constexpr int kWaveQueue = Q_WAVE;
+    // 3 waves per SIMD (3072 workers): the sort leaf of 512 keys and a 64-entry staging area
+    // keep a workgroup's LDS at 50 KB (3 per CU); at 2 waves per SIMD the batched stream took
+    // 81.6 ms against 65.2 ms
+    static constexpr int kWavesPerSimd = 3;
+    static constexpr int kStageEntries = 64;
     static constexpr int kFrameWords = 10;
     static constexpr uint32_t kMaxFunc = F_CSUM;
     static constexpr uint32_t kFrameBytes = 48;  // the largest arguments (merge: 40 bytes) and the dummy slot
     using Args = StreamArgs;
     struct WaveShared {
         uint32_t keys[kWaveLeaf];
-        int32_t split[18];
+        int32_t split[kMaxWaveMergeLeaf / kWaveLeaf + 2];
     };
     template <class L>
     __device__ __forceinline__ static void run(L& lane);
@@ -222,7 +227,7 @@ __device__ __forceinline__ void Program::run(L& lane) {
 __device__ __forceinline__ uint32_t ldk(const uint32_t* p) { return gld32(p); }
 __device__ __forceinline__ void stk(uint32_t* p, uint32_t v) { gst32(p, v); }
 
-// Sort src[0..n) (n <= 1024) into dst with a bitonic network over 1024 LDS slots.
+// Sort src[0..n) (n <= kWaveLeaf) into dst with a bitonic network over kWaveLeaf LDS slots.
 template <class W>
 __device__ void wave_leaf_sort(W& wt, const uint32_t* src, uint32_t* dst, int64_t n) {
     uint32_t* k = wt.sh->keys;
@@ -266,8 +271,8 @@ __device__ __forceinline__ int32_t merge_path32(LDA lda, int32_t na, LDB ldb, in
     return lo;
 }
 
-// Merge a[0..na) and b[0..nb) into dst (na + nb <= kMaxWaveMergeLeaf) by tiles of 1024
-// outputs, 16 per lane, staged through the wave's LDS tile.
+// Merge a[0..na) and b[0..nb) into dst (na + nb <= kMaxWaveMergeLeaf) by tiles of kWaveLeaf
+// outputs, kWaveLeaf / 64 per lane, staged through the wave's LDS tile.
 template <class W>
 __device__ void wave_leaf_merge(W& wt, const uint32_t* a, int32_t na, const uint32_t* b, int32_t nb, uint32_t* dst) {
     typename Program::WaveShared* sh = wt.sh;
@@ -276,8 +281,8 @@ __device__ void wave_leaf_merge(W& wt, const uint32_t* a, int32_t na, const uint
     auto ga = [a](int32_t i) { return ldk(a + i); };
     auto gb = [b](int32_t i) { return ldk(b + i); };
     constexpr int32_t kTileOut = kWaveLeaf;
-    constexpr int32_t kPer = kTileOut / kWave;  // 16
-    const int32_t ntiles = (total + kTileOut - 1) / kTileOut;  // <= 16
+    constexpr int32_t kPer = kTileOut / kWave;
+    const int32_t ntiles = (total + kTileOut - 1) / kTileOut;  // <= kMaxWaveMergeLeaf / kTileOut
     static_assert(kMaxWaveMergeLeaf / kTileOut + 1 <= (int)(sizeof(sh->split) / sizeof(sh->split[0])), "split table");
     if (l <= ntiles) {
         const int32_t d = (l * kTileOut < total) ? l * kTileOut : total;
@@ -459,7 +464,7 @@ uint64_t splitmix64_next(uint64_t& s) {
 // Task ids a job may hold at once, from its kind and size: at most the job's whole task
 // count (a record is live from its spawn to its completion). fib(n) runs 2 F(n+1) - 1 tasks;
 // n-queens(N) one per partial placement (counted here once per N); a sort job's msort tree
-// (leaves of 1024 keys), its merge tasks (split down to merge_leaf keys, at most about four per
+// (leaves of kWaveLeaf keys), its merge tasks (split down to merge_leaf keys, at most about four per
 // merge_leaf outputs per level), its checksum tasks and its root.
 uint64_t fib_tasks_host(int n) {
     uint64_t a = 0, b = 1;  // F(0), F(1)
@@ -652,7 +657,7 @@ extern "C" int gtap_stream_info(gtap_stream* S, int32_t* kind, int32_t* param, u
 // graph and re-derives each sort job's reservation.
 extern "C" int gtap_stream_set_merge_leaf(gtap_stream* S, int32_t merge_leaf) {
     if (!S || merge_leaf < stream::kWaveLeaf || merge_leaf > stream::kMaxWaveMergeLeaf)
-        return host::invalid_argument("gtap_stream_set_merge_leaf: merge_leaf must be in [1024, 16384]");
+        return host::invalid_argument("gtap_stream_set_merge_leaf: merge_leaf must be in [512, 16384]");
     if (merge_leaf == S->merge_leaf) return 0;
     host::destroy_graph(S->graph);
     S->merge_leaf = merge_leaf;

@@ -193,7 +193,7 @@ int gtap_stream_serve(gtap_stream* stream, double rate, uint64_t arrival_seed, i
 /* Device time from admission to completion of each job of the last run (any of the three
  * forms above), in ms: the execution part of latency_ms. */
 int gtap_stream_exec_ms(gtap_stream* stream, double* exec_ms);
-/* Sort jobs merge runs of at most merge_leaf keys (1024..16384, default 4096) on one wave and
+/* Sort jobs merge runs of at most merge_leaf keys (512..16384, default 4096) on one wave and
  * split larger merges into merge tasks; the counterpart of TASK_SPAWN_CUTOFF_MERGE
  * (evaluation/2-comparison/cilksort/gtap_cilksort.cu:12, 112), as a run-time setting for the
  * autotuner. Results do not depend on it. */

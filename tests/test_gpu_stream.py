@@ -141,11 +141,11 @@ def test_stream_graph_recaptured_after_config_change():
 def test_stream_merge_leaf_setting():
     """The sort jobs' one-wave merge size is a run-time setting: every allowed value gives the
     oracle's results in the batched graph (re-captured per value) and served; values outside
-    [1024, 16384] are refused and leave the setting unchanged."""
+    [512, 16384] are refused and leave the setting unchanged."""
     st = gtap_amd.Stream(seed=31, njobs=120)
     try:
         ref = None
-        for leaf in (1024, 2048, 16384, 4096):
+        for leaf in (512, 1024, 2048, 16384, 4096):
             print(f"case stream merge_leaf={leaf} batched", flush=True)
             st.set_merge_leaf(leaf)
             res, _, stats = st.run_batched()
@@ -157,7 +157,7 @@ def test_stream_merge_leaf_setting():
         st.set_merge_leaf(16384)
         res_s, _, _, _ = st.serve(rate=5000.0, arrival_seed=2)
         check_stream(st, res_s, sort_jobs=8)
-        for bad in (0, 1023, 16385, -5):
+        for bad in (0, 511, 16385, -5):
             with pytest.raises(gtap_amd.GtapError):
                 st.set_merge_leaf(bad)
         res2, _, _ = st.run_batched()  # still 16384
