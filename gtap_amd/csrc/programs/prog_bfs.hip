@@ -124,10 +124,10 @@ __device__ __forceinline__ void bfs_task(L& lane) {
     // Edges in chunks of kChunkEdges: the column loads, the depth probes and the atomicMins of
     // a chunk are issued back to back, so their latencies overlap instead of adding up edge
     // by edge. A plain load of depth[u] filters the atomic: depths only decrease, so a stale
-    // cached value is never below the true one and can only cost an unneeded atomic. With
-    // one queue a chunk is 16 edges (fewer dependent chains per task); the 4-queue kernel,
-    // which also holds the wave scan, keeps 8 (16 measured 3 % slower there, from spills).
-    constexpr int kChunkEdges = NQ == 1 ? 16 : 8;
+    // cached value is never below the true one and can only cost an unneeded atomic. A chunk
+    // is 16 edges: fewer dependent chains per task than 8 (1.33 -> 1.29 ms with one queue,
+    // 1.43 -> 1.40 ms with four).
+    constexpr int kChunkEdges = 16;
     const int32_t nd = d + 1;
     for (int32_t b = s; b < e; b += kChunkEdges) {
         int32_t u[kChunkEdges];
