@@ -92,7 +92,7 @@ struct WaveCtx {
     // of one record leave in one store instruction and merge into one memory transaction
     uint32_t stage[NS][S];
     uint32_t stage_id[NS];
-    uint32_t nstage;        // records staged so far this round (entries pushed out of the window, <= 64)
+    uint32_t nstage;        // records staged so far this round (entries pushed out of the window, <= kEvictStage)
 };
 
 // A program whose tasks never wait for children (P::kNoTaskwait) never suspends a task, so
