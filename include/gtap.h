@@ -141,8 +141,9 @@ int gtap_nqueens_slots(int n, int64_t* result, gtap_run_stats* stats);
 
 /* Asynchronous BFS from `source` over a CSR graph in device memory (row: nv+1 int32,
  * col: ne int32; depth: nv int32 out, 0x3f3f3f3f = unreached). num_queues 1 = one queue,
- * 4 = EPAQ by out-degree bucket (<8, <64, <512, >=512); `split` = edges per range task
- * (0 -> 64). Replaces examples/asynchronous_bfs/asynchronous_bfs_thread.cu:19-141 and
+ * 4 = EPAQ by out-degree bucket (<8, <64, <512, >=512), the last bucket's ranges scanned by a
+ * whole wavefront each; `split` = most edges per lane range task (0 -> 64; longer ranges are
+ * cut into up to 8 pieces). Replaces examples/asynchronous_bfs/asynchronous_bfs_thread.cu:19-141 and
  * evaluation/2-comparison/bfs/gtap_thread_bfs.cu:21-60. */
 int gtap_bfs(const int32_t* row, const int32_t* col, int32_t nv, int64_t ne, int32_t source, int32_t* depth,
              int32_t num_queues, int32_t split, gtap_run_stats* stats);
