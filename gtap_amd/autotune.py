@@ -180,7 +180,7 @@ def workload(name: str, reps: int = 3) -> Tuple[Measure, Dict[str, Sequence[int]
         row, col = gtap_amd.rmat_csr(22, 16, 1)
         queues = 4 if name == "bfs" else 1
         space = dict(DEFAULT_SPACE)
-        space["split"] = (64, 32, 128, 256)
+        space["split"] = (64, 16, 32, 128, 256)
         if queues > 1:
             space["epaq_publish"] = (0, 1)
 
