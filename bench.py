@@ -357,20 +357,27 @@ def bench_epaq_fib(gtap_amd, n: int = 40, cutoffs=(13, 16), reps: int = 5) -> di
 
 def bench_lowered_fib(gtap_amd, n: int = 30, reps: int = 5) -> dict:
     """The lowering's own output against the hand-lowered program: fib(n) from
-    gtap_amd/lower/examples/fib.gtap.cu (written in the reference's dialect, lowered by
-    gtap_amd.lower, launched by the host as fib_kernel<<<grid, 256>>>) and gtap_fib(n), both
-    with no cutoff; hipEvent time of the launch, median of `reps`."""
+    gtap_amd/lower/examples/fib.gtap.cu (written in the reference's dialect with a 64-bit
+    result, lowered by gtap_amd.lower to result slots, launched by the host as
+    fib_kernel<<<grid, 256>>>), from fib32.gtap.cu (the 32-bit result of the reference's
+    examples/fib/fib.cu, which the lowering joins additively) and gtap_fib(n), all with no
+    cutoff; hipEvent time of the launch, median of `reps`."""
     import ctypes
     from gtap_amd import build as gbuild
-    m = gtap_amd.EntryModule(gbuild.lowered_lib("fib"), "fib")
-    ll = ctypes.c_longlong
-    lowered = []
-    for _ in range(reps + 1):
-        m.set_symbol("d_fib_result", -1, ll)
-        st, t = m.launch("fib_kernel", [ctypes.c_int], n, timed=True)
-        if st != 0 or m.get_symbol("d_fib_result", ll) != FIB30:
-            raise RuntimeError("lowered fib: wrong result")
-        lowered.append(t)
+
+    def run_lowered(name, kernel, sym, ctype):
+        m = gtap_amd.EntryModule(gbuild.lowered_lib(name), name)
+        ms = []
+        for _ in range(reps + 1):
+            m.set_symbol(sym, -1, ctype)
+            st, t = m.launch(kernel, [ctypes.c_int], n, timed=True)
+            if st != 0 or m.get_symbol(sym, ctype) != FIB30:
+                raise RuntimeError(f"lowered {name}: wrong result")
+            ms.append(t)
+        return ms
+
+    lowered = run_lowered("fib", "fib_kernel", "d_fib_result", ctypes.c_longlong)
+    lowered32 = run_lowered("fib32", "fib32_kernel", "d_fib32_result", ctypes.c_int)
     builtin = []
     for _ in range(reps + 1):
         v, s = gtap_amd.fib(n)
@@ -378,9 +385,12 @@ def bench_lowered_fib(gtap_amd, n: int = 30, reps: int = 5) -> dict:
             raise RuntimeError("gtap_fib: wrong result")
         builtin.append(s["kernel_ms"])
     out = {"n": n, "lowered_host_launch_ms": statistics.median(lowered[1:]),
+           "lowered_int32_additive_join_ms": statistics.median(lowered32[1:]),
            "hand_lowered_ms": statistics.median(builtin[1:])}
     out["lowered_over_hand"] = out["lowered_host_launch_ms"] / out["hand_lowered_ms"]
-    log(f"fib({n}): lowered source, host launch {out['lowered_host_launch_ms']:.3f} ms; hand-lowered "
+    out["lowered_int32_over_hand"] = out["lowered_int32_additive_join_ms"] / out["hand_lowered_ms"]
+    log(f"fib({n}): lowered source, host launch {out['lowered_host_launch_ms']:.3f} ms (result slots), "
+        f"{out['lowered_int32_additive_join_ms']:.3f} ms (32-bit, additive join); hand-lowered "
         f"{out['hand_lowered_ms']:.3f} ms")
     return out
 

@@ -26,7 +26,9 @@ Input: a CUDA/HIP source in the reference's dialect, unmodified --
     }
 
 plus the compile-time switches the reference takes on its command line (`-D
-GTAP_NUM_QUEUES=3`, `-D GTAP_TERMINATE_ON_FIRST_TASK_FINISH`, program macros).
+GTAP_NUM_QUEUES=3`, `-D GTAP_TERMINATE_ON_FIRST_TASK_FINISH`, program macros), and
+`GTAP_WAVES_PER_SIMD=N`, which asks for at least N waves per SIMD in the scheduler kernels (the
+compiler keeps their registers within that occupancy, spilling if it must).
 
 Output: one .hip file that keeps the source's device side and its host-launch convention:
   * the source's preprocessor definitions, types, file-scope `__device__` / `__constant__`
@@ -164,6 +166,11 @@ COMPOUND_ASSIGN = 115
 ASM_STMTS = (215, 216)
 ANNOTATE_ATTR = 406
 EXPR_KINDS = range(100, 200)
+# statements that make what they contain conditional or repeated: label, case, default, if,
+# switch, while, do, for, range-for
+CONTROL_KINDS = (201, 203, 204, 205, 206, 207, 208, 209, 225)
+# 4-byte integer types whose sums the additive join carries exactly
+INT32_TYPES = {"int", "signed", "signed int", "unsigned", "unsigned int", "int32_t", "uint32_t"}
 
 
 class LoweringError(Exception):
@@ -208,6 +215,7 @@ class TaskFn:
     func_id: int = 0
     taskwaits: List[int] = field(default_factory=list)  # pragma offsets
     lds_locals: set = field(default_factory=set)        # offsets of __shared__ local declarations
+    sum_vars: Optional[set] = None                      # result locals carried by the additive join
 
 
 def _stmt_end(text: str, pos: int) -> int:
@@ -297,6 +305,16 @@ class _Lowerer:
         if not 1 <= self.num_queues <= 8:
             raise LoweringError("GTAP_NUM_QUEUES must be in [1, 8]")
         self.first_task_termination = "GTAP_TERMINATE_ON_FIRST_TASK_FINISH" in alldefs
+        # minimum waves per SIMD of the scheduler kernels (the compiler keeps registers within
+        # it, spilling if it must); absent: the compiler's choice
+        self.waves_per_simd: Optional[int] = None
+        if "GTAP_WAVES_PER_SIMD" in alldefs:
+            try:
+                self.waves_per_simd = int(alldefs["GTAP_WAVES_PER_SIMD"])
+            except ValueError:
+                raise LoweringError(f"GTAP_WAVES_PER_SIMD must be an integer ({alldefs['GTAP_WAVES_PER_SIMD']!r})")
+            if not 1 <= self.waves_per_simd <= 8:
+                raise LoweringError("GTAP_WAVES_PER_SIMD must be in [1, 8]")
         if mode is None:
             mode = "block" if re.search(r"#\s*include\s*[<\"][^>\"]*gtap_block\.cuh", text) else "thread"
         if mode not in ("thread", "block"):
@@ -542,7 +560,80 @@ class _Lowerer:
             reps += self._wrap_reps(body, n, reps)
         return reps
 
+    @staticmethod
+    def _int32(type_str: str, size: int) -> bool:
+        return size == 4 and type_str.replace("const ", "").strip() in INT32_TYPES
+
+    def _plus_leaves(self, e: cl.Node) -> Optional[List[cl.Node]]:
+        """The operands of a tree of `+` over variable references, or None."""
+        e = _strip(e)
+        if e.kind == cl.BINARY_OPERATOR and e.spelling == "+" and len(e.children) == 2:
+            a, b = self._plus_leaves(e.children[0]), self._plus_leaves(e.children[1])
+            return None if a is None or b is None else a + b
+        return [e] if e.kind == cl.DECL_REF_EXPR else None
+
+    def _sum_join_vars(self, t: TaskFn) -> Optional[set]:
+        """The result locals of t when its children's results are only summed: thread mode, one
+        taskwait, every task statement `x = f(...);` outside any condition or loop with x a
+        distinct 4-byte integer local and f returning a 4-byte integer, and every return after
+        the taskwait the sum of exactly those locals, which are used nowhere else. Such a task
+        joins additively (each child's result is added into the join word by the atomic that
+        counts the join down, with no result slot); the sum of up to 4095 32-bit values is
+        exact in the join word and converts to the return type as the C sum does. Any other
+        shape keeps result slots."""
+        if self.mode == "block" or len(t.taskwaits) != 1 or t.func_id > 127 or not self._int32(t.ret_type, t.ret_size):
+            return None
+        tw = t.taskwaits[0]
+        nodes = list(t.body.walk())
+        controls = [n for n in nodes if n.kind in CONTROL_KINDS]
+        if any(c.start <= tw < c.end for c in controls):
+            return None
+        task_pragmas = [p for p in self.pragmas if p.kind == "task" and t.body.start <= p.start < t.body.end]
+        if not task_pragmas or len(task_pragmas) > 4095:
+            return None
+        stmts: Dict[str, cl.Node] = {}
+        for p in task_pragmas:
+            if p.start > tw:
+                return None
+            cands = [n for n in nodes if n.start >= p.end and n.kind in (cl.BINARY_OPERATOR, cl.CALL_EXPR, cl.DECL_STMT)]
+            if not cands:
+                return None
+            first = min(cands, key=lambda n: (n.start, -n.end))
+            if first.kind != cl.BINARY_OPERATOR or first.spelling != "=" or len(first.children) != 2:
+                return None
+            if any(c.start <= first.start < c.end for c in controls):
+                return None
+            lhs, call = _strip(first.children[0]), first.children[1]
+            while call.kind != cl.CALL_EXPR and len(call.children) == 1:
+                call = call.children[0]
+            callee = self.tasks.get(call.spelling) if call.kind == cl.CALL_EXPR else None
+            v = self._var_for_ref(t, lhs) if lhs.kind == cl.DECL_REF_EXPR else None
+            if (callee is None or v is None or v.is_param or v.array_len >= 0 or v.name in stmts
+                    or not self._int32(v.type, v.size) or not self._int32(callee.ret_type, callee.ret_size)):
+                return None
+            stmts[v.name] = first
+        uses = set()
+        returns = [n for n in nodes if n.kind == cl.RETURN_STMT and n.start > tw]
+        if not returns:
+            return None
+        for r in returns:
+            leaves = self._plus_leaves(r.children[-1]) if r.children else None
+            if leaves is None:
+                return None
+            named = [self._var_for_ref(t, x) for x in leaves]
+            if any(x is None for x in named) or sorted(x.name for x in named) != sorted(stmts):
+                return None
+            uses.update(x.start for x in leaves)
+        for n in nodes:
+            if n.kind == cl.DECL_REF_EXPR and n.start not in uses:
+                v = self._var_for_ref(t, n)
+                if v is not None and v.name in stmts and not any(
+                        st.children[0].start <= n.start < st.children[0].end for st in stmts.values()):
+                    return None
+        return set(stmts)
+
     def lower_task(self, t: TaskFn) -> str:
+        t.sum_vars = self._sum_join_vars(t)
         leaf = self._leaf_reps(t)
         stmt: List[Tuple[int, int, str]] = []
         body_pragmas = [p for p in self.pragmas if t.body.start <= p.start < t.body.end]
@@ -577,7 +668,9 @@ class _Lowerer:
         for n in nodes:
             if n.kind == cl.RETURN_STMT:
                 end = _stmt_end(self.text, n.end)
-                if n.children:
+                if n.children and t.sum_vars is not None and n.start > t.taskwaits[0]:
+                    code = f"{{ __gtap_return(lane, ({t.ret_type})(__gtap_get_join_sum(lane))); return; }}"
+                elif n.children:
                     e = n.children[-1]
                     val = self._apply(e.start, e.end, leaf)
                     code = f"{{ __gtap_return(lane, ({t.ret_type})({val})); return; }}"
@@ -656,6 +749,9 @@ __device__ __forceinline__ void {t.name}__task(L& lane) {{
             code = (f"{{ const {callee.name}__args __a{{{arg_txt}}}; "
                     f"__gtap_spawn_task_detached(lane, F_{callee.name}, (uint32_t)({q}), __a); }}")
             return (p.start, end, code)
+        if t.sum_vars is not None:
+            return (p.start, end, f"{{ const {callee.name}__args __a{{{arg_txt}}}; "
+                                  f"__gtap_spawn_task_sum(lane, F_{callee.name}, (uint32_t)({q}), __a); }}")
         if lhs_node is None:
             off = f"offsetof({t.name}__frame, __discard)"
         else:
@@ -703,7 +799,8 @@ __device__ __forceinline__ void {t.name}__task(L& lane) {{
 
     def _taskwait_code(self, t: TaskFn, state: int, queue: str, leaf) -> str:
         q = self._apply_text_queue(queue, t)
-        return (f"if (__gtap_set_state_for_join(lane, {state}u, (uint32_t)({q}))) {{ {t.name}__save(lane, fr); return; }} "
+        join = "__gtap_set_state_for_join_sum" if t.sum_vars is not None else "__gtap_set_state_for_join"
+        return (f"if ({join}(lane, {state}u, (uint32_t)({q}))) {{ {t.name}__save(lane, fr); return; }} "
                 f"if (0) {{ case {state}: {t.name}__load(lane, fr); }}")
 
     # ------------------------------------------------------------------ emission
@@ -738,6 +835,8 @@ static_assert(sizeof({t.name}__args) % 8 == 0, "argument blocks are 8-byte padde
         sv, ld = [], []
         for v in t.params + t.locals:
             off = f"offsetof({t.name}__frame, {v.name})"
+            if t.sum_vars is not None and v.name in t.sum_vars:
+                continue  # carried by the additive join, never read
             if v.is_param:
                 if v.name in assigned:
                     sv.append(f"    lane.template frame_st<{v.type}>({off}, fr.{v.name});")
@@ -861,8 +960,9 @@ __device__ __forceinline__ void {t.name}__load(L& lane, {t.name}__frame& fr) {{
             body_txt = self._apply(body.start, body.end, reps)
             params = [c for c in k.children if c.kind == cl.PARM_DECL]
             ptxt = ", ".join(self.text[c.start:c.end] for c in params)
+            occ = f" __attribute__((amdgpu_waves_per_eu({self.waves_per_simd})))" if self.waves_per_simd else ""
             kern_txt.append(f"// {k.spelling}: launched by the host with <<<GTAP_GRID_SIZE, GTAP_BLOCK_SIZE>>>\n"
-                            f"__global__ __launch_bounds__(GTAP_BLOCK_SIZE) void {k.spelling}({ptxt}) {body_txt}\n")
+                            f"__global__ __launch_bounds__(GTAP_BLOCK_SIZE){occ} void {k.spelling}({ptxt}) {body_txt}\n")
             # host helpers: the launch as a C function, and the library-launched form
             cparams = ", ".join(f"{c.type} {c.spelling}" for c in params)
             cnames = ", ".join(c.spelling for c in params)
@@ -918,7 +1018,8 @@ extern "C" int gtap_entry_{k.spelling}({cparams}{sep}{res}gtap_run_stats* stats)
         defines = "\n".join(f"#define {k} {v}".rstrip() for k, v in self.cli_defines.items())
         term = "GTAP_TERM_JOBS" if self.first_task_termination else "GTAP_TERM_QUIESCE"
         if self.mode == "block":
-            prog = f"""struct Program {{
+            prog = f"""struct Program {{{
+            chr(10) + f"    static constexpr int kWavesPerSimd = {self.waves_per_simd};" if self.waves_per_simd else ""}
     static constexpr int kFrameWords = (int)(kMaxOf({{{abytes}}}) / 4) < 2 ? 2 : (int)(kMaxOf({{{abytes}}}) / 4);
     static constexpr uint32_t kMaxFunc = {len(tasks)};
     static constexpr uint32_t kFrameBytes = (kMaxOf({{{fbytes}}}) + 15u) & ~15u;
@@ -934,7 +1035,8 @@ extern "C" int gtap_entry_{k.spelling}({cparams}{sep}{res}gtap_run_stats* stats)
 }};"""
         else:
             prog = f"""struct Program {{
-    static constexpr int kQueues = {self.num_queues};
+    static constexpr int kQueues = {self.num_queues};{
+            chr(10) + f"    static constexpr int kWavesPerSimd = {self.waves_per_simd};" if self.waves_per_simd else ""}
     static constexpr int kFrameWords = (int)(kMaxOf({{{abytes}}}) / 4) < 2 ? 2 : (int)(kMaxOf({{{abytes}}}) / 4);
     static constexpr uint32_t kMaxFunc = {len(tasks)};
     static constexpr uint32_t kFrameBytes = (kMaxOf({{{fbytes}}}) + 15u) & ~15u;{

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module", autouse=True)
 def runtime():
-    need = max(gtap_amd.lowered(n)[f"gtap_lowered_{n}_record_bytes"]() for n in ("fib", "nqueens", "msort"))
+    need = max(gtap_amd.lowered(n)[f"gtap_lowered_{n}_record_bytes"]() for n in ("fib", "fib32", "nqueens", "msort"))
     stride = (need + 63) // 64 * 64
     gtap_amd.gtap_set_config(timeout_ms=15000, record_stride=stride)
     assert gtap_amd.gtap_initialize() == 0
@@ -35,6 +35,19 @@ def test_lowered_fib(n):
     f = entry(gtap_amd.lowered("fib"), "gtap_entry_fib_kernel",
               [ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), ctypes.c_void_p])
     r = ctypes.c_longlong(-1)
+    s = gtap_amd.RunStats()
+    assert f(n, ctypes.byref(r), ctypes.addressof(s)) == 0
+    assert r.value == oracle.fib(n)
+    assert s.tasks_executed == (3 * oracle.fib(n + 1) - 2 if n >= 1 else 1)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 5, 10, 20, 27])
+def test_lowered_fib32_additive_join(n):
+    """The 32-bit fib (the reference's fib.cu shape) is lowered to the additive join."""
+    print(f"case lowered fib32({n})", flush=True)
+    f = entry(gtap_amd.lowered("fib32"), "gtap_entry_fib32_kernel",
+              [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p])
+    r = ctypes.c_int(-1)
     s = gtap_amd.RunStats()
     assert f(n, ctypes.byref(r), ctypes.addressof(s)) == 0
     assert r.value == oracle.fib(n)

@@ -33,6 +33,67 @@ def test_fib_lowering_structure():
     assert "long long fib__seq(int n)" in out
 
 
+def test_int_sum_of_children_joins_additively():
+    """A task whose 32-bit children's results are only summed after one taskwait (the
+    reference's fib.cu) is lowered to the additive join: no result slots, the sum read from
+    the join word."""
+    out = lower.lower(read("fib32.gtap.cu"), "fib32.gtap.cu")
+    assert out.count("__gtap_spawn_task_sum(lane, F_fib") == 2
+    assert "__gtap_spawn_task(lane" not in out
+    assert "__gtap_set_state_for_join_sum(lane, 1u" in out
+    assert "__gtap_return(lane, (int)(__gtap_get_join_sum(lane)))" in out
+    load = out[out.index("void fib__load"):out.index("constexpr uint32_t kMaxOf")]
+    assert "frame_ld" not in load  # nothing to read back after the join
+    assert "__gtap_return(lane, (int)(fr.n))" in out  # the early return is unchanged
+
+
+SUM_SHAPE = """#include "gtap_thread.cuh"
+#pragma gtap function
+__device__ RET f(int n) {
+    if (n < 2) return n;
+    DECL;
+    BODY
+}
+__global__ void k(int n) {
+    #pragma gtap entry
+    f(n);
+}
+"""
+
+KEEPS_SLOTS = {
+    "64-bit result": ("long long", "long long a, b",
+                      "#pragma gtap task\n a = f(n - 1);\n #pragma gtap task\n b = f(n - 2);\n"
+                      " #pragma gtap taskwait\n return a + b;"),
+    "task under a condition": ("int", "int a = 0, b = 0",
+                               "#pragma gtap task\n a = f(n - 1);\n if (n > 4) {\n #pragma gtap task\n b = f(n - 2);\n }\n"
+                               " #pragma gtap taskwait\n return a + b;"),
+    "other use of a result": ("int", "int a, b",
+                              "#pragma gtap task\n a = f(n - 1);\n #pragma gtap task\n b = f(n - 2);\n"
+                              " #pragma gtap taskwait\n if (a > 3) return a;\n return a + b;"),
+    "not a plain sum": ("int", "int a, b",
+                        "#pragma gtap task\n a = f(n - 1);\n #pragma gtap task\n b = f(n - 2);\n"
+                        " #pragma gtap taskwait\n return a + b + 1;"),
+    "product": ("int", "int a, b",
+                "#pragma gtap task\n a = f(n - 1);\n #pragma gtap task\n b = f(n - 2);\n"
+                " #pragma gtap taskwait\n return a * b;"),
+    "two taskwaits": ("int", "int a, b",
+                      "#pragma gtap task\n a = f(n - 1);\n #pragma gtap taskwait\n"
+                      " #pragma gtap task\n b = f(n - 2);\n #pragma gtap taskwait\n return a + b;"),
+    "task in a loop": ("int", "int a = 0, i",
+                       "for (i = 0; i < 2; ++i) {\n #pragma gtap task\n a = f(n - 1 - i);\n }\n"
+                       " #pragma gtap taskwait\n return a;"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(KEEPS_SLOTS))
+def test_other_shapes_keep_result_slots(case):
+    ret, decl, body = KEEPS_SLOTS[case]
+    src = SUM_SHAPE.replace("RET", ret).replace("DECL", decl).replace("BODY", body.replace("\\n", "\n"))
+    out = lower.lower(src, "shape.cu")
+    assert "__gtap_spawn_task_sum" not in out and "__gtap_get_join_sum" not in out
+    assert "__gtap_spawn_task(lane, F_f" in out
+
+
 def test_nqueens_array_result_slots():
     out = lower.lower(read("nqueens.gtap.cu"), "nqueens.gtap.cu")
     assert "offsetof(nqueens__frame, counts) + 8 * (fr.k++)" in out
@@ -187,6 +248,13 @@ def test_switches_from_the_command_line():
     assert "constexpr uint32_t kTermMode = GTAP_TERM_JOBS;" in out
     with pytest.raises(lower.LoweringError):
         lower.lower(src, "fib.gtap.cu", defines={"GTAP_BLOCK_SIZE": "32"})  # 32-lane warps do not exist here
+    assert "amdgpu_waves_per_eu" not in out and "kWavesPerSimd" not in out
+    out = lower.lower(src, "fib.gtap.cu", defines={"GTAP_WAVES_PER_SIMD": "4"})
+    assert "static constexpr int kWavesPerSimd = 4;" in out
+    assert "__attribute__((amdgpu_waves_per_eu(4))) void fib_kernel(" in out
+    for bad in ("0", "9", "four"):
+        with pytest.raises(lower.LoweringError):
+            lower.lower(src, "fib.gtap.cu", defines={"GTAP_WAVES_PER_SIMD": bad})
 
 
 def test_detached_spawns_without_taskwait():
