@@ -39,8 +39,10 @@ Output: one .hip file that keeps the source's device side and its host-launch co
     `fr` (protothread style). A spawn names the child's function id and the byte offset of
     its result slot in the parent's frame. A spawn that a later taskwait of the function
     joins is a joined child; one that no taskwait follows is detached (the reference's
-    semantics for children of a task that returns without a taskwait). `f__seq` is an
-    ordinary device function for direct calls;
+    semantics for children of a task that returns without a taskwait). A program with no
+    detached spawn ends on the job count (its root ends last), one with detached spawns on
+    quiescence, unless GTAP_TERMINATE_ON_FIRST_TASK_FINISH asks for the root's end. `f__seq`
+    is an ordinary device function for direct calls;
   * every `__global__` kernel with `#pragma gtap entry`, under its own name and parameters,
     launchable by the host as `kernel<<<GTAP_GRID_SIZE, GTAP_BLOCK_SIZE>>>(...)`: the
     kernel's own statements run as written, the entry call runs the persistent scheduler
@@ -305,6 +307,7 @@ class _Lowerer:
         if not 1 <= self.num_queues <= 8:
             raise LoweringError("GTAP_NUM_QUEUES must be in [1, 8]")
         self.first_task_termination = "GTAP_TERMINATE_ON_FIRST_TASK_FINISH" in alldefs
+        self.detached_spawns = 0
         # minimum waves per SIMD of the scheduler kernels (the compiler keeps registers within
         # it, spilling if it must); absent: the compiler's choice
         self.waves_per_simd: Optional[int] = None
@@ -746,6 +749,7 @@ __device__ __forceinline__ void {t.name}__task(L& lane) {{
             if lhs_node is not None:
                 raise LoweringError(f"{t.name}: the task at line {self.line(p.start)} assigns a result but no "
                                     "taskwait follows it")
+            self.detached_spawns += 1
             code = (f"{{ const {callee.name}__args __a{{{arg_txt}}}; "
                     f"__gtap_spawn_task_detached(lane, F_{callee.name}, (uint32_t)({q}), __a); }}")
             return (p.start, end, code)
@@ -1016,7 +1020,11 @@ extern "C" int gtap_entry_{k.spelling}({cparams}{sep}{res}gtap_run_stats* stats)
         kernels, host = self.entry_kernels()
         no_taskwait = not any(t.taskwaits for t in tasks)
         defines = "\n".join(f"#define {k} {v}".rstrip() for k, v in self.cli_defines.items())
-        term = "GTAP_TERM_JOBS" if self.first_task_termination else "GTAP_TERM_QUIESCE"
+        # With every spawn joined, a task ends only after all of its descendants, so the root's
+        # end is the moment the last task finishes: the job count ends the launch without the
+        # idle-worker accounting that quiescence detection needs.
+        term = ("GTAP_TERM_JOBS" if self.first_task_termination or self.detached_spawns == 0
+                else "GTAP_TERM_QUIESCE")
         if self.mode == "block":
             prog = f"""struct Program {{{
             chr(10) + f"    static constexpr int kWavesPerSimd = {self.waves_per_simd};" if self.waves_per_simd else ""}

@@ -238,7 +238,8 @@ def test_device_variables_entry_kernel_and_symbols():
     assert "__gtap_set_state_for_join(lane, 1u, (uint32_t)(2))" in out
     assert 'if (strcmp(name, "d_cutoff") == 0)' in out
     assert 'extern "C" int gtap_launch_fib_kernel(uint32_t grid, uint32_t block, int n)' in out
-    assert "constexpr uint32_t kTermMode = GTAP_TERM_QUIESCE;" in out
+    # every spawn is joined, so the root's end is the last task's end and the job count ends it
+    assert "constexpr uint32_t kTermMode = GTAP_TERM_JOBS;" in out
 
 
 def test_switches_from_the_command_line():
@@ -264,6 +265,8 @@ def test_detached_spawns_without_taskwait():
     out = lower.lower(lower_src_nq(), "nq.cu")
     assert "__gtap_spawn_task_detached(lane, F_nq" in out and "__gtap_spawn_task(lane" not in out
     assert "static constexpr bool kNoTaskwait = true;" in out
+    # detached tasks may outlive the root: the launch ends when no task is left anywhere
+    assert "constexpr uint32_t kTermMode = GTAP_TERM_QUIESCE;" in out
 
 
 def lower_src_nq():
