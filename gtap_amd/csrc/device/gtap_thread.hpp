@@ -42,6 +42,7 @@ constexpr uint32_t kPrivCap = 8192;         // wave-private overflow stack in HB
 constexpr uint32_t kChunk = 64;             // ids per pool cell
 constexpr uint32_t kAnchorFunc = 0xFFFu;    // function id stored in anchor headers
 constexpr uint32_t kMargin = 1024;          // deque slots kept free beyond the tail
+constexpr uint32_t kIdleRegisterAfter = 4;  // failed acquire passes before a wave counts itself idle
 
 enum Outcome : uint32_t { OUT_RUNNING = 0, OUT_FINISHED = 1, OUT_SUSPENDED = 2 };
 
@@ -1671,8 +1672,12 @@ __device__ __forceinline__ void thread_scheduler(const GtapDev& dev, const typen
                 }
                 // ---- idle: termination, abort, watchdog, backoff. A wave counts itself idle
                 // only while no overflow ring holds tickets: a wave that handed tasks to a
-                // ring therefore sees them before it can count itself idle.
-                if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered && rings_seen_empty) {
+                // ring therefore sees them before it can count itself idle. It also waits
+                // for kIdleRegisterAfter failed passes, so a short spell between steals costs
+                // no atomics on the shared count. A wave not yet counted can only delay the
+                // end of the launch (the count must reach every started wave), never cause it.
+                if (dev.term_mode == GTAP_TERM_QUIESCE && !idle_registered && rings_seen_empty &&
+                    idle_iter >= kIdleRegisterAfter) {
                     idle_registered = true;
                     int32_t c = 0;
                     int32_t started = 0;
