@@ -14,7 +14,8 @@
 //          with the started count, job owners counted as started by the reset. Invariant:
 //          done is never set while a task is outstanding, also when the job owner starts
 //          after every other worker went idle; the former rule (owners count themselves when
-//          they start) is shown to end early in that schedule.
+//          they start) is shown to end early in that schedule. Also run with the idle count
+//          joined only after 4 consecutive failed acquire passes (kIdleRegisterAfter).
 //  ovf     the deque overflow rings (gtap_thread.hpp ovf_fill / ovf_pop / ovf_spill_deque,
 //          ovf_drain_kernel in gtap_host.hip): cells of state | position tag | id, a ticket
 //          fetch-add per batch, producers wait for the cell of the previous lap to be freed,
@@ -381,9 +382,11 @@ namespace quiesce_model {
 
 // Workers 0 .. owners-1 start one job each by index; a job is `work` tasks that its owner
 // runs and may publish to a shared pool from which idle workers take. `owner_delay` makes
-// the owners start late (not yet resident). Returns true if done was set while tasks were
-// still outstanding.
-bool run(unsigned seed, int workers, int owners, int work, bool count_owners_at_reset, int owner_delay_us) {
+// the owners start late (not yet resident); a worker joins the idle count only after
+// `register_after` consecutive failed acquire passes. Returns true if done was set while tasks
+// were still outstanding.
+bool run(unsigned seed, int workers, int owners, int work, bool count_owners_at_reset, int owner_delay_us,
+         int register_after = 0) {
     std::atomic<int> idle{0}, started{count_owners_at_reset ? owners : 0}, done{0};
     std::atomic<int> pool{0};         // published tasks
     std::atomic<int> outstanding{owners * work};
@@ -397,6 +400,7 @@ bool run(unsigned seed, int workers, int owners, int work, bool count_owners_at_
             if (!count_owners_at_reset || !owner) started.fetch_add(1);
             int mine = owner ? work : 0;  // tasks this worker holds (its job, then taken ones)
             bool registered = false;
+            int fails = 0;  // consecutive failed acquire passes
             for (long it = 0; it < 2000000; ++it) {
                 if (mine > 0) {
                     // run a task; sometimes publish half of the rest (the others may take them)
@@ -418,11 +422,12 @@ bool run(unsigned seed, int workers, int owners, int work, bool count_owners_at_
                     }
                     if (pool.compare_exchange_strong(p, p - 1, std::memory_order_acquire)) {
                         mine = 1;
+                        fails = 0;
                         continue;
                     }
                     continue;
                 }
-                if (!registered) {
+                if (!registered && fails++ >= register_after) {
                     registered = true;
                     const int c = idle.fetch_add(1) + 1;
                     if (c == started.load()) {
@@ -581,6 +586,10 @@ int main(int argc, char** argv) {
         CHECK(!early, "quiesce: done while tasks were outstanding (seed %u)", s);
         const bool early_late = quiesce_model::run(s, 6, 2, 2000, true, 20000);
         CHECK(!early_late, "quiesce: done while a late job owner's tasks were outstanding (seed %u)", s);
+        // the idle count joined after 4 failed passes, as thread_scheduler does
+        CHECK(!quiesce_model::run(s, 6, 1, 2000, true, 0, 4), "quiesce, delayed count: early end (seed %u)", s);
+        CHECK(!quiesce_model::run(s, 6, 2, 2000, true, 20000, 4),
+              "quiesce, delayed count: early end with a late job owner (seed %u)", s);
     }
     for (unsigned s = 1; s <= seeds; ++s) {
         ovf_model::run(s, 3, 3, 3000, 64);
